@@ -1,0 +1,131 @@
+"""Benchmark of the RDME step on one MI355X: voxel-steps/s on the 256^3 large-cell workload.
+
+Workload: cell_simulation_amd.configs.large_cell -- 256^3 lattice, 16 one-byte slots per voxel, 4
+regions, 20 species, 30 reactions, ~2M particles (seed 1234), dt = 10 us. Inputs are resident in HBM
+before the timed region. One "step" = one full RDME step (diffusion, reaction, compaction) of the whole
+lattice.
+
+Prints ONE JSON line. `roofline` prices the step kernel against HBM: algorithmic bytes per voxel-step
+= 2*S (read and write every slot) + 1 (region label) = 33 B at S=16, divided by the kernel's average
+duration measured with HIP events on the simulation's stream. `cpu_baseline` times the CPU oracle (the
+serial restatement of the reference shaders, one core) on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--edge", type=int, default=256)
+    ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps for the CPU baseline (0 = skip)")
+    ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "large_cell_256_traffic.json"),
+                    help="per-launch HBM bytes measured with rocprofv3 --pmc (FETCH_SIZE / WRITE_SIZE)")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on exactly one GPU: the RDME step is benchmarked per device and this build "
+              "has no multi-GPU path (slab decomposition is a design note only).", file=sys.stderr)
+        return 2
+
+    from cell_simulation_amd import configs, default_itime_schedule
+
+    log(f"building large_cell edge={args.edge}")
+    sim = configs.large_cell(seed=1234, edge=args.edge)
+    info = sim.info()
+    N, S = info["n_voxels"], info["max_particles_site"]
+    sim.prepare_for_gpu(0)
+    log("device ready")
+    its = default_itime_schedule(1234, args.warmup + args.steps)
+    if args.warmup:
+        sim.run(0, its[: args.warmup])
+    sim.synchronize()
+    sim.reset_counters()
+
+    # Timed region: K steps enqueued back to back, bracketed by stream synchronisation.
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.run(args.warmup, its[args.warmup:])
+    sim.synchronize()
+    wall = time.perf_counter() - t0
+    log(f"timed {args.steps} steps in {wall:.4f} s")
+    counters = sim.counters()
+
+    # Kernel duration with HIP events around each launch (separate pass over the same number of steps).
+    its2 = default_itime_schedule(4321, args.steps)
+    kernel_ms = sim.run_timed(its2)
+    avg_kernel_s = kernel_ms / 1e3 / args.steps
+
+    b_alg = 2 * S + 1
+    achieved_gbs = b_alg * N / avg_kernel_s / 1e9
+    traffic = None
+    if args.traffic_json and os.path.exists(args.traffic_json):
+        with open(args.traffic_json) as f:
+            tj = json.load(f)
+        if tj.get("edge") == args.edge:
+            traffic = tj.get("bytes_per_launch")
+
+    value = N * args.steps / wall
+    cpu = None
+    if args.cpu_steps > 0:
+        from oracle.cs_oracle import Oracle
+
+        log(f"cpu baseline: oracle, {args.cpu_steps} steps")
+        orc = Oracle.from_simulation(configs.large_cell(seed=1234, edge=args.edge))
+        ci = default_itime_schedule(1234, args.cpu_steps)
+        orc.step(int(ci[0]))  # warm-up step (page-in), not timed
+        t1 = time.perf_counter()
+        orc.run(ci[1:])
+        ct = time.perf_counter() - t1
+        n_cpu = args.cpu_steps - 1
+        cpu = {"value": N * n_cpu / ct if n_cpu else None, "unit": "voxel-steps/s", "cores": 1, "kind": "port",
+               "sample": f"{n_cpu} step(s) of the {args.edge}^3 S={S} large-cell workload after one warm-up step, "
+                         "serial C oracle (canonical order of the reference shaders)"}
+
+    out = {
+        "metric": "voxel-steps/s (RDME diff+react, 256^3)",
+        "value": value,
+        "unit": "voxel-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic",
+        "config": {"workload": f"large_cell {args.edge}^3 S={S}: 4 regions, 20 species, 30 reactions, "
+                               "~2M particles, dt=10us, seed 1234, plain one-launch-per-step path",
+                   "voxels": N, "slots_per_voxel": S, "path": "plain"},
+        "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "rdme_step_kernel", "bytes_per_voxel_step": b_alg,
+                     "avg_kernel_us": avg_kernel_s * 1e6},
+        "cpu_baseline": cpu,
+        "diagnostics": {"tight_voxel_steps": int(counters[0]), "exceptions": int(counters[1]),
+                        "near_threshold": int(counters[2]), "fired": int(counters[3])},
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

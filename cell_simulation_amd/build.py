@@ -1,0 +1,63 @@
+"""Builds libcellsim.so (the HIP path and its C ABI) in-tree for gfx950.
+
+Usage: python -m cell_simulation_amd.build   (or build() from Python)
+
+Device code is compiled with -ffp-contract=off and IEEE-rounded f32 division so every f32 operation of
+the step rounds exactly as the reference shaders specify; packed-FP32 instructions are disabled.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+CSRC = PKG / "csrc"
+OUT = PKG / "libcellsim.so"
+OBJ = PKG / "_obj"
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-function"]
+HIP_FLAGS = COMMON + [
+    f"--offload-arch={ARCH}",
+    "-fhip-fp32-correctly-rounded-divide-sqrt",
+    "-fno-gpu-flush-denormals-to-zero",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+CXX_FLAGS = COMMON
+
+
+def _run(cmd: list[str]) -> None:
+    proc = subprocess.run(cmd, capture_output=True, text=True)
+    if proc.returncode != 0:
+        sys.stderr.write(" ".join(cmd) + "\n" + proc.stdout + proc.stderr)
+        raise RuntimeError(f"build step failed: {cmd[-1]}")
+
+
+def build(verbose: bool = False) -> Path:
+    if shutil.which(HIPCC) is None and not Path(HIPCC).exists():
+        raise RuntimeError(f"hipcc not found at {HIPCC}")
+    OBJ.mkdir(exist_ok=True)
+    objs = []
+    for src in sorted(CSRC.glob("*.cpp")):
+        obj = OBJ / (src.stem + ".o")
+        _run([HIPCC, *CXX_FLAGS, "-x", "c++", "-c", str(src), "-o", str(obj)])
+        objs.append(str(obj))
+    for src in sorted(CSRC.glob("*.hip")):
+        obj = OBJ / (src.stem + ".hip.o")
+        _run([HIPCC, *HIP_FLAGS, "-c", str(src), "-o", str(obj)])
+        objs.append(str(obj))
+    tmp = OUT.with_suffix(".so.tmp")
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", str(tmp)])
+    os.replace(tmp, OUT)
+    if verbose:
+        print(f"built {OUT}")
+    return OUT
+
+
+if __name__ == "__main__":
+    build(verbose=True)

@@ -1,0 +1,62 @@
+// device.hpp -- device state of one simulation on one MI355X (gfx950) and the step launcher.
+//
+// Replaces the reference's GPU half: Simulation::prepare_for_gpu (src/simulation.rs:152-208),
+// RDME::step / CME::step dispatches (src/rdme.rs:46-76, src/cme.rs:47-79), the final_stage compaction
+// pass and the dest->src copies (src/simulation.rs:99-150), and the statistics readback
+// (src/simulation.rs:361-418).
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "model.hpp"
+
+namespace cs {
+
+constexpr int kNumCounters = 8;
+
+class Device {
+public:
+    Device(const Model &m, int hip_device);
+    ~Device();
+    Device(const Device &) = delete;
+    Device &operator=(const Device &) = delete;
+
+    // Enqueue one full step (diffusion, reaction, compaction) seeded with `itime`.
+    void step(uint32_t itime);
+    // Steps with an event pair around each kernel; returns the summed kernel milliseconds.
+    float run_timed(const uint32_t *itime, uint32_t nsteps);
+    void synchronize();
+
+    void read_lattice(uint8_t *out);
+    void write_lattice(const uint8_t *in);
+    // Species counts per region, int32[nreg][ns+1], reduced on the device.
+    void region_counts(int32_t *out);
+    void read_counters(uint64_t *out);
+    void reset_counters();
+
+    uint64_t n_voxels() const { return N_; }
+    uint32_t slots() const { return S_; }
+
+private:
+    int dev_ = 0;
+    void *stream_ = nullptr;       // hipStream_t
+    uint32_t res_[3] = {0, 0, 0};
+    uint64_t N_ = 0;
+    uint32_t S_ = 0, ns_ = 0, nr_ = 0, nreg_ = 0, nrw_ = 0;
+    float tau_ = 0.f;
+    uint32_t mod_zero_ = 0xFFFFFFFFu;
+    int cur_ = 0;                  // index of the buffer holding the current lattice
+
+    uint8_t *lat_[2] = {nullptr, nullptr};
+    uint8_t *reg_ = nullptr;
+    float *p2_ = nullptr;          // (D*tau)/(lambda*lambda), [src][dst][sp]
+    float *rates_ = nullptr;       // [nr+1]
+    uint8_t *ridx_ = nullptr;      // [nr+1][4]
+    int8_t *stoich_ = nullptr;     // [nr+1][ns+1]
+    uint64_t *rmask_ = nullptr;    // [ns+1][nrw]: reactions having the species as a reactant; row 0: zero-order
+    unsigned long long *counters_ = nullptr;
+    int32_t *hist_ = nullptr;      // [nreg][ns+1]
+};
+
+}  // namespace cs

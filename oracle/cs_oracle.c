@@ -1,0 +1,332 @@
+/*
+ * cs_oracle.c -- serial CPU restatement of cell_simulation's RDME step (TEST INFRASTRUCTURE ONLY).
+ *
+ * Parity checker for the HIP path; never linked into the product. Each function cites the reference
+ * lines it restates. Compile with -O2 -ffp-contract=off (no fast-math): every f32 operation must round
+ * exactly as written, in the order written.
+ */
+#include "cs_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* random.wgsl:14-20. The LCG line 17 is dead code in the reference; the output hashes the input. */
+uint32_t cso_pcg(uint32_t state)
+{
+    uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+    return (word >> 22u) ^ word;
+}
+
+/* random.wgsl:32-45 */
+uint32_t cso_xorshift(uint32_t s)
+{
+    s ^= s << 13u;
+    s ^= s >> 17u;
+    s ^= s << 5u;
+    return s;
+}
+
+/* random.wgsl:27-29: f32(u) * bitcast<f32>(0x2F800000) = f32(u) * 2^-32. */
+float cso_uint_to_float(uint32_t u)
+{
+    float f = (float)u;          /* round-to-nearest-even conversion */
+    return f * 2.3283064365386963e-10f;
+}
+
+/* random.wgsl:47-49 */
+float cso_uniform_float(uint32_t state)
+{
+    return cso_uint_to_float(cso_xorshift(state));
+}
+
+float cso_expf(float x)
+{
+    return (float)exp((double)x);
+}
+
+static inline uint64_t nvox(const cso_model *m)
+{
+    return (uint64_t)m->res[0] * m->res[1] * m->res[2];
+}
+
+/* functions.wgsl:48-51 */
+static inline uint32_t vidx(const cso_model *m, uint32_t x, uint32_t y, uint32_t z)
+{
+    return z + y * m->res[2] + x * m->res[2] * m->res[1];
+}
+
+/* rdme.wgsl:132-136: (D * tau) / (lambda * lambda), f32, D indexed [src][dst][sp]. */
+static inline float prob_value(const cso_model *m, uint32_t src_reg, uint32_t dst_reg, uint32_t sp)
+{
+    uint32_t n1 = m->ns + 1u;
+    float val = m->diffusion[sp + n1 * dst_reg + n1 * m->nreg * src_reg];
+    float num = val * m->tau;
+    float den = m->lambda * m->lambda;
+    return num / den;
+}
+
+/* rdme.wgsl:138-191: cumulative hop probabilities in the order -x,+x,-y,+y,-z,+z; a missing
+ * neighbour repeats the previous entry; p[6] = 1. */
+void cso_probability_vector(const cso_model *m, uint32_t x, uint32_t y, uint32_t z, uint32_t sp,
+                            float p[7])
+{
+    uint32_t src = m->regions[vidx(m, x, y, z)];
+    if (x > 0u)
+        p[0] = prob_value(m, src, m->regions[vidx(m, x - 1u, y, z)], sp);
+    else
+        p[0] = 0.0f;
+    if (x < m->res[0] - 1u)
+        p[1] = p[0] + prob_value(m, src, m->regions[vidx(m, x + 1u, y, z)], sp);
+    else
+        p[1] = p[0];
+    if (y > 0u)
+        p[2] = p[1] + prob_value(m, src, m->regions[vidx(m, x, y - 1u, z)], sp);
+    else
+        p[2] = p[1];
+    if (y < m->res[1] - 1u)
+        p[3] = p[2] + prob_value(m, src, m->regions[vidx(m, x, y + 1u, z)], sp);
+    else
+        p[3] = p[2];
+    if (z > 0u)
+        p[4] = p[3] + prob_value(m, src, m->regions[vidx(m, x, y, z - 1u)], sp);
+    else
+        p[4] = p[3];
+    if (z < m->res[2] - 1u)
+        p[5] = p[4] + prob_value(m, src, m->regions[vidx(m, x, y, z + 1u)], sp);
+    else
+        p[5] = p[4];
+    p[6] = 1.0f;
+}
+
+/* rdme.wgsl:25-42 writeLatticeSite: first-fit over slots 0..S INCLUSIVE of the destination voxel.
+ * Slot S is the next voxel's slot 0 (hazard H2). An index past the end of the lattice counts as an
+ * occupied slot (the reference's out-of-bounds atomic result is undefined). Returns the slot index
+ * written or -1. */
+static int write_lattice_site(const cso_model *m, uint32_t *lat, uint64_t base, uint32_t value)
+{
+    uint64_t total = nvox(m) * m->S;
+    for (uint32_t j = 0; j <= m->S; ++j) {
+        uint64_t idx = base + j;
+        if (idx < total && lat[idx] == 0u) {
+            lat[idx] = value;
+            return (int)j;
+        }
+    }
+    return -1;
+}
+
+/* rdme.wgsl:45-88 move_particle_site, executed without interleaving (canonical order). */
+static void move_particle_site(const cso_model *m, cso_state *st, uint32_t sp, uint32_t v_src,
+                               uint32_t v_dst, uint64_t i_part, cso_counters *cnt)
+{
+    uint32_t n1 = m->ns + 1u;
+    uint32_t old = st->occupancy[v_dst];
+    st->occupancy[v_dst] = old + 1u;                    /* atomicAdd, :61 */
+    if (old <= m->S) {                                  /* :62, hazard H3 */
+        int j = write_lattice_site(m, st->lattice, (uint64_t)v_dst * m->S, sp);
+        if (j >= 0) {
+            st->counts[(uint64_t)v_dst * n1 + sp] += 1;   /* :66 */
+            st->occupancy[v_src] -= 1u;                 /* :69 */
+            if (i_part < nvox(m) * m->S)
+                st->lattice[i_part] = 0u;               /* :70 */
+            st->counts[(uint64_t)v_src * n1 + sp] -= 1;   /* :71 */
+            if (cnt) {
+                cnt->moves++;
+                if ((uint32_t)j == m->S)
+                    cnt->alias_writes++;
+            }
+            return;
+        }
+    }
+    st->occupancy[v_dst] -= 1u;                         /* :75 / :80 */
+    if (cnt)
+        cnt->failed_claims++;
+}
+
+/* rdme.wgsl:194-229 (with move_particle :91-130), all voxels in ascending order. */
+void cso_phase_r(const cso_model *m, const uint32_t *lat_src, const uint32_t *occ_src,
+                 cso_state *st, uint32_t itime, cso_counters *cnt)
+{
+    const uint32_t R0 = m->res[0], R1 = m->res[1], R2 = m->res[2];
+    const uint64_t total = nvox(m) * m->S;
+    for (uint32_t x = 0; x < R0; ++x)
+        for (uint32_t y = 0; y < R1; ++y)
+            for (uint32_t z = 0; z < R2; ++z) {
+                uint32_t v = vidx(m, x, y, z);
+                uint32_t idx_lattice = v * m->S;             /* u32, functions.wgsl:42-46 */
+                uint32_t occ = occ_src[v];
+                uint32_t xmy = (y != 0u) ? (x % y) : m->mod_zero;  /* hazard H4 */
+                for (uint32_t i_part = idx_lattice; i_part < idx_lattice + occ; ++i_part) {
+                    uint32_t sp = (i_part < total) ? lat_src[i_part] : 0u;
+                    if (sp == 0u)
+                        continue;
+                    float p[7];
+                    cso_probability_vector(m, x, y, z, sp, p);
+                    uint32_t state = cso_pcg(itime + xmy + z + i_part);
+                    float r = cso_uniform_float(state);
+                    int i = 0;
+                    while (i < 6 && r > p[i])
+                        ++i;
+                    /* move_particle: reflecting boundary returns without moving; 6 = stay */
+                    uint32_t dx = x, dy = y, dz = z;
+                    int ok = 1;
+                    switch (i) {
+                    case 0: if (x == 0u) ok = 0; else dx = x - 1u; break;
+                    case 1: if (x == R0 - 1u) ok = 0; else dx = x + 1u; break;
+                    case 2: if (y == 0u) ok = 0; else dy = y - 1u; break;
+                    case 3: if (y == R1 - 1u) ok = 0; else dy = y + 1u; break;
+                    case 4: if (z == 0u) ok = 0; else dz = z - 1u; break;
+                    case 5: if (z == R2 - 1u) ok = 0; else dz = z + 1u; break;
+                    default: ok = 0; break;
+                    }
+                    if (!ok)
+                        continue;
+                    move_particle_site(m, st, sp, v, vidx(m, dx, dy, dz), i_part, cnt);
+                }
+            }
+}
+
+/* cme.wgsl:25-120 for one voxel. */
+static void cme_voxel(const cso_model *m, cso_state *st, uint32_t x, uint32_t y, uint32_t z,
+                      uint32_t itime, float *cumm, cso_counters *cnt)
+{
+    const uint32_t n1 = m->ns + 1u, S = m->S, nr = m->nr;
+    const uint32_t v = vidx(m, x, y, z);
+    int32_t *C = st->counts + (uint64_t)v * n1;
+    C[0] = 1;                                                       /* :31 */
+
+    cumm[0] = 0.0f;                                                 /* :36-47 */
+    float total = 0.0f;
+    for (uint32_t r = 1; r <= nr; ++r) {
+        float k = m->rates[r];
+        const int32_t *ix = m->reactants + 3u * r;
+        int32_t prod = (int32_t)((uint32_t)C[ix[0]] * (uint32_t)C[ix[1]] * (uint32_t)C[ix[2]]);
+        float a = k * (float)prod;
+        total = total + a;
+        cumm[r] = cumm[r - 1] + a;
+    }
+    for (uint32_t i = 0; i < nr; ++i)                               /* :50-52 */
+        cumm[i] = cumm[i] / total;
+
+    uint32_t *L = st->lattice + (uint64_t)v * S;
+    uint32_t cell_copy[16];                                         /* :56-59 */
+    for (uint32_t i = 0; i < S; ++i)
+        cell_copy[i] = L[i];
+
+    uint32_t state = cso_pcg(itime + x * y + z);                    /* :64-66 */
+    float r1 = cso_uniform_float(state);
+    float xt = total * m->tau;
+    float e = cso_expf(-xt);
+    float thr = 1.0f - e;
+    /* exp(-0) = 1 exactly, so only a non-zero exponent can leave the outcome to an exp ulp */
+    if (cnt && xt != 0.0f && fabsf(r1 - thr) <= 5.9604644775390625e-08f)
+        cnt->near_threshold++;
+    if (!(r1 <= thr))
+        return;
+    if (cnt)
+        cnt->fired++;
+
+    float r2 = cso_uniform_float(state + 1u);                       /* :68-73 */
+    uint32_t i = 0;
+    while ((r2 > cumm[i]) & (i < nr))
+        ++i;
+
+    uint32_t j = 0;                                                 /* :76-119 */
+    const int32_t *row = m->stoich + (uint64_t)i * n1;
+    int32_t new_occ = (int32_t)st->occupancy[v];
+    uint64_t res_idx = (uint64_t)v * S;                             /* hazard H6: lattice index */
+    uint32_t res_sp = (m->reservoir && res_idx < nvox(m)) ? m->reservoir[res_idx] : 0u;
+    for (uint32_t sp = 1; sp <= m->ns; ++sp) {
+        if (res_sp == sp)
+            continue;
+        C[sp] += row[sp];
+        new_occ += row[sp];
+        st->stat[sp] += row[sp];
+        int32_t cc = C[sp];
+        while (cc > 0) {
+            L[j] = sp;
+            cc -= 1;
+            j += 1;
+            if (j >= S) {                                           /* rollback, :95-108 */
+                for (uint32_t q = 0; q < S; ++q)
+                    L[q] = cell_copy[q];
+                for (uint32_t q = 1; q <= sp; ++q) {
+                    C[q] -= row[q];
+                    st->stat[q] -= row[q];
+                }
+                if (cnt)
+                    cnt->rollbacks++;
+                return;
+            }
+        }
+    }
+    st->occupancy[v] = (uint32_t)new_occ;                           /* :114 */
+    while (j < S) {                                                 /* :116-119 */
+        L[j] = 0u;
+        j += 1;
+    }
+}
+
+/* cme.wgsl over the dispatch of cme.rs:61-66. The x overhang of an odd R0 (hazard H5) is a no-op. */
+void cso_phase_c(const cso_model *m, cso_state *st, uint32_t itime, cso_counters *cnt)
+{
+    float *cumm = (float *)malloc(sizeof(float) * (m->nr + 1u));
+    for (uint32_t x = 0; x < m->res[0]; ++x)
+        for (uint32_t y = 0; y < m->res[1]; ++y)
+            for (uint32_t z = 0; z < m->res[2]; ++z)
+                cme_voxel(m, st, x, y, z, itime, cumm, cnt);
+    free(cumm);
+}
+
+/* final_stage.wgsl:25-33: stable partition of non-zero slots to the front of each voxel. */
+void cso_phase_f(const cso_model *m, cso_state *st)
+{
+    uint64_t n = nvox(m);
+    for (uint64_t v = 0; v < n; ++v) {
+        uint32_t *L = st->lattice + v * m->S;
+        uint32_t j = 0;
+        for (uint32_t i = 0; i < m->S; ++i) {
+            if (L[i] != 0u) {
+                uint32_t t = L[i];
+                L[i] = L[j];
+                L[j] = t;
+                j += 1;
+            }
+        }
+    }
+}
+
+/* simulation.rs:99-150: rdme -> cme -> final_stage; the dest->src copies make the next step's source
+ * equal to this step's destination, so one in-place state plus a copy of the source suffices. */
+int cso_step(const cso_model *m, cso_state *st, uint32_t itime, cso_counters *cnt)
+{
+    if (!m || !st || m->S == 0u || m->S > 16u || m->res[0] == 0u || m->res[1] == 0u || m->res[2] == 0u)
+        return 1;
+    uint64_t n = nvox(m);
+    uint32_t *lat_src = (uint32_t *)malloc(sizeof(uint32_t) * n * m->S);
+    uint32_t *occ_src = (uint32_t *)malloc(sizeof(uint32_t) * n);
+    if (!lat_src || !occ_src) {
+        free(lat_src);
+        free(occ_src);
+        return 2;
+    }
+    memcpy(lat_src, st->lattice, sizeof(uint32_t) * n * m->S);
+    memcpy(occ_src, st->occupancy, sizeof(uint32_t) * n);
+    cso_phase_r(m, lat_src, occ_src, st, itime, cnt);
+    cso_phase_c(m, st, itime, cnt);
+    cso_phase_f(m, st);
+    free(lat_src);
+    free(occ_src);
+    return 0;
+}
+
+int cso_run(const cso_model *m, cso_state *st, const uint32_t *itime, uint32_t nsteps, cso_counters *cnt)
+{
+    for (uint32_t t = 0; t < nsteps; ++t) {
+        int rc = cso_step(m, st, itime[t], cnt);
+        if (rc)
+            return rc;
+    }
+    return 0;
+}

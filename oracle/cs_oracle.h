@@ -1,0 +1,90 @@
+/*
+ * cs_oracle.h -- CPU restatement of cell_simulation's RDME step (TEST INFRASTRUCTURE ONLY).
+ *
+ * This is the parity checker. It is loaded only by tests/, by __graft_entry__.smoke() and by
+ * bench.py's cpu_baseline leg. The product (cell_simulation_amd) never links or calls it.
+ *
+ * The step is executed in the canonical serial order: voxels in ascending linear index
+ * v = z + y*R2 + x*R2*R1, particle slots in ascending order. The reference kernels race (spin locks
+ * and first-fit CAS, res/shader/rdme.wgsl:25-88); this order is the serialisation both the oracle and
+ * the HIP path are defined against.
+ *
+ * Followed files (reference, read-only):
+ *   res/shader/random.wgsl:14-49        PCG hash, xorshift32, u32 -> f32
+ *   res/shader/functions.wgsl:42-51     voxel / lattice index math
+ *   res/shader/rdme.wgsl:25-229         diffusion sweep (lock + first-fit claim)
+ *   res/shader/cme.wgsl:25-120          per-voxel reaction sampling, slot rewrite, rollback
+ *   res/shader/final_stage.wgsl:25-33   stable slot compaction
+ *   src/simulation.rs:99-150            pass order rdme -> cme -> final_stage -> copy dest->src
+ */
+#ifndef CS_ORACLE_H
+#define CS_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    uint32_t res[3];          /* lattice resolution (x, y, z); x is the slowest index */
+    uint32_t S;               /* particle slots per voxel (MAX_PARTICLES_SITE), 1..16 */
+    uint32_t ns;              /* species count excluding void (species 0) */
+    uint32_t nr;              /* reaction count excluding the dummy row 0 */
+    uint32_t nreg;            /* region count */
+    float tau;                /* time step (s) */
+    float lambda;             /* voxel edge (m) */
+    uint32_t mod_zero;        /* value taken for X % 0 in the RDME seed (hazard H4) */
+    const float *diffusion;   /* [nreg][nreg][ns+1]: index sp + (ns+1)*dst + (ns+1)*nreg*src */
+    const int32_t *stoich;    /* [nr+1][ns+1] */
+    const int32_t *reactants; /* [nr+1][3], padded with species 0 */
+    const float *rates;       /* [nr+1] */
+    const uint32_t *regions;  /* [N] region label per voxel */
+    const uint32_t *reservoir;/* [N] reservoir species per voxel (0 = none); may be NULL */
+} cso_model;
+
+typedef struct {
+    uint32_t *lattice;        /* [N*S] species id per slot */
+    uint32_t *occupancy;      /* [N] */
+    int32_t *counts;          /* [N][ns+1] per-voxel species counts */
+    int32_t *stat;            /* [ns+1] global species counts (concentrations_stat) */
+} cso_state;
+
+typedef struct {
+    uint64_t moves;           /* successful hops */
+    uint64_t failed_claims;   /* hops rejected because the destination was full */
+    uint64_t alias_writes;    /* first-fit landed in slot S, i.e. the next voxel's slot 0 (hazard H2) */
+    uint64_t fired;           /* voxels where a reaction fired (including rollbacks) */
+    uint64_t rollbacks;       /* fired reactions undone because the rewrite reached S slots (H8) */
+    uint64_t near_threshold;  /* |r1 - (1-exp(-a0*tau))| <= 2^-24: an exp ulp could flip the outcome */
+} cso_counters;
+
+/* Random number primitives (random.wgsl:14-20, 32-45, 27-29). */
+uint32_t cso_pcg(uint32_t state);
+uint32_t cso_xorshift(uint32_t state);
+float cso_uint_to_float(uint32_t u);
+float cso_uniform_float(uint32_t state);
+/* The build's definition of the f32 exponential: exp evaluated in double, rounded once to f32. */
+float cso_expf(float x);
+
+/* One full step (phase R, phase C, phase F). Returns 0 on success, nonzero on bad arguments. */
+int cso_step(const cso_model *m, cso_state *st, uint32_t itime, cso_counters *cnt);
+
+/* The phases separately, for tests. Phase R needs copies of the source lattice and occupancy. */
+void cso_phase_r(const cso_model *m, const uint32_t *lat_src, const uint32_t *occ_src,
+                 cso_state *st, uint32_t itime, cso_counters *cnt);
+void cso_phase_c(const cso_model *m, cso_state *st, uint32_t itime, cso_counters *cnt);
+void cso_phase_f(const cso_model *m, cso_state *st);
+
+/* Probability vector of rdme.wgsl:138-191 for one particle, exposed for tests. */
+void cso_probability_vector(const cso_model *m, uint32_t x, uint32_t y, uint32_t z, uint32_t sp,
+                            float p[7]);
+
+/* Multi-step driver: steps t = 0..nsteps-1 use itime[t]. */
+int cso_run(const cso_model *m, cso_state *st, const uint32_t *itime, uint32_t nsteps, cso_counters *cnt);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,77 @@
+"""libcellsim.so loads and exports every entry point include/cell_sim.h declares (CPU only)."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from cell_simulation_amd import CellSimError, configs
+from cell_simulation_amd import _lib
+
+HEADER = Path(__file__).resolve().parent.parent / "include" / "cell_sim.h"
+
+
+def declared_functions():
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(cs_[a-z_]+)\s*\(", text)))
+
+
+def test_every_declared_symbol_is_exported():
+    names = declared_functions()
+    assert len(names) >= 40
+    dll = C.CDLL(str(_lib.LIB_PATH))
+    missing = [n for n in names if not hasattr(dll, n)]
+    assert not missing, missing
+    # and the Python binding covers exactly the declared API
+    assert sorted(_lib.SIGNATURES) == names
+
+
+def test_header_constants_match_binding():
+    text = HEADER.read_text()
+    for name in ("CS_OK", "CS_ERR_ARG", "CS_ERR_NOT_FOUND", "CS_ERR_STATE", "CS_ERR_HIP", "CS_ERR_UNSUPPORTED",
+                 "CS_ERR_FULL", "CS_ERR_PARSE", "CS_CNT_TIGHT", "CS_CNT_EXCEPTION", "CS_CNT_NEAR_THRESHOLD",
+                 "CS_CNT_FIRED", "CS_CNT_ROLLBACK", "CS_NUM_COUNTERS"):
+        m = re.search(rf"#define {name} (\d+)", text)
+        assert m and int(m.group(1)) == getattr(_lib, name), name
+
+
+def test_null_and_size_checks():
+    L = _lib.lib()
+    assert L.cs_sim_create(None, None) == _lib.CS_ERR_ARG
+    assert b"null" in L.cs_last_error()
+    sim = configs.spherical_cell(seed=1, edge=16, counts=(10, 10))
+    buf = np.zeros(10, np.uint8)
+    assert L.cs_sim_read_lattice(sim._h, buf.ctypes.data, buf.size) == _lib.CS_ERR_ARG
+    bad = np.full(16 ** 3 * 8, 200, np.uint8)
+    assert L.cs_sim_write_lattice(sim._h, bad.ctypes.data, bad.size) == _lib.CS_ERR_ARG
+    s = _lib.StatSampleC()
+    assert L.cs_sim_pop_stat(sim._h, C.byref(s)) == _lib.CS_ERR_NOT_FOUND
+
+
+def test_device_preparation_fails_loudly_without_a_gpu():
+    """Where no HIP device exists the C ABI reports an error instead of falling back to the CPU."""
+    sim = configs.spherical_cell(seed=1, edge=16, counts=(10, 10))
+    try:
+        sim.prepare_for_gpu(0)
+    except CellSimError as e:
+        assert e.code in (_lib.CS_ERR_HIP, _lib.CS_ERR_ARG)
+        with pytest.raises(CellSimError) as e2:
+            sim.step(0, 1, 0)
+        assert e2.value.code == _lib.CS_ERR_STATE
+    else:
+        assert sim.info()["device_ready"]
+
+
+def test_unsupported_models_are_rejected_before_any_device_work():
+    sim = Simulation_with_reservoir()
+    with pytest.raises(CellSimError) as e:
+        sim.prepare_for_gpu(0)
+    assert e.value.code in (_lib.CS_ERR_UNSUPPORTED, _lib.CS_ERR_HIP, _lib.CS_ERR_ARG)
+
+
+def Simulation_with_reservoir():
+    from cell_simulation_amd import Simulation
+
+    return Simulation.from_file(Path(__file__).resolve().parent / "golden" / "easy.json", seed=1)
