@@ -447,10 +447,16 @@ int cs_sim_write_lattice(cs_sim *sim, const uint8_t *in, size_t nbytes)
         const size_t n = m.n_voxels() * m.slots();
         if (nbytes != n)
             throw cs::Error(CS_ERR_ARG, "lattice buffer must hold N*S bytes");
-        const uint32_t ns = m.num_species();
+        const uint32_t ns = m.num_species(), S = m.slots();
         for (size_t i = 0; i < n; ++i)
             if (in[i] > ns)
                 throw cs::Error(CS_ERR_ARG, "lattice holds an unknown species id");
+        // Occupancy is the number of occupied slots, which must come first in each voxel (the state
+        // final_stage.wgsl:25-33 leaves behind).
+        for (size_t v = 0; v < n / S; ++v)
+            for (uint32_t k = 1; k < S; ++k)
+                if (in[v * S + k] != 0 && in[v * S + k - 1] == 0)
+                    throw cs::Error(CS_ERR_ARG, "lattice slots must be compacted (no empty slot before an occupied one)");
         if (sim->device)
             sim->device->write_lattice(in);
         else

@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -33,6 +34,7 @@ namespace {
 
 constexpr int kMaxRW = 4;  // reaction bitmask words: up to 255 reactions
 constexpr int kBlock = 256;
+constexpr uint32_t kMaxTight = 1024;  // tight voxels replayed exactly per step; more are counted as exceptions
 
 void hip_check(hipError_t e, const char *what)
 {
@@ -50,6 +52,13 @@ struct StepArgs {
     const int8_t *stoich;
     const uint64_t *rmask;
     unsigned long long *counters;
+    uint32_t *fix_ctl;    // [0] tight voxels noted this step, [1] blocks finished
+    uint32_t *fix_list;   // tight voxels (max_tight)
+    uint32_t *fix_F;      // affected voxels (8 * max_tight)
+    uint32_t *fix_W;      // sources acting on them (56 * max_tight)
+    uint8_t *fix_L;       // slots of the affected voxels during the replay
+    uint32_t *fix_O;      // their occupancy during the replay
+    uint32_t max_tight;
     uint32_t R0, R1, R2, R12;
     uint32_t N;
     uint32_t ns, nr, nreg, nrw;
@@ -136,25 +145,40 @@ __device__ __forceinline__ uint32_t get_byte(const Slots<S> &s, int i)
     return (s.w[i >> 2] >> ((i & 3) * 8)) & 0xFFu;
 }
 
-// byte at a run-time position j < S
+// Byte at a run-time position j < S. Written as 64-bit shifts over whole values, never as an indexed
+// element: an indexed 4-word array is kept in scratch memory by the compiler.
 template <int S>
 __device__ __forceinline__ uint32_t get_byte_dyn(const Slots<S> &s, uint32_t j)
 {
-    uint32_t w = s.w[0];
-#pragma unroll
-    for (int q = 1; q < S / 4; ++q)
-        w = ((j >> 2) == uint32_t(q)) ? s.w[q] : w;
-    return (w >> ((j & 3u) * 8u)) & 0xFFu;
+    if constexpr (S == 4) {
+        return (s.w[0] >> ((j & 3u) * 8u)) & 0xFFu;
+    } else {
+        uint64_t v = uint64_t(s.w[0]) | (uint64_t(s.w[1]) << 32);
+        if constexpr (S == 16) {
+            const uint64_t hi = uint64_t(s.w[2]) | (uint64_t(s.w[3]) << 32);
+            v = (j & 8u) ? hi : v;
+        }
+        return uint32_t(v >> ((j & 7u) * 8u)) & 0xFFu;
+    }
 }
 
 // OR a byte into a zero slot at run-time position j; positions >= S are dropped
 template <int S>
 __device__ __forceinline__ void put_byte_dyn(Slots<S> &s, uint32_t j, uint32_t val)
 {
-    const uint32_t sh = (j & 3u) * 8u;
-#pragma unroll
-    for (int q = 0; q < S / 4; ++q)
-        s.w[q] |= ((j >> 2) == uint32_t(q)) ? (val << sh) : 0u;
+    if constexpr (S == 4) {
+        s.w[0] |= (j < 4u) ? (val << ((j & 3u) * 8u)) : 0u;
+    } else {
+        const uint64_t bit = uint64_t(val) << ((j & 7u) * 8u);
+        const uint64_t lo = (j < 8u) ? bit : 0ull;
+        s.w[0] |= uint32_t(lo);
+        s.w[1] |= uint32_t(lo >> 32);
+        if constexpr (S == 16) {
+            const uint64_t hi = (j >= 8u && j < 16u) ? bit : 0ull;
+            s.w[2] |= uint32_t(hi);
+            s.w[3] |= uint32_t(hi >> 32);
+        }
+    }
 }
 
 // ---- hop decision (rdme.wgsl:132-136, 138-191, 220-226) -----------------------------------------
@@ -185,29 +209,49 @@ __device__ __forceinline__ Hood make_hood(const StepArgs &a, uint32_t x, uint32_
     return h;
 }
 
+// The model tables as the kernels read them: in LDS for the tiled kernel, in global memory otherwise.
+struct Tab {
+    const float *p2;        // [nreg][nreg][ns+1] hop probability per direction
+    const float *rates;     // [nr+1]
+    const uint32_t *ridx;   // [nr+1] three reactant species, one byte each
+    const int8_t *stoich;   // [nr+1][ns+1]
+    const uint64_t *rmask;  // [ns+1][nrw] reactions having the species as a reactant; row 0: zero-order
+};
+
+__device__ __forceinline__ Tab global_tab(const StepArgs &a)
+{
+    return Tab{a.p2, a.rates, reinterpret_cast<const uint32_t *>(a.ridx), a.stoich, a.rmask};
+}
+
 // Direction 0..5 of an existing neighbour the particle hops to, or 6 if it stays (stay draw or
-// reflecting boundary). The cumulative sum is the reference's p[0..5] in f32, in the same order.
-__device__ __forceinline__ uint32_t hop(const StepArgs &a, const Hood &h, uint32_t sp, uint32_t slot)
+// reflecting boundary). The cumulative sum is the reference's p[0..5] in f32, in the same order; all six
+// table entries are fetched before the comparisons so the loads overlap.
+__device__ __forceinline__ uint32_t hop(const StepArgs &a, const Tab &tb, const Hood &h, uint32_t sp, uint32_t slot)
 {
     const float r = uniform_float(pcg_hash(h.seed0 + slot));
     const uint32_t n1 = a.ns + 1u;
-    const float *row = a.p2 + size_t(n1) * a.nreg * h.src_reg + sp;
+    const float *row = tb.p2 + n1 * a.nreg * h.src_reg + sp;
+    float val[6];
+#pragma unroll
+    for (int d = 0; d < 6; ++d)
+        val[d] = row[n1 * h.rg[d]];
     float acc = 0.0f;
+    uint32_t dir = 6u;  // 6: no stop yet; 7: stopped on a missing neighbour (stays)
 #pragma unroll
     for (int d = 0; d < 6; ++d) {
         if (h.inside & (1u << d))
-            acc = acc + row[size_t(n1) * h.rg[d]];
-        if (!(r > acc))
-            return ((h.inside >> d) & 1u) ? uint32_t(d) : 6u;
+            acc = acc + val[d];
+        if (dir == 6u && !(r > acc))
+            dir = ((h.inside >> d) & 1u) ? uint32_t(d) : 7u;
     }
-    return 6u;
+    return dir == 7u ? 6u : dir;
 }
 
 // Append the particles of neighbour u that hop in direction `want` (toward the voxel being gathered),
 // in source-slot order. Returns the number found (may exceed the list capacity S).
 template <int S>
-__device__ __forceinline__ uint32_t gather_from(const StepArgs &a, uint32_t ux, uint32_t uy, uint32_t uz,
-                                                uint32_t u, uint32_t want, Slots<S> &list, uint32_t n)
+__device__ __forceinline__ uint32_t gather_from(const StepArgs &a, const Tab &tb, uint32_t ux, uint32_t uy,
+                                                uint32_t uz, uint32_t u, uint32_t want, Slots<S> &list, uint32_t n)
 {
     const Slots<S> su = load_slots<S>(a.src, u);
     const uint32_t occ = count_nonzero(su);
@@ -218,7 +262,7 @@ __device__ __forceinline__ uint32_t gather_from(const StepArgs &a, uint32_t ux, 
     for (int s = 0; s < S; ++s) {
         if (uint32_t(s) < occ) {
             const uint32_t sp = get_byte(su, s);
-            if (hop(a, h, sp, s) == want) {
+            if (hop(a, tb, h, sp, s) == want) {
                 put_byte_dyn(list, n, sp);
                 ++n;
             }
@@ -242,11 +286,12 @@ __device__ __forceinline__ uint32_t species_count(const Slots<S> &s, uint32_t sp
 }
 
 template <int S>
-__device__ __forceinline__ float propensity(const StepArgs &a, const Slots<S> &s, uint32_t r)
+__device__ __forceinline__ float propensity(const Tab &tb, const Slots<S> &s, uint32_t r)
 {
-    const uchar4 ix = *reinterpret_cast<const uchar4 *>(a.ridx + 4u * r);
-    const uint32_t prod = species_count(s, ix.x) * species_count(s, ix.y) * species_count(s, ix.z);
-    return a.rates[r] * float(int32_t(prod));
+    const uint32_t ix = tb.ridx[r];
+    const uint32_t prod = species_count(s, ix & 0xFFu) * species_count(s, (ix >> 8) & 0xFFu) *
+                          species_count(s, (ix >> 16) & 0xFFu);
+    return tb.rates[r] * float(int32_t(prod));
 }
 
 // Applies the reaction step to `out` (cnt particles, compacted). Reactions with a reactant absent from
@@ -254,8 +299,8 @@ __device__ __forceinline__ float propensity(const StepArgs &a, const Slots<S> &s
 // (zero-order ones and those with a reactant present) are visited; the f32 sums match the reference's
 // sequential sums bit for bit.
 template <int S>
-__device__ void react(const StepArgs &a, Slots<S> &out, uint32_t cnt, uint32_t x, uint32_t y, uint32_t z,
-                      bool &fired, bool &rolled, bool &near, bool &bad)
+__device__ __forceinline__ void react(const StepArgs &a, const Tab &tb, Slots<S> &out, uint32_t cnt, uint32_t x,
+                                      uint32_t y, uint32_t z, bool &fired, bool &rolled, bool &near, bool &bad)
 {
     fired = rolled = near = bad = false;
     const uint32_t P = pcg_hash(a.itime + x * y + z);
@@ -264,13 +309,13 @@ __device__ void react(const StepArgs &a, Slots<S> &out, uint32_t cnt, uint32_t x
     uint64_t cand[kMaxRW];
 #pragma unroll
     for (int w = 0; w < kMaxRW; ++w)
-        cand[w] = (uint32_t(w) < a.nrw) ? a.rmask[w] : 0ull;
+        cand[w] = (uint32_t(w) < a.nrw) ? tb.rmask[w] : 0ull;
     for (uint32_t j = 0; j < cnt; ++j) {
         const uint32_t sp = get_byte_dyn(out, j);
 #pragma unroll
         for (int w = 0; w < kMaxRW; ++w)
             if (uint32_t(w) < a.nrw)
-                cand[w] |= a.rmask[sp * a.nrw + w];
+                cand[w] |= tb.rmask[sp * a.nrw + w];
     }
 
     float tot = 0.0f;
@@ -280,7 +325,7 @@ __device__ void react(const StepArgs &a, Slots<S> &out, uint32_t cnt, uint32_t x
         while (m) {
             const uint32_t r = uint32_t(w) * 64u + uint32_t(__ffsll((long long)m) - 1);
             m &= m - 1ull;
-            tot = tot + propensity(a, out, r);
+            tot = tot + propensity(tb, out, r);
         }
     }
 
@@ -306,7 +351,7 @@ __device__ void react(const StepArgs &a, Slots<S> &out, uint32_t cnt, uint32_t x
         while (m && !chosen) {
             const uint32_t r = uint32_t(w) * 64u + uint32_t(__ffsll((long long)m) - 1);
             m &= m - 1ull;
-            cum = cum + propensity(a, out, r);
+            cum = cum + propensity(tb, out, r);
             i = r;
             if (r >= a.nr || !(r2 > cum / tot))
                 chosen = true;
@@ -317,7 +362,7 @@ __device__ void react(const StepArgs &a, Slots<S> &out, uint32_t cnt, uint32_t x
 
     // Stoichiometry and slot rewrite in species order (cme.wgsl:76-119). The rewrite is undone when it
     // reaches S slots (cme.wgsl:95-108), leaving the pre-reaction slots.
-    const int8_t *row = a.stoich + size_t(i) * (a.ns + 1u);
+    const int8_t *row = tb.stoich + i * (a.ns + 1u);
     Slots<S> res = zero_slots<S>();
     uint32_t j = 0;
     for (uint32_t sp = 1; sp <= a.ns; ++sp) {
@@ -342,13 +387,17 @@ __device__ __forceinline__ Slots<S> assemble(const Slots<S> &own, uint32_t occ, 
                                              const Slots<S> &lo, uint32_t nlo, const Slots<S> &hi, uint32_t nhi,
                                              uint32_t &cnt)
 {
-    Slots<S> out = zero_slots<S>();
     uint32_t j = 0, h = 0;
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
-        if (uint32_t(s) < occ) {
+    Slots<S> out;
+    if (nhi == 0u && (stay_mask & ((1ull << occ) - 1ull)) == ((1ull << occ) - 1ull)) {
+        // nothing left and nothing refills: the own slots stay as they are, arrivals follow
+        out = own;
+        j = occ;
+    } else {
+        out = zero_slots<S>();
+        for (uint32_t s = 0; s < occ; ++s) {
             if ((stay_mask >> s) & 1ull) {
-                put_byte_dyn(out, j++, get_byte(own, s));
+                put_byte_dyn(out, j++, get_byte_dyn(own, s));
             } else if (h < nhi) {
                 put_byte_dyn(out, j++, get_byte_dyn(hi, h));
                 ++h;
@@ -359,7 +408,7 @@ __device__ __forceinline__ Slots<S> assemble(const Slots<S> &own, uint32_t occ, 
         put_byte_dyn(out, j++, get_byte_dyn(lo, q));
     for (; h < nhi; ++h)
         put_byte_dyn(out, j++, get_byte_dyn(hi, h));
-    cnt = j;
+    cnt = j < uint32_t(S) ? j : uint32_t(S);  // a tight voxel may collect more than S; extras were dropped
     return out;
 }
 
@@ -371,49 +420,667 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb)
     return xcd * q + min(xcd, rem) + idx;
 }
 
+struct VoxelFlags {
+    bool tight = false, fired = false, rolled = false, near = false, bad = false;
+};
+
+// The gathered step of voxel v, everything read from global memory (the one-lane-per-voxel form).
+template <int S>
+__device__ __forceinline__ Slots<S> gather_voxel(const StepArgs &a, const Tab &tb, uint32_t v, VoxelFlags &f)
+{
+    const uint32_t z = v % a.R2;
+    const uint32_t y = (v / a.R2) % a.R1;
+    const uint32_t x = v / a.R12;
+    const Slots<S> own = load_slots<S>(a.src, v);
+    const uint32_t occ = count_nonzero(own);
+    uint64_t stay = ~0ull;
+    if (occ) {
+        const Hood h = make_hood(a, x, y, z, v, S);
+        stay = 0ull;
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+            if (uint32_t(s) < occ && hop(a, tb, h, get_byte(own, s), s) == 6u)
+                stay |= 1ull << s;
+    }
+    Slots<S> lo = zero_slots<S>(), hi = zero_slots<S>();
+    uint32_t nlo = 0, nhi = 0;
+    if (x > 0u) nlo = gather_from<S>(a, tb, x - 1u, y, z, v - a.R12, 1u, lo, nlo);      // from -x, hopped +x
+    if (y > 0u) nlo = gather_from<S>(a, tb, x, y - 1u, z, v - a.R2, 3u, lo, nlo);       // from -y, hopped +y
+    if (z > 0u) nlo = gather_from<S>(a, tb, x, y, z - 1u, v - 1u, 5u, lo, nlo);         // from -z, hopped +z
+    if (z + 1u < a.R2) nhi = gather_from<S>(a, tb, x, y, z + 1u, v + 1u, 4u, hi, nhi);  // from +z, hopped -z
+    if (y + 1u < a.R1) nhi = gather_from<S>(a, tb, x, y + 1u, z, v + a.R2, 2u, hi, nhi);  // from +y, hopped -y
+    if (x + 1u < a.R0) nhi = gather_from<S>(a, tb, x + 1u, y, z, v + a.R12, 0u, hi, nhi);  // from +x, hopped -x
+    f.tight = occ + nlo + nhi > uint32_t(S);
+    uint32_t cnt = 0;
+    Slots<S> out = assemble<S>(own, occ, stay, lo, nlo, hi, nhi, cnt);
+    react<S>(a, tb, out, cnt, x, y, z, f.fired, f.rolled, f.near, f.bad);
+    return out;
+}
+
+// ---- exact step around tight voxels -----------------------------------------------------------------
+// A tight voxel t (own particles + intended arrivals > S) is where the gather may differ from the serial
+// order: a claim can find t full (rdme.wgsl:62, 30-40). Claims into any other voxel always succeed. So
+// only F = T + the six neighbours of T + the voxel after t in memory (slot S of t aliases its slot 0) can
+// end differently, and only sources in W = F + the neighbours of F act on F. The last block of the step
+// replays W in canonical order on a private copy of F's slots (the scatter form of rdme.wgsl:45-88), then
+// applies the reaction to F and overwrites F's output. Everything outside F was exact already.
+__device__ __forceinline__ uint32_t neighbor_or_none(const StepArgs &a, uint32_t v, int k)
+{
+    const uint32_t z = v % a.R2, y = (v / a.R2) % a.R1, x = v / a.R12;
+    switch (k) {
+    case 0: return x > 0u ? v - a.R12 : 0xFFFFFFFFu;
+    case 1: return x + 1u < a.R0 ? v + a.R12 : 0xFFFFFFFFu;
+    case 2: return y > 0u ? v - a.R2 : 0xFFFFFFFFu;
+    case 3: return y + 1u < a.R1 ? v + a.R2 : 0xFFFFFFFFu;
+    case 4: return z > 0u ? v - 1u : 0xFFFFFFFFu;
+    case 5: return z + 1u < a.R2 ? v + 1u : 0xFFFFFFFFu;
+    case 6: return v;
+    default: return v + 1u < a.N ? v + 1u : 0xFFFFFFFFu;  // the voxel that slot S of v aliases
+    }
+}
+
+// out = sorted unique union over `nstreams` monotone maps of a sorted input. Returns the length, or
+// cap + 1 on overflow.
+__device__ __noinline__ uint32_t expand_sorted(const StepArgs &a, const uint32_t *in, uint32_t n, int nstreams, uint32_t *out,
+                                  uint32_t cap)
+{
+    uint32_t pos[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    uint32_t m = 0, last = 0xFFFFFFFFu;
+    for (;;) {
+        uint32_t best = 0xFFFFFFFFu;
+        int bk = -1;
+        for (int k = 0; k < nstreams; ++k) {
+            while (pos[k] < n && neighbor_or_none(a, in[pos[k]], k) == 0xFFFFFFFFu)
+                ++pos[k];
+            if (pos[k] < n) {
+                const uint32_t c = neighbor_or_none(a, in[pos[k]], k);
+                if (c < best) {
+                    best = c;
+                    bk = k;
+                }
+            }
+        }
+        if (bk < 0)
+            return m;
+        ++pos[bk];
+        if (best != last) {
+            if (m >= cap)
+                return cap + 1u;
+            out[m++] = best;
+            last = best;
+        }
+    }
+}
+
+__device__ __forceinline__ int find_sorted(const uint32_t *arr, uint32_t n, uint32_t v)
+{
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) / 2u;
+        if (arr[mid] < v)
+            lo = mid + 1u;
+        else
+            hi = mid;
+    }
+    return (lo < n && arr[lo] == v) ? int(lo) : -1;
+}
+
+__device__ __forceinline__ void counter_add(unsigned long long *c, int idx, long long d)
+{
+    if (d)
+        atomicAdd(c + idx, (unsigned long long)d);
+}
+
+template <int S>
+__device__ __noinline__ void fixup_tight(const StepArgs &a)
+{
+    const Tab tb = global_tab(a);
+    const uint32_t nt_all = a.fix_ctl[0];
+    if (nt_all > a.max_tight) {
+        counter_add(a.counters, CS_CNT_EXCEPTION, (long long)nt_all);  // too many to replay
+        return;
+    }
+    uint32_t *T = a.fix_list;
+    const uint32_t nt = nt_all;
+    for (uint32_t i = 1; i < nt; ++i) {  // insertion sort (a handful of entries in practice)
+        const uint32_t x = T[i];
+        uint32_t j = i;
+        while (j > 0u && T[j - 1u] > x) {
+            T[j] = T[j - 1u];
+            --j;
+        }
+        T[j] = x;
+    }
+    const uint32_t capF = 8u * a.max_tight, capW = 7u * capF;
+    const uint32_t nF = expand_sorted(a, T, nt, 8, a.fix_F, capF);
+    const uint32_t nW = nF > capF ? capW + 1u : expand_sorted(a, a.fix_F, nF, 7, a.fix_W, capW);
+    if (nF > capF || nW > capW) {
+        counter_add(a.counters, CS_CNT_EXCEPTION, (long long)nt);
+        return;
+    }
+    const uint32_t *F = a.fix_F, *W = a.fix_W;
+    uint8_t *Lr = a.fix_L;
+    uint32_t *Or = a.fix_O;
+    long long d_fired = 0, d_rolled = 0, d_near = 0, d_bad = 0;
+    for (uint32_t k = 0; k < nF; ++k) {
+        // undo the counts the gathered result of this voxel contributed
+        VoxelFlags g;
+        (void)gather_voxel<S>(a, tb, F[k], g);
+        d_fired -= g.fired;
+        d_rolled -= g.rolled;
+        d_near -= g.near;
+        d_bad -= g.bad;
+        const Slots<S> s = load_slots<S>(a.src, F[k]);
+        for (int q = 0; q < S; ++q)
+            Lr[size_t(k) * S + q] = uint8_t(get_byte(s, q));
+        Or[k] = count_nonzero(s);
+    }
+    // canonical replay of the sources in W
+    for (uint32_t wi = 0; wi < nW; ++wi) {
+        const uint32_t w = W[wi];
+        const Slots<S> sw = load_slots<S>(a.src, w);
+        const uint32_t occ = count_nonzero(sw);
+        if (!occ)
+            continue;
+        const int wF = find_sorted(F, nF, w);
+        const uint32_t z = w % a.R2, y = (w / a.R2) % a.R1, x = w / a.R12;
+        const Hood h = make_hood(a, x, y, z, w, S);
+        for (uint32_t s = 0; s < occ; ++s) {
+            const uint32_t sp = get_byte(sw, int(s));
+            const uint32_t d = hop(a, tb, h, sp, s);
+            if (d >= 6u)
+                continue;
+            const uint32_t t = neighbor_or_none(a, w, int(d));
+            const int tF = find_sorted(F, nF, t);
+            if (wF < 0 && tF < 0)
+                continue;
+            if (tF >= 0) {
+                const uint32_t old = Or[tF];
+                Or[tF] = old + 1u;
+                bool ok = false;
+                if (old <= uint32_t(S)) {
+                    for (int j = 0; j < S && !ok; ++j)
+                        if (Lr[size_t(tF) * S + j] == 0u) {
+                            Lr[size_t(tF) * S + j] = uint8_t(sp);
+                            ok = true;
+                        }
+                    if (!ok) {  // slot S: the next voxel's slot 0 (out of the lattice counts as taken)
+                        const int nF1 = (t + 1u < a.N) ? find_sorted(F, nF, t + 1u) : -1;
+                        if (nF1 >= 0 && Lr[size_t(nF1) * S] == 0u) {
+                            Lr[size_t(nF1) * S] = uint8_t(sp);
+                            ok = true;
+                            d_bad += 1;  // occupancy S+1 with S slots: the byte-slot state cannot hold it
+                        }
+                    }
+                }
+                if (!ok) {
+                    Or[tF] = old;
+                    continue;
+                }
+            }
+            if (wF >= 0) {
+                Or[wF] -= 1u;
+                Lr[size_t(wF) * S + s] = 0u;
+            }
+        }
+    }
+    // reaction and compaction of F, overwriting the gathered output
+    for (uint32_t k = 0; k < nF; ++k) {
+        const uint32_t v = F[k];
+        Slots<S> out = zero_slots<S>();
+        uint32_t cnt = 0;
+        for (int j = 0; j < S; ++j)
+            if (Lr[size_t(k) * S + j]) {
+                put_byte_dyn(out, cnt, Lr[size_t(k) * S + j]);
+                ++cnt;
+            }
+        bool fired, rolled, near, bad;
+        react<S>(a, tb, out, cnt, v / a.R12, (v / a.R2) % a.R1, v % a.R2, fired, rolled, near, bad);
+        d_fired += fired;
+        d_rolled += rolled;
+        d_near += near;
+        d_bad += bad;
+        store_slots<S>(a.dst, v, out);
+    }
+    counter_add(a.counters, CS_CNT_FIRED, d_fired);
+    counter_add(a.counters, CS_CNT_ROLLBACK, d_rolled);
+    counter_add(a.counters, CS_CNT_NEAR_THRESHOLD, d_near);
+    counter_add(a.counters, CS_CNT_EXCEPTION, d_bad);
+}
+
+__device__ __forceinline__ void note_tight(const StepArgs &a, uint32_t v)
+{
+    const uint32_t i = atomicAdd(a.fix_ctl, 1u);
+    if (i < a.max_tight)
+        a.fix_list[i] = v;
+}
+
+// One block launched after every step kernel on the same stream: replays the tight neighbourhoods noted
+// by the step (if any) and re-arms the count. It returns at once when nothing was tight, which is the
+// normal case; a per-block "last block done" atomic in the step kernel would serialise thousands of
+// same-address device-scope atomics per step instead.
+template <int S>
+__global__ __launch_bounds__(64) void tight_replay_kernel(StepArgs a)
+{
+    if (threadIdx.x != 0 || a.fix_ctl[0] == 0u)
+        return;
+    fixup_tight<S>(a);
+    a.fix_ctl[0] = 0u;
+}
+
 template <int S>
 __global__ __launch_bounds__(kBlock) void rdme_step_kernel(StepArgs a)
 {
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t v = blk * kBlock + threadIdx.x;
-    const bool live = v < a.N;
-    bool tight = false, fired = false, rolled = false, near = false, bad = false;
-    if (live) {
-        const uint32_t z = v % a.R2;
-        const uint32_t y = (v / a.R2) % a.R1;
-        const uint32_t x = v / a.R12;
-
-        const Slots<S> own = load_slots<S>(a.src, v);
-        const uint32_t occ = count_nonzero(own);
-        uint64_t stay = ~0ull;
-        if (occ) {
-            const Hood h = make_hood(a, x, y, z, v, S);
-            stay = 0ull;
-#pragma unroll
-            for (int s = 0; s < S; ++s)
-                if (uint32_t(s) < occ && hop(a, h, get_byte(own, s), s) == 6u)
-                    stay |= 1ull << s;
-        }
-        Slots<S> lo = zero_slots<S>(), hi = zero_slots<S>();
-        uint32_t nlo = 0, nhi = 0;
-        if (x > 0u) nlo = gather_from<S>(a, x - 1u, y, z, v - a.R12, 1u, lo, nlo);      // from -x, hopped +x
-        if (y > 0u) nlo = gather_from<S>(a, x, y - 1u, z, v - a.R2, 3u, lo, nlo);       // from -y, hopped +y
-        if (z > 0u) nlo = gather_from<S>(a, x, y, z - 1u, v - 1u, 5u, lo, nlo);         // from -z, hopped +z
-        if (z + 1u < a.R2) nhi = gather_from<S>(a, x, y, z + 1u, v + 1u, 4u, hi, nhi);  // from +z, hopped -z
-        if (y + 1u < a.R1) nhi = gather_from<S>(a, x, y + 1u, z, v + a.R2, 2u, hi, nhi);  // from +y, hopped -y
-        if (x + 1u < a.R0) nhi = gather_from<S>(a, x + 1u, y, z, v + a.R12, 0u, hi, nhi);  // from +x, hopped -x
-        tight = occ + nlo + nhi > uint32_t(S);
-
-        uint32_t cnt = 0;
-        Slots<S> out = assemble<S>(own, occ, stay, lo, nlo, hi, nhi, cnt);
-        react<S>(a, out, cnt, x, y, z, fired, rolled, near, bad);
+    VoxelFlags f;
+    if (v < a.N) {
+        const Slots<S> out = gather_voxel<S>(a, global_tab(a), v, f);
         store_slots<S>(a.dst, v, out);
+        if (f.tight)
+            note_tight(a, v);
     }
-    count_event(a.counters, CS_CNT_TIGHT, tight);
-    count_event(a.counters, CS_CNT_EXCEPTION, bad);
-    count_event(a.counters, CS_CNT_NEAR_THRESHOLD, near);
-    count_event(a.counters, CS_CNT_FIRED, fired);
-    count_event(a.counters, CS_CNT_ROLLBACK, rolled);
+    count_event(a.counters, CS_CNT_TIGHT, f.tight);
+    count_event(a.counters, CS_CNT_EXCEPTION, f.bad);
+    count_event(a.counters, CS_CNT_NEAR_THRESHOLD, f.near);
+    count_event(a.counters, CS_CNT_FIRED, f.fired);
+    count_event(a.counters, CS_CNT_ROLLBACK, f.rolled);
+}
+
+// ---- tiled plane-march kernel ------------------------------------------------------------------------
+// A block owns a kTY x kTZ tile of (y, z) and marches along x through `xc` planes. Each plane is staged in
+// LDS with a one-voxel halo in y and z ("records": slots, per-slot hop codes, a summary of hop
+// directions), so every particle's hop is computed once per block instead of once per neighbour, and the
+// gather reads neighbour records from LDS. Region labels are staged with a two-voxel halo (a halo
+// record's hop needs its own neighbours' labels). Three planes of records are kept (x-1, x, x+1); the
+// slots and labels of the next plane are loaded into registers one iteration ahead, so their HBM
+// latency overlaps the current plane's work. The model tables live in LDS for the whole block.
+constexpr int kTZ = 64;                    // one wave per z-row
+constexpr int kTY = 4;                     // four rows: 256 lanes
+constexpr int kRZ = kTZ + 2;               // record row with z halo
+constexpr int kRec = (kTY + 2) * kRZ;      // records per plane (396)
+constexpr int kHalo = kRec - kTY * kTZ;    // halo records per plane (140)
+constexpr int kGZ = kTZ + 4;               // label row with a two-voxel z halo
+constexpr int kGrid = (kTY + 4) * kGZ;     // labels per plane (544)
+constexpr int kLabPerLane = (kGrid + kBlock - 1) / kBlock;  // 3
+constexpr uint32_t kTabLdsMax = 24 * 1024;  // tables larger than this run on the simple kernel
+constexpr int kTightLds = 64;               // tight voxels a block can note in LDS per step
+
+struct TileArgs {
+    uint32_t tiles_z, tiles_y, chunks, xc;
+    uint32_t has_zero_order;               // a reaction without reactants exists (fires in empty voxels)
+    uint32_t tab_words;                    // size of the table blob in 32-bit words
+    uint32_t off_p2, off_rates, off_ridx, off_stoich, off_rmask;  // byte offsets in the blob
+    const uint32_t *tab;                   // the blob in global memory
+};
+
+template <int S>
+struct TileLds {
+    uint32_t slots[3][kRec][S / 4];
+    uint64_t codes[3][kRec];
+    uint8_t summ[3][kRec];
+    uint8_t reg[3][kGrid];
+    uint32_t out[kTY * kTZ][S / 4];  // gathered output of this plane's full-path voxels
+    uint16_t hop_list[kRec];         // records of the staged plane that hold particles
+    uint16_t slow_list[kTY * kTZ];   // interior voxels that need the full gather
+    uint32_t n_hop, n_slow, n_tight;
+    uint32_t tight[kTightLds];
+};
+
+// Appends `item` to an LDS list for the lanes where `pred` holds, in lane order, with one LDS atomic
+// per wave.
+__device__ __forceinline__ void wave_append(bool pred, uint32_t item, uint32_t *counter, uint16_t *list)
+{
+    const uint64_t m = __ballot(pred);
+    if (m == 0ull)
+        return;
+    const uint32_t lane = __lane_id();
+    const uint32_t leader = uint32_t(__ffsll((long long)m) - 1);
+    uint32_t base = 0;
+    if (lane == leader)
+        base = atomicAdd(counter, uint32_t(__popcll(m)));
+    base = __shfl(base, int(leader));
+    if (pred)
+        list[base + uint32_t(__popcll(m & ((1ull << lane) - 1ull)))] = uint16_t(item);
+}
+
+// record index -> tile coordinates (yy in [-1, kTY], zz in [-1, kTZ])
+__device__ __forceinline__ int rec_of(int yy, int zz) { return (yy + 1) * kRZ + (zz + 1); }
+
+// The halo record handled by lane t < kHalo.
+__device__ __forceinline__ void halo_coords(int t, int &yy, int &zz)
+{
+    if (t < kRZ) { yy = -1; zz = t - 1; }
+    else if (t < 2 * kRZ) { yy = kTY; zz = t - kRZ - 1; }
+    else if (t < 2 * kRZ + kTY) { yy = t - 2 * kRZ; zz = -1; }
+    else { yy = t - 2 * kRZ - kTY; zz = kTZ; }
+}
+
+// Slots of (X, Y, Z) loaded without a branch: outside the lattice the load reads voxel 0 and `ok` is
+// cleared, to be applied (masked) when the value is used.
+template <int S>
+__device__ __forceinline__ Slots<S> load_slots_clamped(const StepArgs &a, int X, int Y, int Z, bool &ok)
+{
+    ok = X >= 0 && Y >= 0 && Z >= 0 && uint32_t(X) < a.R0 && uint32_t(Y) < a.R1 && uint32_t(Z) < a.R2;
+    const uint32_t v = ok ? uint32_t(Z) + uint32_t(Y) * a.R2 + uint32_t(X) * a.R12 : 0u;
+    return load_slots<S>(a.src, v);
+}
+
+template <int S>
+__device__ __forceinline__ Slots<S> masked(const Slots<S> &s, bool ok)
+{
+    Slots<S> r;
+#pragma unroll
+    for (int q = 0; q < S / 4; ++q)
+        r.w[q] = ok ? s.w[q] : 0u;
+    return r;
+}
+
+// Hop codes (one nibble per slot: 0..5 = hops in that direction, 6 = stays, 0xF = empty) and the
+// direction summary of one record at plane X. Labels come from the staged planes X-1, X, X+1.
+template <int S>
+__device__ __forceinline__ void record_hops(const StepArgs &a, const Tab &tb, const Slots<S> &sl, int X, int Y,
+                                            int Z, int yy, int zz, const uint8_t *rprev, const uint8_t *rcur,
+                                            const uint8_t *rnext, uint64_t &codes, uint32_t &summ)
+{
+    codes = ~0ull;
+    summ = 0u;
+    const uint32_t occ = count_nonzero(sl);
+    if (occ == 0u)
+        return;
+    const int gi = (yy + 2) * kGZ + (zz + 2);
+    Hood h;
+    h.src_reg = rcur[gi];
+    h.inside = 0u;
+    h.rg[0] = rprev[gi];
+    h.rg[1] = rnext[gi];
+    h.rg[2] = rcur[gi - kGZ];
+    h.rg[3] = rcur[gi + kGZ];
+    h.rg[4] = rcur[gi - 1];
+    h.rg[5] = rcur[gi + 1];
+    if (X > 0) h.inside |= 1u;
+    if (uint32_t(X) + 1u < a.R0) h.inside |= 2u;
+    if (Y > 0) h.inside |= 4u;
+    if (uint32_t(Y) + 1u < a.R1) h.inside |= 8u;
+    if (Z > 0) h.inside |= 16u;
+    if (uint32_t(Z) + 1u < a.R2) h.inside |= 32u;
+    const uint32_t ux = uint32_t(X), uy = uint32_t(Y), uz = uint32_t(Z);
+    const uint32_t xmy = (uy != 0u) ? (ux % uy) : a.mod_zero;
+    h.seed0 = a.itime + xmy + uz + (uz + uy * a.R2 + ux * a.R12) * uint32_t(S);
+    for (uint32_t s = 0; s < occ; ++s) {
+        const uint32_t d = hop(a, tb, h, get_byte_dyn(sl, s), s);
+        codes &= ~(0xFull << (4u * s));
+        codes |= uint64_t(d) << (4u * s);
+        if (d < 6u)
+            summ |= 1u << d;
+    }
+}
+
+// Bit 4k set iff nibble k of `codes` equals `want`.
+__device__ __forceinline__ uint64_t nibble_matches(uint64_t codes, uint32_t want)
+{
+    uint64_t x = codes ^ (uint64_t(want) * 0x1111111111111111ull);
+    x |= x >> 1;
+    x |= x >> 2;
+    return ~x & 0x1111111111111111ull;
+}
+
+// Append the slots of a neighbour record whose hop code equals `want`, in slot order.
+template <int S>
+__device__ __forceinline__ uint32_t take_arrivals(const uint32_t *words, uint64_t codes, uint32_t want,
+                                                  Slots<S> &list, uint32_t n)
+{
+    uint64_t m = nibble_matches(codes, want);
+    while (m) {
+        const uint32_t s = uint32_t(__ffsll((long long)m) - 1) >> 2;
+        m &= m - 1ull;
+        put_byte_dyn(list, n, (words[s >> 2] >> ((s & 3u) * 8u)) & 0xFFu);
+        ++n;
+    }
+    return n;
+}
+
+// One plane of the march runs in four phases separated by barriers:
+//   T0 stage the plane's slots and the next labels in LDS; list the records that hold particles;
+//   T1 compute the hops of the listed records (consecutive lanes, so ~one wave does the sparse work);
+//      store the previous plane's output;
+//   T2 for the plane behind, find the voxels that need the full gather (particles present or arriving);
+//   T3 run the full gather + reaction for the listed voxels, results into LDS.
+// Voxels with no particle and no arrival are written as zeros by their own lane without entering T3.
+template <int S>
+__global__ __launch_bounds__(kBlock) void rdme_step_tiled(StepArgs a, TileArgs t)
+{
+    __shared__ TileLds<S> L;
+    extern __shared__ uint32_t tab_lds[];
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t tz = blk % t.tiles_z;
+    const uint32_t ty = (blk / t.tiles_z) % t.tiles_y;
+    const uint32_t chunk = blk / (t.tiles_z * t.tiles_y);
+    const int y0 = int(ty) * kTY, z0 = int(tz) * kTZ;
+    const int xb = int(chunk * t.xc);
+    const int xe = min(int(a.R0), xb + int(t.xc));
+    const int lane = int(threadIdx.x);
+    const int iy = lane / kTZ, iz = lane % kTZ;  // interior voxel of this lane
+    int hy = 0, hz = 0;
+    const bool has_halo = lane < kHalo;
+    if (has_halo)
+        halo_coords(lane, hy, hz);
+    const int irec = rec_of(iy, iz);
+    const int hrec = has_halo ? rec_of(hy, hz) : 0;
+    const int Y = y0 + iy, Z = z0 + iz;
+    const bool in_lattice = uint32_t(Y) < a.R1 && uint32_t(Z) < a.R2;
+
+    for (uint32_t i = uint32_t(lane); i < t.tab_words; i += kBlock)
+        tab_lds[i] = t.tab[i];
+    const uint8_t *tbase = reinterpret_cast<const uint8_t *>(tab_lds);
+    const Tab tb{reinterpret_cast<const float *>(tbase + t.off_p2), reinterpret_cast<const float *>(tbase + t.off_rates),
+                 reinterpret_cast<const uint32_t *>(tbase + t.off_ridx),
+                 reinterpret_cast<const int8_t *>(tbase + t.off_stoich),
+                 reinterpret_cast<const uint64_t *>(tbase + t.off_rmask)};
+    if (lane == 0) {
+        L.n_hop = 0u;
+        L.n_slow = 0u;
+        L.n_tight = 0u;
+    }
+
+    // Label loads are unconditional from a clamped address into a full register; whether the value is
+    // real is decided when it is stored a plane later. A load under a branch, merged into a packed byte,
+    // makes the compiler wait for every outstanding load (the slot prefetch included) on the spot.
+    auto fetch_labels = [&](int X, uint32_t (&r)[kLabPerLane]) {
+#pragma unroll
+        for (int k = 0; k < kLabPerLane; ++k) {
+            const int i = lane + k * kBlock;
+            const int gy = y0 + i / kGZ - 2, gz = z0 + i % kGZ - 2;
+            const bool ok = i < kGrid && X >= 0 && uint32_t(X) < a.R0 && gy >= 0 && uint32_t(gy) < a.R1 &&
+                            gz >= 0 && uint32_t(gz) < a.R2;
+            const uint32_t idx = ok ? uint32_t(gz) + uint32_t(gy) * a.R2 + uint32_t(X) * a.R12 : 0u;
+            r[k] = uint32_t(a.reg[idx]) | (ok ? 0u : 0x100u);  // bit 8: not a lattice voxel
+        }
+    };
+    auto store_labels = [&](int slot, const uint32_t (&r)[kLabPerLane]) {
+#pragma unroll
+        for (int k = 0; k < kLabPerLane; ++k)
+            if (lane + k * kBlock < kGrid)
+                L.reg[slot][lane + k * kBlock] = (r[k] & 0x100u) ? uint8_t(0) : uint8_t(r[k]);
+    };
+    auto mod3 = [](int p) { return (p + 3) % 3; };
+
+    {   // labels of planes xb-2 and xb-1 (needed by the hops of plane xb-1)
+        uint32_t r[kLabPerLane];
+        fetch_labels(xb - 2, r);
+        store_labels(mod3(xb - 2), r);
+        fetch_labels(xb - 1, r);
+        store_labels(mod3(xb - 1), r);
+    }
+    // registers holding the next plane to stage: its slots and the labels one plane further
+    bool nx_mine_ok, nx_hal_ok;
+    Slots<S> nx_mine = load_slots_clamped<S>(a, xb - 1, Y, Z, nx_mine_ok);
+    Slots<S> nx_hal = load_slots_clamped<S>(a, xb - 1, y0 + hy, z0 + hz, nx_hal_ok);
+    nx_hal_ok = nx_hal_ok && has_halo;
+    uint32_t nx_lab[kLabPerLane];
+    fetch_labels(xb, nx_lab);
+
+    Slots<S> cur = zero_slots<S>();  // this lane's interior voxel at the plane being gathered
+    uint32_t n_tight = 0, n_bad = 0, n_near = 0, n_fired = 0, n_rolled = 0;
+    uint32_t prev_v = 0;             // output of the previous plane, stored in T1 of the next iteration
+    bool prev_ok = false, prev_slow = false;
+    __syncthreads();  // list counters are zero before any wave appends
+
+    for (int p = xb - 1; p <= xe; ++p) {
+        const int rs = mod3(p);
+        // ---- T0: stage plane p, list its records with particles
+        const Slots<S> mine = masked(nx_mine, nx_mine_ok), hal = masked(nx_hal, nx_hal_ok);
+        uint32_t lab[kLabPerLane];
+#pragma unroll
+        for (int k = 0; k < kLabPerLane; ++k)
+            lab[k] = nx_lab[k];
+        // prefetch plane p+1 and the labels of plane p+2 (past the chunk end the loads are clamped and
+        // their results never used)
+        nx_mine = load_slots_clamped<S>(a, p + 1, Y, Z, nx_mine_ok);
+        nx_hal = load_slots_clamped<S>(a, p + 1, y0 + hy, z0 + hz, nx_hal_ok);
+        nx_hal_ok = nx_hal_ok && has_halo;
+        fetch_labels(p + 2, nx_lab);
+        store_labels(mod3(p + 1), lab);
+#pragma unroll
+        for (int q = 0; q < S / 4; ++q) {
+            L.slots[rs][irec][q] = mine.w[q];
+            if (has_halo)
+                L.slots[rs][hrec][q] = hal.w[q];
+        }
+        L.codes[rs][irec] = ~0ull;
+        L.summ[rs][irec] = 0u;
+        if (has_halo) {
+            L.codes[rs][hrec] = ~0ull;
+            L.summ[rs][hrec] = 0u;
+        }
+        wave_append(count_nonzero(mine) != 0u, uint32_t(irec), &L.n_hop, L.hop_list);
+        wave_append(has_halo && count_nonzero(hal) != 0u, uint32_t(hrec), &L.n_hop, L.hop_list);
+        __syncthreads();
+
+        // ---- T1: hops of the listed records; store the previous plane's output
+        {
+            const uint32_t nh = L.n_hop;
+            const uint8_t *rp = L.reg[mod3(p - 1)], *rc = L.reg[rs], *rn = L.reg[mod3(p + 1)];
+            for (uint32_t i = uint32_t(lane); i < nh; i += kBlock) {
+                const int rec = int(L.hop_list[i]);
+                const int yy = rec / kRZ - 1, zz = rec % kRZ - 1;
+                Slots<S> sl;
+#pragma unroll
+                for (int q = 0; q < S / 4; ++q)
+                    sl.w[q] = L.slots[rs][rec][q];
+                uint64_t c;
+                uint32_t sm;
+                record_hops<S>(a, tb, sl, p, y0 + yy, z0 + zz, yy, zz, rp, rc, rn, c, sm);
+                L.codes[rs][rec] = c;
+                L.summ[rs][rec] = uint8_t(sm);
+            }
+        }
+        if (prev_ok) {
+            Slots<S> o = zero_slots<S>();
+            if (prev_slow) {
+#pragma unroll
+                for (int q = 0; q < S / 4; ++q)
+                    o.w[q] = L.out[lane][q];
+            }
+            store_slots<S>(a.dst, prev_v, o);
+        }
+        prev_ok = false;
+        if (lane == 0)
+            L.n_slow = 0u;
+        __syncthreads();
+
+        // ---- T2: which voxels of plane x = p-1 need the full gather
+        const int x = p - 1;
+        const bool gather_here = x >= xb && x < xe && in_lattice;
+        bool slow = false;
+        if (gather_here) {
+            const int r0 = mod3(x - 1), r1 = mod3(x), r2 = rs;
+            const uint32_t any_in = (L.summ[r0][irec] & 2u) | (L.summ[r1][irec - kRZ] & 8u) |
+                                    (L.summ[r1][irec - 1] & 32u) | (L.summ[r1][irec + 1] & 16u) |
+                                    (L.summ[r1][irec + kRZ] & 4u) | (L.summ[r2][irec] & 1u);
+            slow = count_nonzero(cur) != 0u || any_in != 0u || t.has_zero_order != 0u;
+            prev_v = uint32_t(Z) + uint32_t(Y) * a.R2 + uint32_t(x) * a.R12;
+            prev_ok = true;
+            prev_slow = slow;
+            if (!slow)  // empty, nothing arrives: only the zero-propensity draw at seed 0 "fires" (a no-op)
+                n_fired += (a.itime + uint32_t(x) * uint32_t(Y) + uint32_t(Z)) == 0u;
+        }
+        wave_append(slow, uint32_t(lane), &L.n_slow, L.slow_list);
+        if (lane == 0)
+            L.n_hop = 0u;
+        cur = mine;
+        __syncthreads();
+
+        // ---- T3: full gather + reaction of the listed voxels
+        {
+            const uint32_t ns = L.n_slow;
+            const int r0 = mod3(x - 1), r1 = mod3(x), r2 = rs;
+            for (uint32_t i = uint32_t(lane); i < ns; i += kBlock) {
+                const int vl = int(L.slow_list[i]);
+                const int vy = vl / kTZ, vz = vl % kTZ;
+                const int vr = rec_of(vy, vz);
+                const uint32_t gy = uint32_t(y0 + vy), gz = uint32_t(z0 + vz);
+                Slots<S> own;
+#pragma unroll
+                for (int q = 0; q < S / 4; ++q)
+                    own.w[q] = L.slots[r1][vr][q];
+                const uint64_t own_codes = L.codes[r1][vr];
+                const uint32_t occ = count_nonzero(own);
+                uint64_t stay = 0ull;
+                for (uint32_t s = 0; s < occ; ++s)
+                    if (((own_codes >> (4u * s)) & 0xFull) >= 6u)
+                        stay |= 1ull << s;
+                Slots<S> lo = zero_slots<S>(), hi = zero_slots<S>();
+                uint32_t nlo = 0, nhi = 0;
+                if (L.summ[r0][vr] & 2u) nlo = take_arrivals<S>(L.slots[r0][vr], L.codes[r0][vr], 1u, lo, nlo);
+                if (L.summ[r1][vr - kRZ] & 8u) nlo = take_arrivals<S>(L.slots[r1][vr - kRZ], L.codes[r1][vr - kRZ], 3u, lo, nlo);
+                if (L.summ[r1][vr - 1] & 32u) nlo = take_arrivals<S>(L.slots[r1][vr - 1], L.codes[r1][vr - 1], 5u, lo, nlo);
+                if (L.summ[r1][vr + 1] & 16u) nhi = take_arrivals<S>(L.slots[r1][vr + 1], L.codes[r1][vr + 1], 4u, hi, nhi);
+                if (L.summ[r1][vr + kRZ] & 4u) nhi = take_arrivals<S>(L.slots[r1][vr + kRZ], L.codes[r1][vr + kRZ], 2u, hi, nhi);
+                if (L.summ[r2][vr] & 1u) nhi = take_arrivals<S>(L.slots[r2][vr], L.codes[r2][vr], 0u, hi, nhi);
+                const uint32_t V = gz + gy * a.R2 + uint32_t(x) * a.R12;
+                if (occ + nlo + nhi > uint32_t(S)) {
+                    ++n_tight;
+                    const uint32_t k = atomicAdd(&L.n_tight, 1u);
+                    if (k < uint32_t(kTightLds))
+                        L.tight[k] = V;
+                    else
+                        note_tight(a, V);  // block list full: straight to the global list
+                }
+                uint32_t cnt = 0;
+                Slots<S> out = assemble<S>(own, occ, stay, lo, nlo, hi, nhi, cnt);
+                bool fired, rolled, near, bad;
+                react<S>(a, tb, out, cnt, uint32_t(x), gy, gz, fired, rolled, near, bad);
+                n_fired += fired;
+                n_rolled += rolled;
+                n_near += near;
+                n_bad += bad;
+#pragma unroll
+                for (int q = 0; q < S / 4; ++q)
+                    L.out[vl][q] = out.w[q];
+            }
+        }
+        __syncthreads();
+    }
+    if (prev_ok) {
+        Slots<S> o = zero_slots<S>();
+        if (prev_slow) {
+#pragma unroll
+            for (int q = 0; q < S / 4; ++q)
+                o.w[q] = L.out[lane][q];
+        }
+        store_slots<S>(a.dst, prev_v, o);
+    }
+    // the list is stable: the last barrier of the loop follows every append
+    const uint32_t nt = min(L.n_tight, uint32_t(kTightLds));
+    for (uint32_t i = uint32_t(lane); i < nt; i += kBlock)
+        note_tight(a, L.tight[i]);
+    const uint32_t vals[5] = {n_tight, n_bad, n_near, n_fired, n_rolled};
+    const int idx[5] = {CS_CNT_TIGHT, CS_CNT_EXCEPTION, CS_CNT_NEAR_THRESHOLD, CS_CNT_FIRED, CS_CNT_ROLLBACK};
+#pragma unroll
+    for (int k = 0; k < 5; ++k)
+        if (vals[k])
+            atomicAdd(a.counters + idx[k], (unsigned long long)vals[k]);
 }
 
 // Per-region species histogram of the lattice (statistics; the reference keeps global counts only,
@@ -512,27 +1179,57 @@ Device::Device(const Model &m, int hip_device) : dev_(hip_device)
     std::vector<uint8_t> reg8(N_);
     for (uint64_t v = 0; v < N_; ++v)
         reg8[v] = uint8_t(m.regions()[v]);
+    for (uint32_t w = 0; w < nrw_; ++w)
+        has_zero_order_ = has_zero_order_ || rmask[w] != 0ull;
+    if (const char *k = std::getenv("CS_STEP_KERNEL"))
+        simple_ = std::string(k) == "simple";
+    if (const char *x = std::getenv("CS_TILE_PLANES")) {
+        const long v = std::strtol(x, nullptr, 10);
+        if (v >= 1 && v <= 4096)
+            xc_ = uint32_t(v);
+    }
 
     hip_check(hipMalloc(&lat_[0], N_ * S_), "hipMalloc lattice");
     hip_check(hipMalloc(&lat_[1], N_ * S_), "hipMalloc lattice");
     hip_check(hipMalloc(&reg_, N_), "hipMalloc regions");
-    hip_check(hipMalloc(&p2_, p2.size() * sizeof(float)), "hipMalloc p2");
-    hip_check(hipMalloc(&rates_, m.rates().size() * sizeof(float)), "hipMalloc rates");
-    hip_check(hipMalloc(&ridx_, ridx.size()), "hipMalloc ridx");
-    hip_check(hipMalloc(&stoich_, st8.size()), "hipMalloc stoich");
-    hip_check(hipMalloc(&rmask_, rmask.size() * sizeof(uint64_t)), "hipMalloc rmask");
+
+    // All tables in one blob (8-byte aligned sections): the tiled kernel copies it into LDS as is.
+    auto align8 = [](size_t n) { return (n + 7u) & ~size_t(7); };
+    off_p2_ = 0;
+    off_rates_ = uint32_t(align8(off_p2_ + p2.size() * sizeof(float)));
+    off_ridx_ = uint32_t(align8(off_rates_ + m.rates().size() * sizeof(float)));
+    off_rmask_ = uint32_t(align8(off_ridx_ + ridx.size()));
+    off_stoich_ = uint32_t(align8(off_rmask_ + rmask.size() * sizeof(uint64_t)));
+    const size_t blob_bytes = align8(off_stoich_ + st8.size());
+    tab_words_ = uint32_t(blob_bytes / 4u);
+    std::vector<uint8_t> blob(blob_bytes, 0);
+    std::memcpy(blob.data() + off_p2_, p2.data(), p2.size() * sizeof(float));
+    std::memcpy(blob.data() + off_rates_, m.rates().data(), m.rates().size() * sizeof(float));
+    std::memcpy(blob.data() + off_ridx_, ridx.data(), ridx.size());
+    std::memcpy(blob.data() + off_rmask_, rmask.data(), rmask.size() * sizeof(uint64_t));
+    std::memcpy(blob.data() + off_stoich_, st8.data(), st8.size());
+    hip_check(hipMalloc(&tab_, blob_bytes), "hipMalloc tables");
+    uint8_t *tb = reinterpret_cast<uint8_t *>(tab_);
+    p2_ = reinterpret_cast<float *>(tb + off_p2_);
+    rates_ = reinterpret_cast<float *>(tb + off_rates_);
+    ridx_ = tb + off_ridx_;
+    rmask_ = reinterpret_cast<uint64_t *>(tb + off_rmask_);
+    stoich_ = reinterpret_cast<int8_t *>(tb + off_stoich_);
+    if (blob_bytes > kTabLdsMax)
+        simple_ = true;  // tables too large to stage in LDS next to the plane records
     hip_check(hipMalloc(&counters_, kNumCounters * sizeof(unsigned long long)), "hipMalloc counters");
     hip_check(hipMalloc(&hist_, size_t(nreg_) * n1 * sizeof(int32_t)), "hipMalloc hist");
+    hip_check(hipMalloc(&fix_ctl_, 4 * sizeof(uint32_t)), "hipMalloc fix_ctl");
+    hip_check(hipMalloc(&fix_list_, size_t(kMaxTight) * sizeof(uint32_t)), "hipMalloc fix_list");
+    hip_check(hipMalloc(&fix_F_, size_t(8 * kMaxTight) * sizeof(uint32_t)), "hipMalloc fix_F");
+    hip_check(hipMalloc(&fix_W_, size_t(56 * kMaxTight) * sizeof(uint32_t)), "hipMalloc fix_W");
+    hip_check(hipMalloc(&fix_L_, size_t(8 * kMaxTight) * S_), "hipMalloc fix_L");
+    hip_check(hipMalloc(&fix_O_, size_t(8 * kMaxTight) * sizeof(uint32_t)), "hipMalloc fix_O");
+    hip_check(hipMemsetAsync(fix_ctl_, 0, 4 * sizeof(uint32_t), st), "clear fix_ctl");
 
     hip_check(hipMemcpyAsync(lat_[0], m.lattice().data(), N_ * S_, hipMemcpyHostToDevice, st), "upload lattice");
     hip_check(hipMemcpyAsync(reg_, reg8.data(), N_, hipMemcpyHostToDevice, st), "upload regions");
-    hip_check(hipMemcpyAsync(p2_, p2.data(), p2.size() * sizeof(float), hipMemcpyHostToDevice, st), "upload p2");
-    hip_check(hipMemcpyAsync(rates_, m.rates().data(), m.rates().size() * sizeof(float), hipMemcpyHostToDevice, st),
-              "upload rates");
-    hip_check(hipMemcpyAsync(ridx_, ridx.data(), ridx.size(), hipMemcpyHostToDevice, st), "upload ridx");
-    hip_check(hipMemcpyAsync(stoich_, st8.data(), st8.size(), hipMemcpyHostToDevice, st), "upload stoich");
-    hip_check(hipMemcpyAsync(rmask_, rmask.data(), rmask.size() * sizeof(uint64_t), hipMemcpyHostToDevice, st),
-              "upload rmask");
+    hip_check(hipMemcpyAsync(tab_, blob.data(), blob_bytes, hipMemcpyHostToDevice, st), "upload tables");
     hip_check(hipMemsetAsync(counters_, 0, kNumCounters * sizeof(unsigned long long), st), "clear counters");
     hip_check(hipStreamSynchronize(st), "upload");
     cur_ = 0;
@@ -546,13 +1243,15 @@ Device::~Device()
     hipFree(lat_[0]);
     hipFree(lat_[1]);
     hipFree(reg_);
-    hipFree(p2_);
-    hipFree(rates_);
-    hipFree(ridx_);
-    hipFree(stoich_);
-    hipFree(rmask_);
+    hipFree(tab_);
     hipFree(counters_);
     hipFree(hist_);
+    hipFree(fix_ctl_);
+    hipFree(fix_list_);
+    hipFree(fix_F_);
+    hipFree(fix_W_);
+    hipFree(fix_L_);
+    hipFree(fix_O_);
     if (stream_)
         hipStreamDestroy(static_cast<hipStream_t>(stream_));
 }
@@ -569,6 +1268,13 @@ void Device::step(uint32_t itime)
     a.stoich = stoich_;
     a.rmask = rmask_;
     a.counters = counters_;
+    a.fix_ctl = fix_ctl_;
+    a.fix_list = fix_list_;
+    a.fix_F = fix_F_;
+    a.fix_W = fix_W_;
+    a.fix_L = fix_L_;
+    a.fix_O = fix_O_;
+    a.max_tight = kMaxTight;
     a.R0 = res_[0];
     a.R1 = res_[1];
     a.R2 = res_[2];
@@ -581,15 +1287,45 @@ void Device::step(uint32_t itime)
     a.itime = itime;
     a.mod_zero = mod_zero_;
     a.tau = tau_;
-    const uint32_t blocks = uint32_t((N_ + kBlock - 1) / kBlock);
     hipStream_t st = static_cast<hipStream_t>(stream_);
-    switch (S_) {
-    case 4: hipLaunchKernelGGL(rdme_step_kernel<4>, dim3(blocks), dim3(kBlock), 0, st, a); break;
-    case 8: hipLaunchKernelGGL(rdme_step_kernel<8>, dim3(blocks), dim3(kBlock), 0, st, a); break;
-    case 16: hipLaunchKernelGGL(rdme_step_kernel<16>, dim3(blocks), dim3(kBlock), 0, st, a); break;
-    default: throw Error(CS_ERR_UNSUPPORTED, "slots per voxel must be 4, 8 or 16");
+    if (simple_) {
+        const uint32_t blocks = uint32_t((N_ + kBlock - 1) / kBlock);
+        switch (S_) {
+        case 4: hipLaunchKernelGGL(rdme_step_kernel<4>, dim3(blocks), dim3(kBlock), 0, st, a); break;
+        case 8: hipLaunchKernelGGL(rdme_step_kernel<8>, dim3(blocks), dim3(kBlock), 0, st, a); break;
+        case 16: hipLaunchKernelGGL(rdme_step_kernel<16>, dim3(blocks), dim3(kBlock), 0, st, a); break;
+        default: throw Error(CS_ERR_UNSUPPORTED, "slots per voxel must be 4, 8 or 16");
+        }
+    } else {
+        TileArgs t;
+        t.tiles_z = (res_[2] + kTZ - 1) / kTZ;
+        t.tiles_y = (res_[1] + kTY - 1) / kTY;
+        t.xc = xc_;
+        t.chunks = (res_[0] + xc_ - 1) / xc_;
+        t.has_zero_order = has_zero_order_ ? 1u : 0u;
+        t.tab_words = tab_words_;
+        t.off_p2 = off_p2_;
+        t.off_rates = off_rates_;
+        t.off_ridx = off_ridx_;
+        t.off_stoich = off_stoich_;
+        t.off_rmask = off_rmask_;
+        t.tab = static_cast<const uint32_t *>(tab_);
+        const uint32_t blocks = t.tiles_z * t.tiles_y * t.chunks;
+        const size_t shm = size_t(tab_words_) * 4u;
+        switch (S_) {
+        case 4: hipLaunchKernelGGL(rdme_step_tiled<4>, dim3(blocks), dim3(kBlock), shm, st, a, t); break;
+        case 8: hipLaunchKernelGGL(rdme_step_tiled<8>, dim3(blocks), dim3(kBlock), shm, st, a, t); break;
+        case 16: hipLaunchKernelGGL(rdme_step_tiled<16>, dim3(blocks), dim3(kBlock), shm, st, a, t); break;
+        default: throw Error(CS_ERR_UNSUPPORTED, "slots per voxel must be 4, 8 or 16");
+        }
     }
-    hip_check(hipGetLastError(), "rdme_step_kernel launch");
+    hip_check(hipGetLastError(), "rdme step launch");
+    switch (S_) {
+    case 4: hipLaunchKernelGGL(tight_replay_kernel<4>, dim3(1), dim3(64), 0, st, a); break;
+    case 8: hipLaunchKernelGGL(tight_replay_kernel<8>, dim3(1), dim3(64), 0, st, a); break;
+    default: hipLaunchKernelGGL(tight_replay_kernel<16>, dim3(1), dim3(64), 0, st, a); break;
+    }
+    hip_check(hipGetLastError(), "tight replay launch");
     cur_ ^= 1;
 }
 

@@ -47,9 +47,15 @@ private:
     float tau_ = 0.f;
     uint32_t mod_zero_ = 0xFFFFFFFFu;
     int cur_ = 0;                  // index of the buffer holding the current lattice
+    bool has_zero_order_ = false;  // some reaction has no reactants
+    bool simple_ = false;          // one-lane-per-voxel kernel without LDS staging
+    uint32_t xc_ = 16;             // planes marched per block by the tiled kernel
 
     uint8_t *lat_[2] = {nullptr, nullptr};
     uint8_t *reg_ = nullptr;
+    void *tab_ = nullptr;          // one blob with every table below, at the byte offsets off_*
+    uint32_t tab_words_ = 0;
+    uint32_t off_p2_ = 0, off_rates_ = 0, off_ridx_ = 0, off_rmask_ = 0, off_stoich_ = 0;
     float *p2_ = nullptr;          // (D*tau)/(lambda*lambda), [src][dst][sp]
     float *rates_ = nullptr;       // [nr+1]
     uint8_t *ridx_ = nullptr;      // [nr+1][4]
@@ -57,6 +63,12 @@ private:
     uint64_t *rmask_ = nullptr;    // [ns+1][nrw]: reactions having the species as a reactant; row 0: zero-order
     unsigned long long *counters_ = nullptr;
     int32_t *hist_ = nullptr;      // [nreg][ns+1]
+    uint32_t *fix_ctl_ = nullptr;  // tight-voxel replay: control words, lists and slot copies
+    uint32_t *fix_list_ = nullptr;
+    uint32_t *fix_F_ = nullptr;
+    uint32_t *fix_W_ = nullptr;
+    uint8_t *fix_L_ = nullptr;
+    uint32_t *fix_O_ = nullptr;
 };
 
 }  // namespace cs

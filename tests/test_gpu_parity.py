@@ -32,9 +32,9 @@ def run_parity(sim, orc, itimes, check_every, tag):
         sim.run(done, itimes[done:done + n])
         orc.run(itimes[done:done + n])
         done += n
-        tight = int(sim.counters()[CS_CNT_TIGHT])
-        assert tight == 0, f"{tag}: {tight} tight voxel-steps by step {done}; the exact gather does not apply"
         _compare(sim, orc, f"{tag} after {done} steps")
+    # tight voxels (if any) were replayed in canonical order; only a slot-S alias is unrepresentable
+    assert orc.counters.alias_writes == 0, "choose a case without slot-S aliases for a multi-step comparison"
     c = sim.counters()
     assert c[CS_CNT_EXCEPTION] == 0
     assert c[CS_CNT_NEAR_THRESHOLD] == orc.counters.near_threshold
@@ -172,6 +172,112 @@ def test_region_counts_match_lattice(oracle_mod):
     got[:, 0] = 0
     assert np.array_equal(got, exp)
     assert np.array_equal(sim.species_counts()[1:], exp.sum(axis=0)[1:])
+
+
+@pytest.mark.parametrize("kernel", ["tiled", "simple"])
+def test_dense_capacity_replay(oracle_mod, kernel):
+    """S=4 at high density: claims into full voxels are rejected or alias into the next voxel's slot 0
+    (rdme.wgsl:30-40, 62). The tight neighbourhoods are replayed in canonical order, so the lattice stays
+    bit-exact step by step; the comparison runs up to and including the first alias, after which the
+    reference's occupancy (S+1 with S slots) is outside the byte-slot state."""
+    import os
+
+    for walled, dens, nsteps in ((False, 1.4, 30), (True, 0.7, 40)):
+        announce(f"dense capacity replay ({kernel}), 20^3 S=4 density {dens} wall={walled}, step by step")
+        os.environ["CS_STEP_KERNEL"] = kernel
+        try:
+            sim = _walled_model(20, 4, dens, seed=31) if walled else _dense_model(20, 4, dens, seed=31)
+            orc = oracle_mod.Oracle.from_simulation(sim)
+            sim.prepare_for_gpu(0)
+        finally:
+            os.environ.pop("CS_STEP_KERNEL", None)
+        compared = 0
+        for t, it in enumerate(default_itime_schedule(31, nsteps)):
+            sim.step(t, int(it), 0)
+            orc.step(int(it))
+            _compare(sim, orc, f"dense replay step {t}")
+            compared += 1
+            if orc.counters.alias_writes:
+                break
+        c = sim.counters()
+        assert c[CS_CNT_TIGHT] > 0 and orc.counters.failed_claims > 0, "the case must exercise the replay"
+        if not orc.counters.alias_writes:
+            assert compared == nsteps
+            assert c[CS_CNT_EXCEPTION] == 0
+            assert c[CS_CNT_FIRED] == orc.counters.fired
+            assert c[CS_CNT_NEAR_THRESHOLD] == orc.counters.near_threshold
+        print(f"compared {compared} steps; oracle {orc.counters.as_dict()}; tight {int(c[CS_CNT_TIGHT])}")
+
+
+def _walled_model(edge, slots, density, seed):
+    """Every voxel's slot 0 holds an immobile, unreactive species, so a claim into a full voxel finds the
+    next voxel's slot 0 taken and is rejected (never the slot-S alias)."""
+    lam = 32e-9
+    sim = Simulation(LatticeParams([edge * lam] * 3, [edge] * 3, 50e-6, lam, slots))
+    sim.set_seed(seed)
+    sim.add_region(RegionType.Cube("left", [0, 0, 0], [edge * lam / 2, edge * lam, edge * lam]), 4e-12)
+    sim.prepare_regions()
+    sim.fill_region("W", "background", False)
+    sim.fill_region("W2", "left", False)  # "left" voxels: fill_region targets a region, so fill it too
+    n = int(density * edge ** 3 / 2)
+    sim.add_particle_count("A", "background", n, True, False)
+    sim.add_particle_count("B", "left", n // 2, True, False)
+    sim.set_diffusion_rate_particle("W", "background", 0.0)
+    sim.set_diffusion_rate_particle("W", "left", 0.0)
+    sim.set_diffusion_rate_particle("W2", "left", 0.0)
+    sim.set_diffusion_rate_particle("W2", "background", 0.0)
+    sim.set_diffusion_rate_particle("A", "background", 6e-12)
+    sim.set_transition_rate_particle("A", "background", "left", 3e-12)
+    sim.set_transition_rate_particle("B", "left", "background", 3e-12)
+    sim.add_reaction(["A", "B"], ["B", "B"], 40.0)
+    sim.add_reaction(["B"], ["A"], 8.0)
+    return sim
+
+
+def _run_with_env(env, build, seed, nsteps):
+    import os
+
+    saved = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
+    try:
+        sim = build()
+        sim.prepare_for_gpu(0)
+    finally:
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    sim.run(0, default_itime_schedule(seed, nsteps))
+    return sim.lattice(), sim.counters()
+
+
+@pytest.mark.parametrize("planes", ["1", "3", "16", "1000"])
+def test_tiled_kernel_matches_simple_kernel(planes):
+    """The LDS-tiled kernel and the one-lane-per-voxel kernel give identical lattices for any chunk length
+    along x, on a lattice whose y and z extents are not multiples of the tile (4 x 64)."""
+    announce(f"tiled (planes={planes}) vs simple, 37x30x70 S=8, 40 steps")
+
+    def build():
+        lam = 32e-9
+        sim = Simulation(LatticeParams([37 * lam, 30 * lam, 70 * lam], [37, 30, 70], 50e-6, lam, 8))
+        sim.set_seed(21)
+        sim.add_region(RegionType.Sphere("ball", [18 * lam, 15 * lam, 35 * lam], 12 * lam), 2e-12)
+        sim.prepare_regions()
+        sim.add_particle_count("A", "background", 9000, True, False)
+        sim.add_particle_count("B", "ball", 4000, True, False)
+        sim.add_particle_count("C", "ball", 1000, False, False)
+        sim.set_diffusion_rate_particle("A", "background", 4e-12)
+        sim.set_transition_rate_particle("A", "background", "ball", 1e-12)
+        sim.set_transition_rate_particle("B", "ball", "background", 2e-12)
+        sim.add_reaction(["A", "B"], ["C"], 80.0)
+        sim.add_reaction(["C"], ["A", "B"], 9.0)
+        return sim
+
+    a, ca = _run_with_env({"CS_STEP_KERNEL": "tiled", "CS_TILE_PLANES": planes}, build, 21, 40)
+    b, cb = _run_with_env({"CS_STEP_KERNEL": "simple"}, build, 21, 40)
+    assert np.array_equal(a, b)
+    assert np.array_equal(ca[:5], cb[:5])
 
 
 def test_large_cell_256_first_steps(oracle_mod):
