@@ -29,6 +29,22 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def calibrate_copy(mib: int, steps: int) -> int:
+    """Device-to-device copy of `mib` MiB, `steps` times: reads and writes exactly mib MiB per launch."""
+    import torch
+
+    n = mib << 20
+    src = torch.randint(0, 255, (n,), dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(steps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "copy", "bytes_read_per_launch": n, "bytes_written_per_launch": n,
+                      "launches": steps}), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -38,7 +54,12 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps for the CPU baseline (0 = skip)")
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "large_cell_256_traffic.json"),
                     help="per-launch HBM bytes measured with rocprofv3 --pmc (FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="instead of the step: copy a MIB-MiB device buffer --steps times (a kernel of known "
+                         "bytes for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        return calibrate_copy(args.calibrate_copy, args.steps)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on exactly one GPU: the RDME step is benchmarked per device and this build "
@@ -117,7 +138,7 @@ def main() -> int:
                    "voxels": N, "slots_per_voxel": S, "path": "plain"},
         "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "rdme_step_kernel", "bytes_per_voxel_step": b_alg,
+                     "kernel": "rdme_step_tiled", "bytes_per_voxel_step": b_alg,
                      "avg_kernel_us": avg_kernel_s * 1e6},
         "cpu_baseline": cpu,
         "diagnostics": {"tight_voxel_steps": int(counters[0]), "exceptions": int(counters[1]),

@@ -18,6 +18,7 @@
 // the capacity hazards of rdme.wgsl:57-87 (rejected claims, the slot-S alias) change the outcome.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -51,6 +52,7 @@ struct StepArgs {
     const uint8_t *ridx;
     const int8_t *stoich;
     const uint64_t *rmask;
+    const float *bound;
     unsigned long long *counters;
     uint32_t *fix_ctl;    // [0] tight voxels noted this step, [1] blocks finished
     uint32_t *fix_list;   // tight voxels (max_tight)
@@ -216,11 +218,12 @@ struct Tab {
     const uint32_t *ridx;   // [nr+1] three reactant species, one byte each
     const int8_t *stoich;   // [nr+1][ns+1]
     const uint64_t *rmask;  // [ns+1][nrw] reactions having the species as a reactant; row 0: zero-order
+    const float *bound;     // [S+1] upper bound of the total propensity of a voxel holding c particles
 };
 
 __device__ __forceinline__ Tab global_tab(const StepArgs &a)
 {
-    return Tab{a.p2, a.rates, reinterpret_cast<const uint32_t *>(a.ridx), a.stoich, a.rmask};
+    return Tab{a.p2, a.rates, reinterpret_cast<const uint32_t *>(a.ridx), a.stoich, a.rmask, a.bound};
 }
 
 // Direction 0..5 of an existing neighbour the particle hops to, or 6 if it stays (stay draw or
@@ -305,6 +308,10 @@ __device__ __forceinline__ void react(const StepArgs &a, const Tab &tb, Slots<S>
     fired = rolled = near = bad = false;
     const uint32_t P = pcg_hash(a.itime + x * y + z);
     const float r1 = uniform_float(P);
+    // The total propensity never exceeds bound[cnt], and f32 products and sums are monotone, so a draw
+    // above the bound's reject line is above the exact one below as well: nothing fires, nothing is near.
+    if (r1 > tb.bound[cnt] * a.tau * 1.001f + 1e-6f)
+        return;
 
     uint64_t cand[kMaxRW];
 #pragma unroll
@@ -692,9 +699,15 @@ __global__ __launch_bounds__(kBlock) void rdme_step_kernel(StepArgs a)
 // LDS with a one-voxel halo in y and z ("records": slots, per-slot hop codes, a summary of hop
 // directions), so every particle's hop is computed once per block instead of once per neighbour, and the
 // gather reads neighbour records from LDS. Region labels are staged with a two-voxel halo (a halo
-// record's hop needs its own neighbours' labels). Three planes of records are kept (x-1, x, x+1); the
-// slots and labels of the next plane are loaded into registers one iteration ahead, so their HBM
-// latency overlaps the current plane's work. The model tables live in LDS for the whole block.
+// record's hop needs its own neighbours' labels). The slots and labels of the next plane are loaded into
+// registers one iteration ahead, so their HBM latency overlaps the current plane's work. The model tables
+// live in LDS for the whole block.
+//
+// Particles are sparse, so the per-particle work runs from lists: iteration p stages plane p and lists
+// its records that hold particles; the hops of those records set a "needs gather" flag on every voxel a
+// particle enters. The gather runs two planes behind (plane p-2), when every flag it depends on is final,
+// so the hops of plane p and the gather of plane p-2 run in the same phase on different waves: hop items
+// are taken from the first lanes up, gather items from the last lanes down. Two barriers per plane.
 constexpr int kTZ = 64;                    // one wave per z-row
 constexpr int kTY = 4;                     // four rows: 256 lanes
 constexpr int kRZ = kTZ + 2;               // record row with z halo
@@ -710,20 +723,23 @@ struct TileArgs {
     uint32_t tiles_z, tiles_y, chunks, xc;
     uint32_t has_zero_order;               // a reaction without reactants exists (fires in empty voxels)
     uint32_t tab_words;                    // size of the table blob in 32-bit words
-    uint32_t off_p2, off_rates, off_ridx, off_stoich, off_rmask;  // byte offsets in the blob
+    uint32_t off_p2, off_rates, off_ridx, off_stoich, off_rmask, off_bound;  // byte offsets in the blob
     const uint32_t *tab;                   // the blob in global memory
 };
 
+constexpr int kRing = 4;                    // planes of records: the gather of p-2 reads p-3..p-1 while p is staged
+
 template <int S>
 struct TileLds {
-    uint32_t slots[3][kRec][S / 4];
-    uint64_t codes[3][kRec];
-    uint8_t summ[3][kRec];
+    uint32_t slots[kRing][kRec][S / 4];
+    uint64_t codes[kRing][kRec];     // valid for the records listed for hops
+    uint8_t summ[kRing][kRec];       // bit d: some particle of the record hops in direction d
+    uint8_t need[kRing][kTY * kTZ];  // interior voxel holds particles or receives one
     uint8_t reg[3][kGrid];
-    uint32_t out[kTY * kTZ][S / 4];  // gathered output of this plane's full-path voxels
+    uint32_t out[kTY * kTZ][S / 4];  // gathered output of the listed voxels of one plane
     uint16_t hop_list[kRec];         // records of the staged plane that hold particles
-    uint16_t slow_list[kTY * kTZ];   // interior voxels that need the full gather
-    uint32_t n_hop, n_slow, n_tight;
+    uint16_t slow_list[kTY * kTZ];   // interior voxels of the gathered plane that need the full gather
+    uint32_t n_hop[2], n_slow[2], n_tight;
     uint32_t tight[kTightLds];
 };
 
@@ -754,16 +770,6 @@ __device__ __forceinline__ void halo_coords(int t, int &yy, int &zz)
     else if (t < 2 * kRZ) { yy = kTY; zz = t - kRZ - 1; }
     else if (t < 2 * kRZ + kTY) { yy = t - 2 * kRZ; zz = -1; }
     else { yy = t - 2 * kRZ - kTY; zz = kTZ; }
-}
-
-// Slots of (X, Y, Z) loaded without a branch: outside the lattice the load reads voxel 0 and `ok` is
-// cleared, to be applied (masked) when the value is used.
-template <int S>
-__device__ __forceinline__ Slots<S> load_slots_clamped(const StepArgs &a, int X, int Y, int Z, bool &ok)
-{
-    ok = X >= 0 && Y >= 0 && Z >= 0 && uint32_t(X) < a.R0 && uint32_t(Y) < a.R1 && uint32_t(Z) < a.R2;
-    const uint32_t v = ok ? uint32_t(Z) + uint32_t(Y) * a.R2 + uint32_t(X) * a.R12 : 0u;
-    return load_slots<S>(a.src, v);
 }
 
 template <int S>
@@ -840,13 +846,6 @@ __device__ __forceinline__ uint32_t take_arrivals(const uint32_t *words, uint64_
     return n;
 }
 
-// One plane of the march runs in four phases separated by barriers:
-//   T0 stage the plane's slots and the next labels in LDS; list the records that hold particles;
-//   T1 compute the hops of the listed records (consecutive lanes, so ~one wave does the sparse work);
-//      store the previous plane's output;
-//   T2 for the plane behind, find the voxels that need the full gather (particles present or arriving);
-//   T3 run the full gather + reaction for the listed voxels, results into LDS.
-// Voxels with no particle and no arrival are written as zeros by their own lane without entering T3.
 template <int S>
 __global__ __launch_bounds__(kBlock) void rdme_step_tiled(StepArgs a, TileArgs t)
 {
@@ -869,6 +868,22 @@ __global__ __launch_bounds__(kBlock) void rdme_step_tiled(StepArgs a, TileArgs t
     const int hrec = has_halo ? rec_of(hy, hz) : 0;
     const int Y = y0 + iy, Z = z0 + iz;
     const bool in_lattice = uint32_t(Y) < a.R1 && uint32_t(Z) < a.R2;
+    // (y, z) parts of this lane's global offsets. A voxel outside the lattice is loaded from offset 0 of
+    // its plane (in bounds) and masked, so no load sits under a branch.
+    const bool hal_in = has_halo && y0 + hy >= 0 && uint32_t(y0 + hy) < a.R1 && z0 + hz >= 0 &&
+                        uint32_t(z0 + hz) < a.R2;
+    const uint32_t mine_yz = in_lattice ? uint32_t(Z) + uint32_t(Y) * a.R2 : 0u;
+    const uint32_t hal_yz = hal_in ? uint32_t(z0 + hz) + uint32_t(y0 + hy) * a.R2 : 0u;
+    uint32_t lab_yz[kLabPerLane];
+    uint32_t lab_ok = 0u;
+#pragma unroll
+    for (int k = 0; k < kLabPerLane; ++k) {
+        const int i = lane + k * kBlock;
+        const int gy = y0 + i / kGZ - 2, gz = z0 + i % kGZ - 2;
+        const bool ok = i < kGrid && gy >= 0 && uint32_t(gy) < a.R1 && gz >= 0 && uint32_t(gz) < a.R2;
+        lab_yz[k] = ok ? uint32_t(gz) + uint32_t(gy) * a.R2 : 0u;
+        lab_ok |= uint32_t(ok) << k;
+    }
 
     for (uint32_t i = uint32_t(lane); i < t.tab_words; i += kBlock)
         tab_lds[i] = t.tab[i];
@@ -876,26 +891,25 @@ __global__ __launch_bounds__(kBlock) void rdme_step_tiled(StepArgs a, TileArgs t
     const Tab tb{reinterpret_cast<const float *>(tbase + t.off_p2), reinterpret_cast<const float *>(tbase + t.off_rates),
                  reinterpret_cast<const uint32_t *>(tbase + t.off_ridx),
                  reinterpret_cast<const int8_t *>(tbase + t.off_stoich),
-                 reinterpret_cast<const uint64_t *>(tbase + t.off_rmask)};
-    if (lane == 0) {
-        L.n_hop = 0u;
-        L.n_slow = 0u;
-        L.n_tight = 0u;
+                 reinterpret_cast<const uint64_t *>(tbase + t.off_rmask),
+                 reinterpret_cast<const float *>(tbase + t.off_bound)};
+    if (lane < 2) {
+        L.n_hop[lane] = 0u;
+        L.n_slow[lane] = 0u;
     }
+    if (lane == 0)
+        L.n_tight = 0u;
 
-    // Label loads are unconditional from a clamped address into a full register; whether the value is
-    // real is decided when it is stored a plane later. A load under a branch, merged into a packed byte,
-    // makes the compiler wait for every outstanding load (the slot prefetch included) on the spot.
+    auto plane_base = [&](int X) { return (X >= 0 && X < int(a.R0)) ? uint32_t(X) * a.R12 : 0u; };
+    // Label loads are unconditional into full registers; whether a value is real is decided when it is
+    // stored a plane later. (A load under a branch, merged into a packed byte, makes the compiler wait for
+    // every outstanding load, the slot prefetch included, on the spot.)
     auto fetch_labels = [&](int X, uint32_t (&r)[kLabPerLane]) {
+        const bool xo = X >= 0 && X < int(a.R0);
+        const uint32_t base = plane_base(X);
 #pragma unroll
-        for (int k = 0; k < kLabPerLane; ++k) {
-            const int i = lane + k * kBlock;
-            const int gy = y0 + i / kGZ - 2, gz = z0 + i % kGZ - 2;
-            const bool ok = i < kGrid && X >= 0 && uint32_t(X) < a.R0 && gy >= 0 && uint32_t(gy) < a.R1 &&
-                            gz >= 0 && uint32_t(gz) < a.R2;
-            const uint32_t idx = ok ? uint32_t(gz) + uint32_t(gy) * a.R2 + uint32_t(X) * a.R12 : 0u;
-            r[k] = uint32_t(a.reg[idx]) | (ok ? 0u : 0x100u);  // bit 8: not a lattice voxel
-        }
+        for (int k = 0; k < kLabPerLane; ++k)
+            r[k] = uint32_t(a.reg[base + lab_yz[k]]) | ((xo && ((lab_ok >> k) & 1u)) ? 0u : 0x100u);
     };
     auto store_labels = [&](int slot, const uint32_t (&r)[kLabPerLane]) {
 #pragma unroll
@@ -904,6 +918,7 @@ __global__ __launch_bounds__(kBlock) void rdme_step_tiled(StepArgs a, TileArgs t
                 L.reg[slot][lane + k * kBlock] = (r[k] & 0x100u) ? uint8_t(0) : uint8_t(r[k]);
     };
     auto mod3 = [](int p) { return (p + 3) % 3; };
+    auto ring = [](int p) { return (p + 4) & (kRing - 1); };
 
     {   // labels of planes xb-2 and xb-1 (needed by the hops of plane xb-1)
         uint32_t r[kLabPerLane];
@@ -913,109 +928,108 @@ __global__ __launch_bounds__(kBlock) void rdme_step_tiled(StepArgs a, TileArgs t
         store_labels(mod3(xb - 1), r);
     }
     // registers holding the next plane to stage: its slots and the labels one plane further
-    bool nx_mine_ok, nx_hal_ok;
-    Slots<S> nx_mine = load_slots_clamped<S>(a, xb - 1, Y, Z, nx_mine_ok);
-    Slots<S> nx_hal = load_slots_clamped<S>(a, xb - 1, y0 + hy, z0 + hz, nx_hal_ok);
-    nx_hal_ok = nx_hal_ok && has_halo;
+    bool nx_ok = xb - 1 >= 0;
+    Slots<S> nx_mine = load_slots<S>(a.src, plane_base(xb - 1) + mine_yz);
+    Slots<S> nx_hal = load_slots<S>(a.src, plane_base(xb - 1) + hal_yz);
     uint32_t nx_lab[kLabPerLane];
     fetch_labels(xb, nx_lab);
 
-    Slots<S> cur = zero_slots<S>();  // this lane's interior voxel at the plane being gathered
     uint32_t n_tight = 0, n_bad = 0, n_near = 0, n_fired = 0, n_rolled = 0;
-    uint32_t prev_v = 0;             // output of the previous plane, stored in T1 of the next iteration
-    bool prev_ok = false, prev_slow = false;
+    uint32_t out_v = 0;  // voxel of the plane gathered last iteration, stored at the start of this one
+    bool out_ok = false, out_slow = false;
     __syncthreads();  // list counters are zero before any wave appends
 
-    for (int p = xb - 1; p <= xe; ++p) {
-        const int rs = mod3(p);
-        // ---- T0: stage plane p, list its records with particles
-        const Slots<S> mine = masked(nx_mine, nx_mine_ok), hal = masked(nx_hal, nx_hal_ok);
-        uint32_t lab[kLabPerLane];
+    for (int p = xb - 1; p <= xe + 1; ++p) {
+        const int rp = ring(p), par = p & 1;
+        // ---- T0: store the output of plane p-3; stage plane p; list the work of planes p and p-2
+        if (out_ok) {
+            Slots<S> o = zero_slots<S>();
+            if (out_slow) {
 #pragma unroll
-        for (int k = 0; k < kLabPerLane; ++k)
-            lab[k] = nx_lab[k];
-        // prefetch plane p+1 and the labels of plane p+2 (past the chunk end the loads are clamped and
-        // their results never used)
-        nx_mine = load_slots_clamped<S>(a, p + 1, Y, Z, nx_mine_ok);
-        nx_hal = load_slots_clamped<S>(a, p + 1, y0 + hy, z0 + hz, nx_hal_ok);
-        nx_hal_ok = nx_hal_ok && has_halo;
-        fetch_labels(p + 2, nx_lab);
-        store_labels(mod3(p + 1), lab);
+                for (int q = 0; q < S / 4; ++q)
+                    o.w[q] = L.out[lane][q];
+            }
+            store_slots<S>(a.dst, out_v, o);
+        }
+        if (p <= xe) {
+            const Slots<S> mine = masked(nx_mine, nx_ok && in_lattice), hal = masked(nx_hal, nx_ok && hal_in);
+            uint32_t lab[kLabPerLane];
 #pragma unroll
-        for (int q = 0; q < S / 4; ++q) {
-            L.slots[rs][irec][q] = mine.w[q];
+            for (int k = 0; k < kLabPerLane; ++k)
+                lab[k] = nx_lab[k];
+            // prefetch plane p+1 and the labels of plane p+2 (out of the lattice: clamped, never used)
+            nx_ok = p + 1 < int(a.R0);
+            nx_mine = load_slots<S>(a.src, plane_base(p + 1) + mine_yz);
+            nx_hal = load_slots<S>(a.src, plane_base(p + 1) + hal_yz);
+            fetch_labels(p + 2, nx_lab);
+            store_labels(mod3(p + 1), lab);
+#pragma unroll
+            for (int q = 0; q < S / 4; ++q) {
+                L.slots[rp][irec][q] = mine.w[q];
+                if (has_halo)
+                    L.slots[rp][hrec][q] = hal.w[q];
+            }
+            L.summ[rp][irec] = 0u;
             if (has_halo)
-                L.slots[rs][hrec][q] = hal.w[q];
+                L.summ[rp][hrec] = 0u;
+            bool mine_occ = false, hal_occ = false;
+#pragma unroll
+            for (int q = 0; q < S / 4; ++q) {
+                mine_occ = mine_occ || mine.w[q] != 0u;
+                hal_occ = hal_occ || hal.w[q] != 0u;
+            }
+            L.need[ring(p + 1)][lane] = 0u;  // cleared before the hops of plane p mark it
+            if (mine_occ)
+                L.need[rp][lane] = 1u;
+            wave_append(mine_occ, uint32_t(irec), &L.n_hop[par], L.hop_list);
+            wave_append(has_halo && hal_occ, uint32_t(hrec), &L.n_hop[par], L.hop_list);
         }
-        L.codes[rs][irec] = ~0ull;
-        L.summ[rs][irec] = 0u;
-        if (has_halo) {
-            L.codes[rs][hrec] = ~0ull;
-            L.summ[rs][hrec] = 0u;
-        }
-        wave_append(count_nonzero(mine) != 0u, uint32_t(irec), &L.n_hop, L.hop_list);
-        wave_append(has_halo && count_nonzero(hal) != 0u, uint32_t(hrec), &L.n_hop, L.hop_list);
+        const int x = p - 2;  // plane gathered in this iteration
+        const bool gather_here = x >= xb && x < xe && in_lattice;
+        const bool slow = gather_here && (L.need[ring(x)][lane] != 0u || t.has_zero_order != 0u);
+        out_ok = gather_here;
+        out_slow = slow;
+        out_v = mine_yz + uint32_t(x) * a.R12;
+        if (gather_here && !slow)  // empty, nothing arrives: only the zero-propensity draw at seed 0 "fires"
+            n_fired += (a.itime + uint32_t(x) * uint32_t(Y) + uint32_t(Z)) == 0u;
+        wave_append(slow, uint32_t(lane), &L.n_slow[par], L.slow_list);
         __syncthreads();
 
-        // ---- T1: hops of the listed records; store the previous plane's output
+        // ---- T1: hops of plane p (first lanes up) and the gather of plane p-2 (last lanes down)
+        const uint32_t nh = L.n_hop[par], nsl = L.n_slow[par];
+        if (lane < 2)
+            (lane == 0 ? L.n_hop : L.n_slow)[par ^ 1] = 0u;  // last read in the previous iteration's T1
         {
-            const uint32_t nh = L.n_hop;
-            const uint8_t *rp = L.reg[mod3(p - 1)], *rc = L.reg[rs], *rn = L.reg[mod3(p + 1)];
+            const uint8_t *r_prev = L.reg[mod3(p - 1)], *r_cur = L.reg[mod3(p)], *r_next = L.reg[mod3(p + 1)];
             for (uint32_t i = uint32_t(lane); i < nh; i += kBlock) {
                 const int rec = int(L.hop_list[i]);
                 const int yy = rec / kRZ - 1, zz = rec % kRZ - 1;
                 Slots<S> sl;
 #pragma unroll
                 for (int q = 0; q < S / 4; ++q)
-                    sl.w[q] = L.slots[rs][rec][q];
+                    sl.w[q] = L.slots[rp][rec][q];
                 uint64_t c;
                 uint32_t sm;
-                record_hops<S>(a, tb, sl, p, y0 + yy, z0 + zz, yy, zz, rp, rc, rn, c, sm);
-                L.codes[rs][rec] = c;
-                L.summ[rs][rec] = uint8_t(sm);
+                record_hops<S>(a, tb, sl, p, y0 + yy, z0 + zz, yy, zz, r_prev, r_cur, r_next, c, sm);
+                L.codes[rp][rec] = c;
+                L.summ[rp][rec] = uint8_t(sm);
+                if (sm) {  // flag the interior voxels that receive a particle
+                    auto mark = [&](int rg, int vy, int vz) {
+                        if (uint32_t(vy) < uint32_t(kTY) && uint32_t(vz) < uint32_t(kTZ))
+                            L.need[rg][vy * kTZ + vz] = 1u;
+                    };
+                    if (sm & 1u) mark(ring(p - 1), yy, zz);
+                    if (sm & 2u) mark(ring(p + 1), yy, zz);
+                    if (sm & 4u) mark(rp, yy - 1, zz);
+                    if (sm & 8u) mark(rp, yy + 1, zz);
+                    if (sm & 16u) mark(rp, yy, zz - 1);
+                    if (sm & 32u) mark(rp, yy, zz + 1);
+                }
             }
         }
-        if (prev_ok) {
-            Slots<S> o = zero_slots<S>();
-            if (prev_slow) {
-#pragma unroll
-                for (int q = 0; q < S / 4; ++q)
-                    o.w[q] = L.out[lane][q];
-            }
-            store_slots<S>(a.dst, prev_v, o);
-        }
-        prev_ok = false;
-        if (lane == 0)
-            L.n_slow = 0u;
-        __syncthreads();
-
-        // ---- T2: which voxels of plane x = p-1 need the full gather
-        const int x = p - 1;
-        const bool gather_here = x >= xb && x < xe && in_lattice;
-        bool slow = false;
-        if (gather_here) {
-            const int r0 = mod3(x - 1), r1 = mod3(x), r2 = rs;
-            const uint32_t any_in = (L.summ[r0][irec] & 2u) | (L.summ[r1][irec - kRZ] & 8u) |
-                                    (L.summ[r1][irec - 1] & 32u) | (L.summ[r1][irec + 1] & 16u) |
-                                    (L.summ[r1][irec + kRZ] & 4u) | (L.summ[r2][irec] & 1u);
-            slow = count_nonzero(cur) != 0u || any_in != 0u || t.has_zero_order != 0u;
-            prev_v = uint32_t(Z) + uint32_t(Y) * a.R2 + uint32_t(x) * a.R12;
-            prev_ok = true;
-            prev_slow = slow;
-            if (!slow)  // empty, nothing arrives: only the zero-propensity draw at seed 0 "fires" (a no-op)
-                n_fired += (a.itime + uint32_t(x) * uint32_t(Y) + uint32_t(Z)) == 0u;
-        }
-        wave_append(slow, uint32_t(lane), &L.n_slow, L.slow_list);
-        if (lane == 0)
-            L.n_hop = 0u;
-        cur = mine;
-        __syncthreads();
-
-        // ---- T3: full gather + reaction of the listed voxels
         {
-            const uint32_t ns = L.n_slow;
-            const int r0 = mod3(x - 1), r1 = mod3(x), r2 = rs;
-            for (uint32_t i = uint32_t(lane); i < ns; i += kBlock) {
+            const int r0 = ring(x - 1), r1 = ring(x), r2 = ring(x + 1);
+            for (uint32_t i = uint32_t(kBlock - 1 - lane); i < nsl; i += kBlock) {
                 const int vl = int(L.slow_list[i]);
                 const int vy = vl / kTZ, vz = vl % kTZ;
                 const int vr = rec_of(vy, vz);
@@ -1024,12 +1038,14 @@ __global__ __launch_bounds__(kBlock) void rdme_step_tiled(StepArgs a, TileArgs t
 #pragma unroll
                 for (int q = 0; q < S / 4; ++q)
                     own.w[q] = L.slots[r1][vr][q];
-                const uint64_t own_codes = L.codes[r1][vr];
                 const uint32_t occ = count_nonzero(own);
                 uint64_t stay = 0ull;
-                for (uint32_t s = 0; s < occ; ++s)
-                    if (((own_codes >> (4u * s)) & 0xFull) >= 6u)
-                        stay |= 1ull << s;
+                if (occ) {
+                    const uint64_t own_codes = L.codes[r1][vr];
+                    for (uint32_t s = 0; s < occ; ++s)
+                        if (((own_codes >> (4u * s)) & 0xFull) >= 6u)
+                            stay |= 1ull << s;
+                }
                 Slots<S> lo = zero_slots<S>(), hi = zero_slots<S>();
                 uint32_t nlo = 0, nhi = 0;
                 if (L.summ[r0][vr] & 2u) nlo = take_arrivals<S>(L.slots[r0][vr], L.codes[r0][vr], 1u, lo, nlo);
@@ -1038,8 +1054,8 @@ __global__ __launch_bounds__(kBlock) void rdme_step_tiled(StepArgs a, TileArgs t
                 if (L.summ[r1][vr + 1] & 16u) nhi = take_arrivals<S>(L.slots[r1][vr + 1], L.codes[r1][vr + 1], 4u, hi, nhi);
                 if (L.summ[r1][vr + kRZ] & 4u) nhi = take_arrivals<S>(L.slots[r1][vr + kRZ], L.codes[r1][vr + kRZ], 2u, hi, nhi);
                 if (L.summ[r2][vr] & 1u) nhi = take_arrivals<S>(L.slots[r2][vr], L.codes[r2][vr], 0u, hi, nhi);
-                const uint32_t V = gz + gy * a.R2 + uint32_t(x) * a.R12;
                 if (occ + nlo + nhi > uint32_t(S)) {
+                    const uint32_t V = gz + gy * a.R2 + uint32_t(x) * a.R12;
                     ++n_tight;
                     const uint32_t k = atomicAdd(&L.n_tight, 1u);
                     if (k < uint32_t(kTightLds))
@@ -1062,16 +1078,16 @@ __global__ __launch_bounds__(kBlock) void rdme_step_tiled(StepArgs a, TileArgs t
         }
         __syncthreads();
     }
-    if (prev_ok) {
+    if (out_ok) {
         Slots<S> o = zero_slots<S>();
-        if (prev_slow) {
+        if (out_slow) {
 #pragma unroll
             for (int q = 0; q < S / 4; ++q)
                 o.w[q] = L.out[lane][q];
         }
-        store_slots<S>(a.dst, prev_v, o);
+        store_slots<S>(a.dst, out_v, o);
     }
-    // the list is stable: the last barrier of the loop follows every append
+    // the block's tight list is final: the last barrier of the loop follows every append
     const uint32_t nt = min(L.n_tight, uint32_t(kTightLds));
     for (uint32_t i = uint32_t(lane); i < nt; i += kBlock)
         note_tight(a, L.tight[i]);
@@ -1113,6 +1129,69 @@ __global__ __launch_bounds__(kBlock) void region_count_kernel(const uint8_t *lat
 }
 
 }  // namespace
+
+// Upper bound of a voxel's total propensity sum_r k_r * prod(counts of r's reactants) over every voxel
+// holding c particles, c = 0..S: per reaction the largest product of integer counts that sum to at most
+// c (found by enumeration; at most three reactants), times k_r. The sum gets a relative margin of 1e-4,
+// far above the rounding of the device's f32 products and sums (a few hundred terms at 2^-24 each), and is
+// rounded up to f32. With a negative rate no bound holds and every entry is +inf.
+static std::vector<float> propensity_bounds(const Model &m, uint32_t S)
+{
+    const uint32_t nr = m.num_reactions();
+    std::vector<double> b(S + 1u, 0.0);
+    bool valid = true;
+    for (uint32_t r = 1; r <= nr; ++r) {
+        const double k = m.rates()[r];
+        if (!(k >= 0.0)) {
+            valid = false;
+            break;
+        }
+        int sp[3], mult[3], d = 0;
+        for (int j = 0; j < 3; ++j) {
+            const int32_t x = m.reactants()[size_t(r) * 3u + j];
+            if (x <= 0)
+                continue;
+            int f = 0;
+            while (f < d && sp[f] != x)
+                ++f;
+            if (f == d) {
+                sp[d] = x;
+                mult[d++] = 0;
+            }
+            ++mult[f];
+        }
+        for (uint32_t c = 0; c <= S; ++c) {
+            double best = 0.0;
+            if (d == 0) {
+                best = 1.0;  // zero-order: C[v][0] = 1
+            } else {
+                const int lim[3] = {int(c), d > 1 ? int(c) : 0, d > 2 ? int(c) : 0};
+                for (int n0 = 0; n0 <= lim[0]; ++n0)
+                    for (int n1 = 0; n1 <= lim[1] && n0 + n1 <= int(c); ++n1)
+                        for (int n2 = 0; n2 <= lim[2] && n0 + n1 + n2 <= int(c); ++n2) {
+                            const int n[3] = {n0, n1, n2};
+                            double prod = 1.0;
+                            for (int j = 0; j < d; ++j)
+                                for (int e = 0; e < mult[j]; ++e)
+                                    prod *= double(n[j]);
+                            best = std::max(best, prod);
+                        }
+            }
+            b[c] += k * best;
+        }
+    }
+    std::vector<float> out(S + 1u, INFINITY);
+    if (!valid)
+        return out;
+    for (uint32_t c = 0; c <= S; ++c) {
+        const double v = b[c] * (1.0 + 1e-4);
+        float f = float(v);
+        if (double(f) < v)
+            f = std::nextafter(f, INFINITY);
+        out[c] = f;
+    }
+    return out;
+}
 
 Device::Device(const Model &m, int hip_device) : dev_(hip_device)
 {
@@ -1200,7 +1279,9 @@ Device::Device(const Model &m, int hip_device) : dev_(hip_device)
     off_ridx_ = uint32_t(align8(off_rates_ + m.rates().size() * sizeof(float)));
     off_rmask_ = uint32_t(align8(off_ridx_ + ridx.size()));
     off_stoich_ = uint32_t(align8(off_rmask_ + rmask.size() * sizeof(uint64_t)));
-    const size_t blob_bytes = align8(off_stoich_ + st8.size());
+    const std::vector<float> bound = propensity_bounds(m, S_);
+    off_bound_ = uint32_t(align8(off_stoich_ + st8.size()));
+    const size_t blob_bytes = align8(off_bound_ + bound.size() * sizeof(float));
     tab_words_ = uint32_t(blob_bytes / 4u);
     std::vector<uint8_t> blob(blob_bytes, 0);
     std::memcpy(blob.data() + off_p2_, p2.data(), p2.size() * sizeof(float));
@@ -1208,6 +1289,7 @@ Device::Device(const Model &m, int hip_device) : dev_(hip_device)
     std::memcpy(blob.data() + off_ridx_, ridx.data(), ridx.size());
     std::memcpy(blob.data() + off_rmask_, rmask.data(), rmask.size() * sizeof(uint64_t));
     std::memcpy(blob.data() + off_stoich_, st8.data(), st8.size());
+    std::memcpy(blob.data() + off_bound_, bound.data(), bound.size() * sizeof(float));
     hip_check(hipMalloc(&tab_, blob_bytes), "hipMalloc tables");
     uint8_t *tb = reinterpret_cast<uint8_t *>(tab_);
     p2_ = reinterpret_cast<float *>(tb + off_p2_);
@@ -1215,6 +1297,7 @@ Device::Device(const Model &m, int hip_device) : dev_(hip_device)
     ridx_ = tb + off_ridx_;
     rmask_ = reinterpret_cast<uint64_t *>(tb + off_rmask_);
     stoich_ = reinterpret_cast<int8_t *>(tb + off_stoich_);
+    bound_ = reinterpret_cast<float *>(tb + off_bound_);
     if (blob_bytes > kTabLdsMax)
         simple_ = true;  // tables too large to stage in LDS next to the plane records
     hip_check(hipMalloc(&counters_, kNumCounters * sizeof(unsigned long long)), "hipMalloc counters");
@@ -1267,6 +1350,7 @@ void Device::step(uint32_t itime)
     a.ridx = ridx_;
     a.stoich = stoich_;
     a.rmask = rmask_;
+    a.bound = bound_;
     a.counters = counters_;
     a.fix_ctl = fix_ctl_;
     a.fix_list = fix_list_;
@@ -1309,6 +1393,7 @@ void Device::step(uint32_t itime)
         t.off_ridx = off_ridx_;
         t.off_stoich = off_stoich_;
         t.off_rmask = off_rmask_;
+        t.off_bound = off_bound_;
         t.tab = static_cast<const uint32_t *>(tab_);
         const uint32_t blocks = t.tiles_z * t.tiles_y * t.chunks;
         const size_t shm = size_t(tab_words_) * 4u;

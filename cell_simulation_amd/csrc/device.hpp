@@ -55,12 +55,13 @@ private:
     uint8_t *reg_ = nullptr;
     void *tab_ = nullptr;          // one blob with every table below, at the byte offsets off_*
     uint32_t tab_words_ = 0;
-    uint32_t off_p2_ = 0, off_rates_ = 0, off_ridx_ = 0, off_rmask_ = 0, off_stoich_ = 0;
+    uint32_t off_p2_ = 0, off_rates_ = 0, off_ridx_ = 0, off_rmask_ = 0, off_stoich_ = 0, off_bound_ = 0;
     float *p2_ = nullptr;          // (D*tau)/(lambda*lambda), [src][dst][sp]
     float *rates_ = nullptr;       // [nr+1]
     uint8_t *ridx_ = nullptr;      // [nr+1][4]
     int8_t *stoich_ = nullptr;     // [nr+1][ns+1]
     uint64_t *rmask_ = nullptr;    // [ns+1][nrw]: reactions having the species as a reactant; row 0: zero-order
+    float *bound_ = nullptr;       // [S+1]: upper bound of the total propensity of a voxel with c particles
     unsigned long long *counters_ = nullptr;
     int32_t *hist_ = nullptr;      // [nreg][ns+1]
     uint32_t *fix_ctl_ = nullptr;  // tight-voxel replay: control words, lists and slot copies
