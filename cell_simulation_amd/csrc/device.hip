@@ -388,35 +388,119 @@ __device__ __forceinline__ void react(const StepArgs &a, const Tab &tb, Slots<S>
     out = res;
 }
 
-// Canonical slot picture of the voxel after diffusion (see the header comment).
+// ---- byte vectors: a voxel's slots as two 64-bit halves (slots 0..7, 8..15) ---------------------------
+struct BV {
+    uint64_t lo, hi;
+};
+
+template <int S>
+__device__ __forceinline__ BV to_bv(const Slots<S> &s)
+{
+    if constexpr (S == 16)
+        return BV{uint64_t(s.w[0]) | (uint64_t(s.w[1]) << 32), uint64_t(s.w[2]) | (uint64_t(s.w[3]) << 32)};
+    else if constexpr (S == 8)
+        return BV{uint64_t(s.w[0]) | (uint64_t(s.w[1]) << 32), 0ull};
+    else
+        return BV{uint64_t(s.w[0]), 0ull};
+}
+
+template <int S>
+__device__ __forceinline__ Slots<S> from_bv(const BV &v)
+{
+    Slots<S> s;
+    s.w[0] = uint32_t(v.lo);
+    if constexpr (S >= 8)
+        s.w[1] = uint32_t(v.lo >> 32);
+    if constexpr (S == 16) {
+        s.w[2] = uint32_t(v.hi);
+        s.w[3] = uint32_t(v.hi >> 32);
+    }
+    return s;
+}
+
+// Moves every byte k positions up (k <= 16); bytes moved past position 15 are dropped.
+__device__ __forceinline__ BV shl_bytes(const BV &v, uint32_t k)
+{
+    const uint32_t b = 8u * (k & 7u);
+    const uint64_t lo = v.lo << b;
+    const uint64_t hi = (v.hi << b) | (b ? (v.lo >> (64u - b)) : 0ull);
+    return BV{k >= 8u ? 0ull : lo, k >= 16u ? 0ull : (k >= 8u ? lo : hi)};
+}
+
+// Moves every byte k positions down (k <= 16); zeros come in at the top.
+__device__ __forceinline__ BV shr_bytes(const BV &v, uint32_t k)
+{
+    const uint32_t b = 8u * (k & 7u);
+    const uint64_t hi = v.hi >> b;
+    const uint64_t lo = (v.lo >> b) | (b ? (v.hi << (64u - b)) : 0ull);
+    return BV{k >= 16u ? 0ull : (k >= 8u ? hi : lo), k >= 8u ? 0ull : hi};
+}
+
+// Mask of the bytes below position d (d <= 16).
+__device__ __forceinline__ BV bytes_below(uint32_t d)
+{
+    const uint64_t m = (d & 7u) ? ((1ull << (8u * (d & 7u))) - 1ull) : 0ull;
+    return BV{d >= 8u ? ~0ull : m, d >= 16u ? ~0ull : (d >= 8u ? m : 0ull)};
+}
+
+__device__ __forceinline__ uint32_t bv_byte(const BV &v, uint32_t j)
+{
+    // half chosen by masking, not by a select: with j uniform the compiler turns a select of the two
+    // halves into an indexed load from scratch memory
+    const uint64_t w = v.lo ^ ((v.hi ^ v.lo) & (0ull - uint64_t((j >> 3) & 1u)));
+    return uint32_t(w >> (8u * (j & 7u))) & 0xFFu;
+}
+
+// Byte at position d replaced by val.
+__device__ __forceinline__ BV bv_set(const BV &v, uint32_t d, uint32_t val)
+{
+    const uint64_t m = 0xFFull << (8u * (d & 7u)), x = uint64_t(val) << (8u * (d & 7u));
+    return (d & 8u) ? BV{v.lo, (v.hi & ~m) | x} : BV{(v.lo & ~m) | x, v.hi};
+}
+
+// Byte at position d removed; the bytes above it move down one place.
+__device__ __forceinline__ BV bv_delete(const BV &v, uint32_t d)
+{
+    const BV m = bytes_below(d), sh = shr_bytes(v, 1u);
+    return BV{(v.lo & m.lo) | (sh.lo & ~m.lo), (v.hi & m.hi) | (sh.hi & ~m.hi)};
+}
+
+// Bit 4s set for s < n (n <= 16): the nibble mask of the first n slots.
+__device__ __forceinline__ uint64_t first_nibbles(uint32_t n)
+{
+    return (n >= 16u ? ~0ull : ((1ull << (4u * n)) - 1ull)) & 0x1111111111111111ull;
+}
+
+// Canonical slot picture of the voxel after diffusion (see the header comment). `stay_mask` has bit 4s
+// set when own slot s stays. Departed slots are refilled, in slot order, by the higher-index arrivals
+// while they last; departed slots left over close up; the lower-index arrivals and the remaining
+// higher-index arrivals follow. Done with byte shifts: the loops run once per departed particle only.
 template <int S>
 __device__ __forceinline__ Slots<S> assemble(const Slots<S> &own, uint32_t occ, uint64_t stay_mask,
                                              const Slots<S> &lo, uint32_t nlo, const Slots<S> &hi, uint32_t nhi,
                                              uint32_t &cnt)
 {
-    uint32_t j = 0, h = 0;
-    Slots<S> out;
-    if (nhi == 0u && (stay_mask & ((1ull << occ) - 1ull)) == ((1ull << occ) - 1ull)) {
-        // nothing left and nothing refills: the own slots stay as they are, arrivals follow
-        out = own;
-        j = occ;
-    } else {
-        out = zero_slots<S>();
-        for (uint32_t s = 0; s < occ; ++s) {
-            if ((stay_mask >> s) & 1ull) {
-                put_byte_dyn(out, j++, get_byte_dyn(own, s));
-            } else if (h < nhi) {
-                put_byte_dyn(out, j++, get_byte_dyn(hi, h));
-                ++h;
-            }
-        }
+    BV o = to_bv<S>(own), h = to_bv<S>(hi);
+    uint64_t rem = first_nibbles(occ) & ~stay_mask;  // departed own slots
+    uint32_t used = 0;
+    while (rem != 0ull && used < nhi) {
+        const uint32_t sl = uint32_t(__ffsll((long long)rem) - 1) >> 2;
+        rem &= rem - 1ull;
+        o = bv_set(o, sl, bv_byte(h, used));
+        ++used;
     }
-    for (uint32_t q = 0; q < nlo; ++q)
-        put_byte_dyn(out, j++, get_byte_dyn(lo, q));
-    for (; h < nhi; ++h)
-        put_byte_dyn(out, j++, get_byte_dyn(hi, h));
-    cnt = j < uint32_t(S) ? j : uint32_t(S);  // a tight voxel may collect more than S; extras were dropped
-    return out;
+    const uint32_t j = occ - uint32_t(__popcll(rem));
+    while (rem != 0ull) {  // highest first, so the lower positions stay valid
+        const uint32_t bit = 63u - uint32_t(__clzll((long long)rem));
+        rem &= ~(1ull << bit);
+        o = bv_delete(o, bit >> 2);
+    }
+    const BV l = shl_bytes(to_bv<S>(lo), j), hr = shl_bytes(shr_bytes(h, used), j + nlo);
+    o.lo |= l.lo | hr.lo;
+    o.hi |= l.hi | hr.hi;
+    const uint32_t n = j + nlo + (nhi - used);
+    cnt = n < uint32_t(S) ? n : uint32_t(S);  // a tight voxel may collect more than S; extras were dropped
+    return from_bv<S>(o);
 }
 
 __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb)
@@ -447,7 +531,7 @@ __device__ __forceinline__ Slots<S> gather_voxel(const StepArgs &a, const Tab &t
 #pragma unroll
         for (int s = 0; s < S; ++s)
             if (uint32_t(s) < occ && hop(a, tb, h, get_byte(own, s), s) == 6u)
-                stay |= 1ull << s;
+                stay |= 1ull << (4 * s);
     }
     Slots<S> lo = zero_slots<S>(), hi = zero_slots<S>();
     uint32_t nlo = 0, nhi = 0;
@@ -696,28 +780,32 @@ __global__ __launch_bounds__(kBlock) void rdme_step_kernel(StepArgs a)
 
 // ---- tiled plane-march kernel ------------------------------------------------------------------------
 // A block owns a kTY x kTZ tile of (y, z) and marches along x through `xc` planes. Each plane is staged in
-// LDS with a one-voxel halo in y and z ("records": slots, per-slot hop codes, a summary of hop
-// directions), so every particle's hop is computed once per block instead of once per neighbour, and the
-// gather reads neighbour records from LDS. Region labels are staged with a two-voxel halo (a halo
-// record's hop needs its own neighbours' labels). The slots and labels of the next plane are loaded into
-// registers one iteration ahead, so their HBM latency overlaps the current plane's work. The model tables
-// live in LDS for the whole block.
+// LDS with a one-voxel halo in y and z ("records": slots and per-slot hop codes), so every particle's hop
+// is computed once per block instead of once per neighbour, and the gather reads neighbour records from
+// LDS. Region labels are staged with a two-voxel halo (a halo record's hop needs its own neighbours'
+// labels). The slots and labels of the next plane are loaded into registers one iteration ahead, so their
+// HBM latency overlaps the current plane's work. The model tables live in LDS for the whole block.
 //
 // Particles are sparse, so the per-particle work runs from lists: iteration p stages plane p and lists
-// its records that hold particles; the hops of those records set a "needs gather" flag on every voxel a
-// particle enters. The gather runs two planes behind (plane p-2), when every flag it depends on is final,
-// so the hops of plane p and the gather of plane p-2 run in the same phase on different waves: hop items
-// are taken from the first lanes up, gather items from the last lanes down. Two barriers per plane.
+// its records that hold particles; the hops of those records mark, on every voxel a particle enters, the
+// direction it came by ("arrive" bytes) and a "needs gather" bit. The gather runs two planes behind
+// (plane p-2), when every mark it depends on is final, so the hops of plane p and the gather of plane p-2
+// run in the same phase on different waves: hop items are taken from the first lanes up, gather items
+// from the last lanes down. Lists are per-wave segments filled with ballots (no LDS atomics); two
+// barriers per plane.
 constexpr int kTZ = 64;                    // one wave per z-row
 constexpr int kTY = 4;                     // four rows: 256 lanes
+constexpr int kWaves = kBlock / 64;        // == kTY: wave w owns interior row w
 constexpr int kRZ = kTZ + 2;               // record row with z halo
 constexpr int kRec = (kTY + 2) * kRZ;      // records per plane (396)
 constexpr int kHalo = kRec - kTY * kTZ;    // halo records per plane (140)
-constexpr int kGZ = kTZ + 4;               // label row with a two-voxel z halo
+constexpr int kGZ = kTZ + 4;               // label row with a two-voxel z halo (68 bytes, 17 words)
 constexpr int kGrid = (kTY + 4) * kGZ;     // labels per plane (544)
-constexpr int kLabPerLane = (kGrid + kBlock - 1) / kBlock;  // 3
+constexpr int kLabWords = kGrid / 4;       // 136: one word of four labels per lane
+constexpr int kRing = 4;                   // planes of records: the gather of p-2 reads p-3..p-1 while p is staged
 constexpr uint32_t kTabLdsMax = 24 * 1024;  // tables larger than this run on the simple kernel
 constexpr int kTightLds = 64;               // tight voxels a block can note in LDS per step
+static_assert(kWaves == kTY && kGZ % 4 == 0 && kLabWords <= kBlock, "tile geometry");
 
 struct TileArgs {
     uint32_t tiles_z, tiles_y, chunks, xc;
@@ -727,37 +815,78 @@ struct TileArgs {
     const uint32_t *tab;                   // the blob in global memory
 };
 
-constexpr int kRing = 4;                    // planes of records: the gather of p-2 reads p-3..p-1 while p is staged
+// A voxel's slots as one vector, so LDS moves them with one 4/8/16-byte access per lane.
+template <int S> struct SlotVec;
+template <> struct SlotVec<4> { using T = uint32_t; };
+template <> struct SlotVec<8> { using T = uint2; };
+template <> struct SlotVec<16> { using T = uint4; };
+
+template <int S>
+__device__ __forceinline__ typename SlotVec<S>::T to_vec(const Slots<S> &s)
+{
+    if constexpr (S == 16)
+        return make_uint4(s.w[0], s.w[1], s.w[2], s.w[3]);
+    else if constexpr (S == 8)
+        return make_uint2(s.w[0], s.w[1]);
+    else
+        return s.w[0];
+}
+
+template <int S>
+__device__ __forceinline__ Slots<S> from_vec(const typename SlotVec<S>::T &v)
+{
+    Slots<S> s;
+    if constexpr (S == 16) {
+        s.w[0] = v.x; s.w[1] = v.y; s.w[2] = v.z; s.w[3] = v.w;
+    } else if constexpr (S == 8) {
+        s.w[0] = v.x; s.w[1] = v.y;
+    } else {
+        s.w[0] = v;
+    }
+    return s;
+}
+
+constexpr int kHopSeg = 192;  // hop items a wave can list per plane
 
 template <int S>
 struct TileLds {
-    uint32_t slots[kRing][kRec][S / 4];
-    uint64_t codes[kRing][kRec];     // valid for the records listed for hops
-    uint8_t summ[kRing][kRec];       // bit d: some particle of the record hops in direction d
-    uint8_t need[kRing][kTY * kTZ];  // interior voxel holds particles or receives one
-    uint8_t reg[3][kGrid];
-    uint32_t out[kTY * kTZ][S / 4];  // gathered output of the listed voxels of one plane
-    uint16_t hop_list[kRec];         // records of the staged plane that hold particles
-    uint16_t slow_list[kTY * kTZ];   // interior voxels of the gathered plane that need the full gather
-    uint32_t n_hop[2], n_slow[2], n_tight;
+    typename SlotVec<S>::T slots[kRing][kRec];
+    uint64_t codes[kRing][kRec];            // nibble per slot (see record hops); valid for occupied records
+    uint64_t need[kRing][kTY];              // bit z of row y: the voxel holds particles or receives one
+    uint32_t arrive[kRing][kTY * kTZ / 4];  // byte per interior voxel: bit d = a particle arrives by a d-hop
+    uint32_t reg[3][kLabWords];             // region labels, four per word
+    uint32_t hop_list[kWaves][kHopSeg];     // hop items listed by wave w: record | first slot << 9 | slots << 13
+    uint16_t slow_list[kWaves][64];         // interior voxels of wave w that need the full gather
+    uint32_t hop_cnt[kWaves];
+    uint32_t slow_cnt[kWaves];
+    uint32_t xmy[kTY + 2];                  // x % y of the staged plane for the record rows y0-1 .. y0+kTY
+    uint32_t n_tight;
     uint32_t tight[kTightLds];
 };
 
-// Appends `item` to an LDS list for the lanes where `pred` holds, in lane order, with one LDS atomic
-// per wave.
-__device__ __forceinline__ void wave_append(bool pred, uint32_t item, uint32_t *counter, uint16_t *list)
+// Writes `item` at the lane's rank among the lanes where `pred` holds into a per-wave list segment and
+// returns the segment's length (the same in every lane).
+__device__ __forceinline__ uint32_t wave_list(bool pred, uint32_t item, uint16_t *seg)
 {
     const uint64_t m = __ballot(pred);
-    if (m == 0ull)
-        return;
-    const uint32_t lane = __lane_id();
-    const uint32_t leader = uint32_t(__ffsll((long long)m) - 1);
-    uint32_t base = 0;
-    if (lane == leader)
-        base = atomicAdd(counter, uint32_t(__popcll(m)));
-    base = __shfl(base, int(leader));
+    const uint32_t rank = __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
     if (pred)
-        list[base + uint32_t(__popcll(m & ((1ull << lane) - 1ull)))] = uint16_t(item);
+        seg[rank] = uint16_t(item);
+    return uint32_t(__popcll(m));
+}
+
+// Item i of the concatenation of n segments with the given lengths: (segment, position).
+template <int N>
+__device__ __forceinline__ void locate(uint32_t i, const uint32_t (&len)[N], uint32_t &seg, uint32_t &pos)
+{
+    seg = 0u;
+    pos = i;
+#pragma unroll
+    for (int k = 0; k < N - 1; ++k) {
+        const bool past = seg == uint32_t(k) && pos >= len[k];
+        pos = past ? pos - len[k] : pos;
+        seg = past ? seg + 1u : seg;
+    }
 }
 
 // record index -> tile coordinates (yy in [-1, kTY], zz in [-1, kTZ])
@@ -782,18 +911,18 @@ __device__ __forceinline__ Slots<S> masked(const Slots<S> &s, bool ok)
     return r;
 }
 
-// Hop codes (one nibble per slot: 0..5 = hops in that direction, 6 = stays, 0xF = empty) and the
-// direction summary of one record at plane X. Labels come from the staged planes X-1, X, X+1.
+// Hops of `ns` slots from `s0` of one record at plane X: the hop code of slot s (0..5 = hops in that
+// direction, 6 = stays; an empty slot keeps 0xF) is returned as the bits to clear from an all-0xF nibble
+// `clear`, and `summ` gets bit d for every hop in direction d. Labels come from the staged planes X-1, X,
+// X+1; `xmy` is X % Y (the reference's seed term, rdme.wgsl:220-226).
 template <int S>
-__device__ __forceinline__ void record_hops(const StepArgs &a, const Tab &tb, const Slots<S> &sl, int X, int Y,
-                                            int Z, int yy, int zz, const uint8_t *rprev, const uint8_t *rcur,
-                                            const uint8_t *rnext, uint64_t &codes, uint32_t &summ)
+__device__ __forceinline__ void item_hops(const StepArgs &a, const Tab &tb, const Slots<S> &sl, int X, int Y, int Z,
+                                          int yy, int zz, uint32_t xmy, const uint8_t *rprev, const uint8_t *rcur,
+                                          const uint8_t *rnext, uint32_t s0, uint32_t ns, uint64_t &clear,
+                                          uint32_t &summ)
 {
-    codes = ~0ull;
+    clear = 0ull;
     summ = 0u;
-    const uint32_t occ = count_nonzero(sl);
-    if (occ == 0u)
-        return;
     const int gi = (yy + 2) * kGZ + (zz + 2);
     Hood h;
     h.src_reg = rcur[gi];
@@ -811,12 +940,10 @@ __device__ __forceinline__ void record_hops(const StepArgs &a, const Tab &tb, co
     if (Z > 0) h.inside |= 16u;
     if (uint32_t(Z) + 1u < a.R2) h.inside |= 32u;
     const uint32_t ux = uint32_t(X), uy = uint32_t(Y), uz = uint32_t(Z);
-    const uint32_t xmy = (uy != 0u) ? (ux % uy) : a.mod_zero;
     h.seed0 = a.itime + xmy + uz + (uz + uy * a.R2 + ux * a.R12) * uint32_t(S);
-    for (uint32_t s = 0; s < occ; ++s) {
+    for (uint32_t s = s0; s < s0 + ns; ++s) {
         const uint32_t d = hop(a, tb, h, get_byte_dyn(sl, s), s);
-        codes &= ~(0xFull << (4u * s));
-        codes |= uint64_t(d) << (4u * s);
+        clear |= uint64_t(0xFu ^ d) << (4u * s);
         if (d < 6u)
             summ |= 1u << d;
     }
@@ -831,16 +958,16 @@ __device__ __forceinline__ uint64_t nibble_matches(uint64_t codes, uint32_t want
     return ~x & 0x1111111111111111ull;
 }
 
-// Append the slots of a neighbour record whose hop code equals `want`, in slot order.
+// Appends the slots of a neighbour record selected by `m` (bit 4s for slot s), in slot order.
 template <int S>
-__device__ __forceinline__ uint32_t take_arrivals(const uint32_t *words, uint64_t codes, uint32_t want,
-                                                  Slots<S> &list, uint32_t n)
+__device__ __forceinline__ uint32_t take_arrivals(const typename SlotVec<S>::T &src, uint64_t m, Slots<S> &list,
+                                                  uint32_t n)
 {
-    uint64_t m = nibble_matches(codes, want);
+    const Slots<S> sl = from_vec<S>(src);
     while (m) {
         const uint32_t s = uint32_t(__ffsll((long long)m) - 1) >> 2;
         m &= m - 1ull;
-        put_byte_dyn(list, n, (words[s >> 2] >> ((s & 3u) * 8u)) & 0xFFu);
+        put_byte_dyn(list, n, get_byte_dyn(sl, s));
         ++n;
     }
     return n;
@@ -859,7 +986,8 @@ __global__ __launch_bounds__(kBlock) void rdme_step_tiled(StepArgs a, TileArgs t
     const int xb = int(chunk * t.xc);
     const int xe = min(int(a.R0), xb + int(t.xc));
     const int lane = int(threadIdx.x);
-    const int iy = lane / kTZ, iz = lane % kTZ;  // interior voxel of this lane
+    const int wv = lane / 64, wl = lane % 64;  // wave and lane in wave
+    const int iy = wv, iz = wl;                 // interior voxel of this lane
     int hy = 0, hz = 0;
     const bool has_halo = lane < kHalo;
     if (has_halo)
@@ -874,15 +1002,19 @@ __global__ __launch_bounds__(kBlock) void rdme_step_tiled(StepArgs a, TileArgs t
                         uint32_t(z0 + hz) < a.R2;
     const uint32_t mine_yz = in_lattice ? uint32_t(Z) + uint32_t(Y) * a.R2 : 0u;
     const uint32_t hal_yz = hal_in ? uint32_t(z0 + hz) + uint32_t(y0 + hy) * a.R2 : 0u;
-    uint32_t lab_yz[kLabPerLane];
+    // Labels: lane l < kLabWords loads grid row l / 17, columns 4*(l % 17) .. +3 (z0-2 .. z0+65 over a row).
+    const bool lab_lane = lane < kLabWords;
+    uint32_t lab_off[4];
     uint32_t lab_ok = 0u;
+    {
+        const int gy = y0 + lane / (kGZ / 4) - 2, gz0 = z0 + 4 * (lane % (kGZ / 4)) - 2;
 #pragma unroll
-    for (int k = 0; k < kLabPerLane; ++k) {
-        const int i = lane + k * kBlock;
-        const int gy = y0 + i / kGZ - 2, gz = z0 + i % kGZ - 2;
-        const bool ok = i < kGrid && gy >= 0 && uint32_t(gy) < a.R1 && gz >= 0 && uint32_t(gz) < a.R2;
-        lab_yz[k] = ok ? uint32_t(gz) + uint32_t(gy) * a.R2 : 0u;
-        lab_ok |= uint32_t(ok) << k;
+        for (int b = 0; b < 4; ++b) {
+            const int gz = gz0 + b;
+            const bool ok = lab_lane && gy >= 0 && uint32_t(gy) < a.R1 && gz >= 0 && uint32_t(gz) < a.R2;
+            lab_off[b] = ok ? uint32_t(gz) + uint32_t(gy) * a.R2 : 0u;
+            lab_ok |= uint32_t(ok) << b;
+        }
     }
 
     for (uint32_t i = uint32_t(lane); i < t.tab_words; i += kBlock)
@@ -893,10 +1025,6 @@ __global__ __launch_bounds__(kBlock) void rdme_step_tiled(StepArgs a, TileArgs t
                  reinterpret_cast<const int8_t *>(tbase + t.off_stoich),
                  reinterpret_cast<const uint64_t *>(tbase + t.off_rmask),
                  reinterpret_cast<const float *>(tbase + t.off_bound)};
-    if (lane < 2) {
-        L.n_hop[lane] = 0u;
-        L.n_slow[lane] = 0u;
-    }
     if (lane == 0)
         L.n_tight = 0u;
 
@@ -904,158 +1032,204 @@ __global__ __launch_bounds__(kBlock) void rdme_step_tiled(StepArgs a, TileArgs t
     // Label loads are unconditional into full registers; whether a value is real is decided when it is
     // stored a plane later. (A load under a branch, merged into a packed byte, makes the compiler wait for
     // every outstanding load, the slot prefetch included, on the spot.)
-    auto fetch_labels = [&](int X, uint32_t (&r)[kLabPerLane]) {
-        const bool xo = X >= 0 && X < int(a.R0);
+    auto fetch_labels = [&](int X, uint32_t (&r)[4]) {
         const uint32_t base = plane_base(X);
 #pragma unroll
-        for (int k = 0; k < kLabPerLane; ++k)
-            r[k] = uint32_t(a.reg[base + lab_yz[k]]) | ((xo && ((lab_ok >> k) & 1u)) ? 0u : 0x100u);
+        for (int b = 0; b < 4; ++b)
+            r[b] = a.reg[base + lab_off[b]];
     };
-    auto store_labels = [&](int slot, const uint32_t (&r)[kLabPerLane]) {
+    auto store_labels = [&](int X, const uint32_t (&r)[4]) {
+        if (lab_lane) {
+            const uint32_t ok = (X >= 0 && X < int(a.R0)) ? lab_ok : 0u;
+            uint32_t w = 0u;
 #pragma unroll
-        for (int k = 0; k < kLabPerLane; ++k)
-            if (lane + k * kBlock < kGrid)
-                L.reg[slot][lane + k * kBlock] = (r[k] & 0x100u) ? uint8_t(0) : uint8_t(r[k]);
+            for (int b = 0; b < 4; ++b)
+                w |= ((ok >> b) & 1u) ? (r[b] & 0xFFu) << (8 * b) : 0u;
+            L.reg[(X + 3) % 3][lane] = w;
+        }
     };
-    auto mod3 = [](int p) { return (p + 3) % 3; };
+    auto labels_of = [&](int X) { return reinterpret_cast<const uint8_t *>(L.reg[(X + 3) % 3]); };
     auto ring = [](int p) { return (p + 4) & (kRing - 1); };
 
     {   // labels of planes xb-2 and xb-1 (needed by the hops of plane xb-1)
-        uint32_t r[kLabPerLane];
+        uint32_t r[4];
         fetch_labels(xb - 2, r);
-        store_labels(mod3(xb - 2), r);
+        store_labels(xb - 2, r);
         fetch_labels(xb - 1, r);
-        store_labels(mod3(xb - 1), r);
+        store_labels(xb - 1, r);
     }
-    // registers holding the next plane to stage: its slots and the labels one plane further
-    bool nx_ok = xb - 1 >= 0;
-    Slots<S> nx_mine = load_slots<S>(a.src, plane_base(xb - 1) + mine_yz);
-    Slots<S> nx_hal = load_slots<S>(a.src, plane_base(xb - 1) + hal_yz);
-    uint32_t nx_lab[kLabPerLane];
-    fetch_labels(xb, nx_lab);
+    // Registers holding the next plane to stage (its slots, and the labels one plane further), in two
+    // sets that the planes use in turn: a set is refilled by the plane that consumes the other one, so
+    // no register is copied while its load is in flight (a copy would wait for the load on the spot).
+    struct Staged {
+        Slots<S> mine, hal;
+        uint32_t lab[4];
+        bool ok;
+    };
+    Staged st0, st1;
+    st0.ok = xb - 1 >= 0;
+    st0.mine = load_slots<S>(a.src, plane_base(xb - 1) + mine_yz);
+    st0.hal = load_slots<S>(a.src, plane_base(xb - 1) + hal_yz);
+    fetch_labels(xb, st0.lab);
 
     uint32_t n_tight = 0, n_bad = 0, n_near = 0, n_fired = 0, n_rolled = 0;
-    uint32_t out_v = 0;  // voxel of the plane gathered last iteration, stored at the start of this one
-    bool out_ok = false, out_slow = false;
-    __syncthreads();  // list counters are zero before any wave appends
 
-    for (int p = xb - 1; p <= xe + 1; ++p) {
-        const int rp = ring(p), par = p & 1;
-        // ---- T0: store the output of plane p-3; stage plane p; list the work of planes p and p-2
-        if (out_ok) {
-            Slots<S> o = zero_slots<S>();
-            if (out_slow) {
-#pragma unroll
-                for (int q = 0; q < S / 4; ++q)
-                    o.w[q] = L.out[lane][q];
-            }
-            store_slots<S>(a.dst, out_v, o);
-        }
+    auto plane = [&](int p, Staged &use, Staged &fill) __attribute__((always_inline)) {
+        const int rp = ring(p);
+        // ---- T0: stage plane p and list its hop items; list the voxels of plane p-2 to gather
+        // prefetch plane p+1 and the labels of plane p+2 (out of the lattice or past the chunk: clamped,
+        // never used). Outside any branch, so the loaded registers are never merged at a join point.
+        fill.ok = p + 1 < int(a.R0);
+        fill.mine = load_slots<S>(a.src, plane_base(p + 1) + mine_yz);
+        fill.hal = load_slots<S>(a.src, plane_base(p + 1) + hal_yz);
+        fetch_labels(p + 2, fill.lab);
+        uint32_t n_items = 0u;
         if (p <= xe) {
-            const Slots<S> mine = masked(nx_mine, nx_ok && in_lattice), hal = masked(nx_hal, nx_ok && hal_in);
-            uint32_t lab[kLabPerLane];
-#pragma unroll
-            for (int k = 0; k < kLabPerLane; ++k)
-                lab[k] = nx_lab[k];
-            // prefetch plane p+1 and the labels of plane p+2 (out of the lattice: clamped, never used)
-            nx_ok = p + 1 < int(a.R0);
-            nx_mine = load_slots<S>(a.src, plane_base(p + 1) + mine_yz);
-            nx_hal = load_slots<S>(a.src, plane_base(p + 1) + hal_yz);
-            fetch_labels(p + 2, nx_lab);
-            store_labels(mod3(p + 1), lab);
-#pragma unroll
-            for (int q = 0; q < S / 4; ++q) {
-                L.slots[rp][irec][q] = mine.w[q];
-                if (has_halo)
-                    L.slots[rp][hrec][q] = hal.w[q];
-            }
-            L.summ[rp][irec] = 0u;
+            const Slots<S> mine = masked(use.mine, use.ok && in_lattice), hal = masked(use.hal, use.ok && hal_in);
+            store_labels(p + 1, use.lab);
+            L.slots[rp][irec] = to_vec<S>(mine);
             if (has_halo)
-                L.summ[rp][hrec] = 0u;
-            bool mine_occ = false, hal_occ = false;
-#pragma unroll
-            for (int q = 0; q < S / 4; ++q) {
-                mine_occ = mine_occ || mine.w[q] != 0u;
-                hal_occ = hal_occ || hal.w[q] != 0u;
+                L.slots[rp][hrec] = to_vec<S>(hal);
+            const uint32_t om = count_nonzero(mine), oh = count_nonzero(hal);
+            // codes of occupied records start all-0xF; the hop items clear the nibbles of their slots
+            if (om)
+                L.codes[rp][irec] = ~0ull;
+            if (oh)
+                L.codes[rp][hrec] = ~0ull;
+            if (lane < kTY + 2) {
+                const int y = y0 + lane - 1;
+                L.xmy[lane] = y > 0 ? uint32_t(p) % uint32_t(y) : (y == 0 ? a.mod_zero : 0u);
             }
-            L.need[ring(p + 1)][lane] = 0u;  // cleared before the hops of plane p mark it
-            if (mine_occ)
-                L.need[rp][lane] = 1u;
-            wave_append(mine_occ, uint32_t(irec), &L.n_hop[par], L.hop_list);
-            wave_append(has_halo && hal_occ, uint32_t(hrec), &L.n_hop[par], L.hop_list);
+            // plane p+1 starts unmarked before the hops of plane p mark it
+            if (wl < kTZ / 4)
+                L.arrive[ring(p + 1)][wv * (kTZ / 4) + wl] = 0u;
+            if (wl == 0)
+                L.need[ring(p + 1)][wv] = 0ull;
+            const uint64_t occm = __ballot(om != 0u);
+            if (wl == 0 && occm != 0ull)
+                atomicOr(reinterpret_cast<unsigned long long *>(&L.need[rp][wv]), (unsigned long long)occm);
+            // Hop items: one per particle when the wave's particles fit its list, else one per record.
+            uint32_t tot = 0u;
+#pragma unroll
+            for (int b = 0; b < 5; ++b)
+                tot += uint32_t(__popcll(__ballot((om >> b) & 1u)) + __popcll(__ballot((oh >> b) & 1u))) << b;
+            uint32_t *seg = L.hop_list[wv];
+            auto rank = [&](uint64_t m) {
+                return __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
+            };
+            if (tot <= uint32_t(kHopSeg)) {
+                for (uint32_t j = 0;; ++j) {
+                    const bool pm = om > j, ph = oh > j;
+                    const uint64_t mm = __ballot(pm), mh = __ballot(ph);
+                    if ((mm | mh) == 0ull)
+                        break;
+                    const uint32_t cm = uint32_t(__popcll(mm));
+                    if (pm)
+                        seg[n_items + rank(mm)] = uint32_t(irec) | (j << 9) | (1u << 13);
+                    if (ph)
+                        seg[n_items + cm + rank(mh)] = uint32_t(hrec) | (j << 9) | (1u << 13);
+                    n_items += cm + uint32_t(__popcll(mh));
+                }
+            } else {
+                const uint64_t mm = __ballot(om != 0u), mh = __ballot(oh != 0u);
+                const uint32_t cm = uint32_t(__popcll(mm));
+                if (om)
+                    seg[rank(mm)] = uint32_t(irec) | (om << 13);
+                if (oh)
+                    seg[cm + rank(mh)] = uint32_t(hrec) | (oh << 13);
+                n_items = cm + uint32_t(__popcll(mh));
+            }
         }
         const int x = p - 2;  // plane gathered in this iteration
         const bool gather_here = x >= xb && x < xe && in_lattice;
-        const bool slow = gather_here && (L.need[ring(x)][lane] != 0u || t.has_zero_order != 0u);
-        out_ok = gather_here;
-        out_slow = slow;
-        out_v = mine_yz + uint32_t(x) * a.R12;
-        if (gather_here && !slow)  // empty, nothing arrives: only the zero-propensity draw at seed 0 "fires"
+        bool slow = false;
+        if (x >= xb && x < xe)
+            slow = gather_here && (((L.need[ring(x)][wv] >> wl) & 1ull) != 0ull || t.has_zero_order != 0u);
+        if (gather_here && !slow) {
+            // empty and nothing arrives: the output is empty; only the zero-propensity draw at seed 0 "fires"
+            store_slots<S>(a.dst, mine_yz + uint32_t(x) * a.R12, zero_slots<S>());
             n_fired += (a.itime + uint32_t(x) * uint32_t(Y) + uint32_t(Z)) == 0u;
-        wave_append(slow, uint32_t(lane), &L.n_slow[par], L.slow_list);
+        }
+        const uint32_t n_slow = wave_list(slow, uint32_t(lane), L.slow_list[wv]);
+        if (wl == 0) {
+            L.hop_cnt[wv] = n_items;
+            L.slow_cnt[wv] = n_slow;
+        }
         __syncthreads();
 
         // ---- T1: hops of plane p (first lanes up) and the gather of plane p-2 (last lanes down)
-        const uint32_t nh = L.n_hop[par], nsl = L.n_slow[par];
-        if (lane < 2)
-            (lane == 0 ? L.n_hop : L.n_slow)[par ^ 1] = 0u;  // last read in the previous iteration's T1
-        {
-            const uint8_t *r_prev = L.reg[mod3(p - 1)], *r_cur = L.reg[mod3(p)], *r_next = L.reg[mod3(p + 1)];
-            for (uint32_t i = uint32_t(lane); i < nh; i += kBlock) {
-                const int rec = int(L.hop_list[i]);
-                const int yy = rec / kRZ - 1, zz = rec % kRZ - 1;
-                Slots<S> sl;
+        uint32_t hc[kWaves], sc[kWaves];
+        uint32_t nh = 0u, nsl = 0u;
 #pragma unroll
-                for (int q = 0; q < S / 4; ++q)
-                    sl.w[q] = L.slots[rp][rec][q];
-                uint64_t c;
+        for (int k = 0; k < kWaves; ++k) {
+            hc[k] = L.hop_cnt[k];
+            nh += hc[k];
+            sc[k] = L.slow_cnt[k];
+            nsl += sc[k];
+        }
+        {
+            const uint8_t *r_prev = labels_of(p - 1), *r_cur = labels_of(p), *r_next = labels_of(p + 1);
+            for (uint32_t i = uint32_t(lane); i < nh; i += kBlock) {
+                uint32_t seg, pos;
+                locate(i, hc, seg, pos);
+                const uint32_t item = L.hop_list[seg][pos];
+                const int rec = int(item & 511u);
+                const int yy = rec / kRZ - 1, zz = rec % kRZ - 1;
+                const Slots<S> sl = from_vec<S>(L.slots[rp][rec]);
+                uint64_t clear;
                 uint32_t sm;
-                record_hops<S>(a, tb, sl, p, y0 + yy, z0 + zz, yy, zz, r_prev, r_cur, r_next, c, sm);
-                L.codes[rp][rec] = c;
-                L.summ[rp][rec] = uint8_t(sm);
-                if (sm) {  // flag the interior voxels that receive a particle
-                    auto mark = [&](int rg, int vy, int vz) {
-                        if (uint32_t(vy) < uint32_t(kTY) && uint32_t(vz) < uint32_t(kTZ))
-                            L.need[rg][vy * kTZ + vz] = 1u;
-                    };
-                    if (sm & 1u) mark(ring(p - 1), yy, zz);
-                    if (sm & 2u) mark(ring(p + 1), yy, zz);
-                    if (sm & 4u) mark(rp, yy - 1, zz);
-                    if (sm & 8u) mark(rp, yy + 1, zz);
-                    if (sm & 16u) mark(rp, yy, zz - 1);
-                    if (sm & 32u) mark(rp, yy, zz + 1);
+                item_hops<S>(a, tb, sl, p, y0 + yy, z0 + zz, yy, zz, L.xmy[yy + 1], r_prev, r_cur, r_next,
+                             (item >> 9) & 15u, item >> 13, clear, sm);
+                if (clear)
+                    atomicAnd(reinterpret_cast<unsigned long long *>(&L.codes[rp][rec]), (unsigned long long)~clear);
+                while (sm) {  // mark the interior voxels that receive a particle, with the hop direction
+                    const uint32_t d = uint32_t(__ffs(sm) - 1);
+                    sm &= sm - 1u;
+                    const int vy = yy + (d == 2u ? -1 : (d == 3u ? 1 : 0));
+                    const int vz = zz + (d == 4u ? -1 : (d == 5u ? 1 : 0));
+                    const int rg = d == 0u ? ring(p - 1) : (d == 1u ? ring(p + 1) : rp);
+                    if (uint32_t(vy) < uint32_t(kTY) && uint32_t(vz) < uint32_t(kTZ)) {
+                        atomicOr(reinterpret_cast<unsigned long long *>(&L.need[rg][vy]),
+                                 (unsigned long long)(1ull << vz));
+                        atomicOr(&L.arrive[rg][(vy * kTZ + vz) >> 2], (1u << d) << (8 * (vz & 3)));
+                    }
                 }
             }
         }
         {
             const int r0 = ring(x - 1), r1 = ring(x), r2 = ring(x + 1);
             for (uint32_t i = uint32_t(kBlock - 1 - lane); i < nsl; i += kBlock) {
-                const int vl = int(L.slow_list[i]);
+                uint32_t seg, pos;
+                locate(i, sc, seg, pos);
+                const int vl = int(L.slow_list[seg][pos]);
                 const int vy = vl / kTZ, vz = vl % kTZ;
                 const int vr = rec_of(vy, vz);
                 const uint32_t gy = uint32_t(y0 + vy), gz = uint32_t(z0 + vz);
-                Slots<S> own;
-#pragma unroll
-                for (int q = 0; q < S / 4; ++q)
-                    own.w[q] = L.slots[r1][vr][q];
+                // every read is issued up front; records without a matching arrival are masked out
+                const Slots<S> own = from_vec<S>(L.slots[r1][vr]);
+                const uint64_t own_c = L.codes[r1][vr];
+                const uint32_t arr = (L.arrive[r1][vl >> 2] >> (8 * (vl & 3))) & 0x3Fu;
+                const typename SlotVec<S>::T sx0 = L.slots[r0][vr], sy0 = L.slots[r1][vr - kRZ],
+                                             sz0 = L.slots[r1][vr - 1], sz1 = L.slots[r1][vr + 1],
+                                             sy1 = L.slots[r1][vr + kRZ], sx1 = L.slots[r2][vr];
+                const uint64_t cx0 = L.codes[r0][vr], cy0 = L.codes[r1][vr - kRZ], cz0 = L.codes[r1][vr - 1],
+                               cz1 = L.codes[r1][vr + 1], cy1 = L.codes[r1][vr + kRZ], cx1 = L.codes[r2][vr];
                 const uint32_t occ = count_nonzero(own);
-                uint64_t stay = 0ull;
-                if (occ) {
-                    const uint64_t own_codes = L.codes[r1][vr];
-                    for (uint32_t s = 0; s < occ; ++s)
-                        if (((own_codes >> (4u * s)) & 0xFull) >= 6u)
-                            stay |= 1ull << s;
-                }
+                // nibble >= 6: stays (6) or empty (0xF)
+                const uint64_t stay = ((own_c >> 3) | ((own_c >> 2) & (own_c >> 1))) & 0x1111111111111111ull;
+                auto sel = [&](uint32_t bit, uint64_t c, uint32_t want) {
+                    return (arr & bit) ? nibble_matches(c, want) : 0ull;
+                };
                 Slots<S> lo = zero_slots<S>(), hi = zero_slots<S>();
                 uint32_t nlo = 0, nhi = 0;
-                if (L.summ[r0][vr] & 2u) nlo = take_arrivals<S>(L.slots[r0][vr], L.codes[r0][vr], 1u, lo, nlo);
-                if (L.summ[r1][vr - kRZ] & 8u) nlo = take_arrivals<S>(L.slots[r1][vr - kRZ], L.codes[r1][vr - kRZ], 3u, lo, nlo);
-                if (L.summ[r1][vr - 1] & 32u) nlo = take_arrivals<S>(L.slots[r1][vr - 1], L.codes[r1][vr - 1], 5u, lo, nlo);
-                if (L.summ[r1][vr + 1] & 16u) nhi = take_arrivals<S>(L.slots[r1][vr + 1], L.codes[r1][vr + 1], 4u, hi, nhi);
-                if (L.summ[r1][vr + kRZ] & 4u) nhi = take_arrivals<S>(L.slots[r1][vr + kRZ], L.codes[r1][vr + kRZ], 2u, hi, nhi);
-                if (L.summ[r2][vr] & 1u) nhi = take_arrivals<S>(L.slots[r2][vr], L.codes[r2][vr], 0u, hi, nhi);
+                nlo = take_arrivals<S>(sx0, sel(2u, cx0, 1u), lo, nlo);   // from -x, hopped +x
+                nlo = take_arrivals<S>(sy0, sel(8u, cy0, 3u), lo, nlo);   // from -y, hopped +y
+                nlo = take_arrivals<S>(sz0, sel(32u, cz0, 5u), lo, nlo);  // from -z, hopped +z
+                nhi = take_arrivals<S>(sz1, sel(16u, cz1, 4u), hi, nhi);  // from +z, hopped -z
+                nhi = take_arrivals<S>(sy1, sel(4u, cy1, 2u), hi, nhi);   // from +y, hopped -y
+                nhi = take_arrivals<S>(sx1, sel(1u, cx1, 0u), hi, nhi);   // from +x, hopped -x
+                const uint32_t V = gz + gy * a.R2 + uint32_t(x) * a.R12;
                 if (occ + nlo + nhi > uint32_t(S)) {
-                    const uint32_t V = gz + gy * a.R2 + uint32_t(x) * a.R12;
                     ++n_tight;
                     const uint32_t k = atomicAdd(&L.n_tight, 1u);
                     if (k < uint32_t(kTightLds))
@@ -1064,28 +1238,23 @@ __global__ __launch_bounds__(kBlock) void rdme_step_tiled(StepArgs a, TileArgs t
                         note_tight(a, V);  // block list full: straight to the global list
                 }
                 uint32_t cnt = 0;
-                Slots<S> out = assemble<S>(own, occ, stay, lo, nlo, hi, nhi, cnt);
+                Slots<S> out = assemble<S>(own, occ, occ ? stay : 0ull, lo, nlo, hi, nhi, cnt);
                 bool fired, rolled, near, bad;
                 react<S>(a, tb, out, cnt, uint32_t(x), gy, gz, fired, rolled, near, bad);
                 n_fired += fired;
                 n_rolled += rolled;
                 n_near += near;
                 n_bad += bad;
-#pragma unroll
-                for (int q = 0; q < S / 4; ++q)
-                    L.out[vl][q] = out.w[q];
+                store_slots<S>(a.dst, V, out);
             }
         }
         __syncthreads();
-    }
-    if (out_ok) {
-        Slots<S> o = zero_slots<S>();
-        if (out_slow) {
-#pragma unroll
-            for (int q = 0; q < S / 4; ++q)
-                o.w[q] = L.out[lane][q];
-        }
-        store_slots<S>(a.dst, out_v, o);
+    };
+    // Planes xb-1 .. xe+1 in pairs; an odd count ends with plane xe+2, which stages and gathers nothing
+    // (calling it unconditionally keeps the register sets free of merges at a branch).
+    for (int p = xb - 1; p <= xe + 1; p += 2) {
+        plane(p, st0, st1);
+        plane(p + 1, st1, st0);
     }
     // the block's tight list is final: the last barrier of the loop follows every append
     const uint32_t nt = min(L.n_tight, uint32_t(kTightLds));
