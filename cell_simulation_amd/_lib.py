@@ -24,6 +24,9 @@ CS_CNT_EXCEPTION = 1
 CS_CNT_NEAR_THRESHOLD = 2
 CS_CNT_FIRED = 3
 CS_CNT_ROLLBACK = 4
+CS_CNT_CME_EVENTS = 5
+CS_CNT_CME_NEAR = 6
+CS_CNT_EMIT_FAIL = 7
 CS_NUM_COUNTERS = 8
 
 
@@ -49,6 +52,8 @@ class SimInfoC(C.Structure):
         ("lambda_", C.c_float),
         ("dims", C.c_float * 3),
         ("device_ready", C.c_int),
+        ("num_cme_species", C.c_uint32),
+        ("num_cme_reactions", C.c_uint32),
     ]
 
 
@@ -113,6 +118,18 @@ SIGNATURES = {
     "cs_sim_pop_stat": (C.c_int, [_SIM, C.POINTER(StatSampleC)]),
     "cs_sim_read_counters": (C.c_int, [_SIM, C.c_void_p, C.c_size_t]),
     "cs_sim_reset_counters": (C.c_int, [_SIM]),
+    "cs_sim_add_cme_species": (C.c_int, [_SIM, C.c_char_p, C.c_int32]),
+    "cs_sim_add_cme_reaction": (C.c_int, [_SIM, C.POINTER(C.c_char_p), C.c_uint32, C.POINTER(C.c_char_p),
+                                          C.c_uint32, C.c_float]),
+    "cs_sim_set_cme_seed": (C.c_int, [_SIM, C.c_uint32]),
+    "cs_sim_read_cme_counts": (C.c_int, [_SIM, C.c_void_p, C.c_size_t]),
+    "cs_sim_cme_species_name": (C.c_int, [_SIM, C.c_uint32, C.c_char_p, C.c_size_t]),
+    "cs_sim_read_cme_tables": (C.c_int, [_SIM, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_void_p,
+                                         C.c_size_t, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t,
+                                         C.POINTER(C.c_uint32)]),
+    "cs_sim_set_cme_audit": (C.c_int, [_SIM, C.c_int]),
+    "cs_sim_read_cme_audit": (C.c_int, [_SIM, C.POINTER(C.c_double), C.POINTER(C.c_uint64),
+                                        C.POINTER(C.c_uint64)]),
 }
 
 _lib = None

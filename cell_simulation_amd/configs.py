@@ -9,6 +9,13 @@
   second-order reactions with log-uniform rates (1e-3..1e1 /s, 1e5..1e8 /M/s); about 2M particles
   placed uniformly by region from seed 1234; dt = 10 us. `edge` scales the lattice down for tests.
 
+* `gene_expression` -- 128^3 lattice, 16 nm voxels, S=8, dt = 20 us; extracellular space, a membrane shell
+  one voxel thick, cytoplasm and a nucleoid sphere of radius 20 voxels; well-mixed gene_off / gene_on
+  switching (0.01 /s on, 0.05 /s off) with transcription of mRNA into the nucleoid at 0.5 /s while on;
+  lattice mRNA (D = 5e-14 m^2/s, degraded at 0.01 /s, translated at 0.2 /s) and protein
+  (D = 2.5e-12 m^2/s, degraded at 0.001 /s) in the cytoplasm and nucleoid. `rate_scale` multiplies every
+  rate (tests use it to see events in few steps); `rate_factors` perturbs them per replicate.
+
 Bimolecular rates are converted from /M/s to per-voxel /s (k / (N_A * V_voxel * 1000 L/m^3)) before
 they reach the model: the reference applies k verbatim (res/shader/cme.wgsl:42-44).
 """
@@ -99,4 +106,44 @@ def large_cell(seed: int = 1234, edge: int = 256, slots: int = 16, n_particles: 
             p = int(rng.integers(n_species))
             sim.add_reaction([names[int(a)], names[int(b)]], [names[p]],
                              per_voxel_rate(loguniform(1e5, 1e8), lam))
+    return sim
+
+
+GENE_REGIONS = ("background", "membrane", "cytoplasm", "nucleoid")
+
+
+def gene_expression(seed: int = 7, edge: int = 128, slots: int = 8, tau: float = 20e-6, lam: float = 16e-9,
+                    rate_scale: float = 1.0, rate_factors=None, nucleoid_radius: float = 20.0,
+                    gene_copies: int = 1) -> Simulation:
+    """Hybrid gene expression: well-mixed gene switching, lattice mRNA and protein.
+
+    `rate_factors` (optional sequence of 7 positive factors) multiplies, in order: gene on, gene off,
+    transcription, mRNA degradation, translation, protein degradation, and both diffusion constants."""
+    f = list(rate_factors) if rate_factors is not None else [1.0] * 7
+    box = edge * lam
+    sim = Simulation(LatticeParams([box] * 3, [edge] * 3, tau, lam, slots))
+    sim.set_seed(seed)
+    c = [box / 2] * 3
+    r_ext = 0.45 * box
+    r_int = r_ext - lam
+    sim.add_region(RegionType.SphericalShell("membrane", "cytoplasm", c, r_int, r_ext), 0.0)
+    sim.add_region(RegionType.Sphere("nucleoid", c, nucleoid_radius * lam * edge / 128), 0.0)
+    sim.prepare_regions()
+    sim.add_particle_count("mRNA", "nucleoid", 0, True, False)
+    sim.add_particle_count("protein", "cytoplasm", 0, True, False)
+    d = {"mRNA": 5e-14 * f[6], "protein": 2.5e-12 * f[6]}
+    for sp, dc in d.items():
+        for reg in GENE_REGIONS:
+            sim.set_diffusion_rate_particle(sp, reg, dc if reg in ("cytoplasm", "nucleoid") else 0.0)
+        sim.set_transition_rate_particle(sp, "cytoplasm", "nucleoid", dc)
+        sim.set_transition_rate_particle(sp, "nucleoid", "cytoplasm", dc)
+    k = rate_scale
+    sim.add_reaction(["mRNA"], [], 0.01 * k * f[3])
+    sim.add_reaction(["mRNA"], ["mRNA", "protein"], 0.2 * k * f[4])
+    sim.add_reaction(["protein"], [], 0.001 * k * f[5])
+    sim.add_cme_species("gene_off", gene_copies)
+    sim.add_cme_species("gene_on", 0)
+    sim.add_cme_reaction(["gene_off"], ["gene_on"], 0.01 * k * f[0])
+    sim.add_cme_reaction(["gene_on"], ["gene_off"], 0.05 * k * f[1])
+    sim.add_cme_reaction(["gene_on"], ["gene_on", "mRNA@nucleoid"], 0.5 * k * f[2])
     return sim

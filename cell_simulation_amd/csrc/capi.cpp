@@ -335,6 +335,108 @@ int cs_sim_add_reaction(cs_sim *sim, const char *const *reactants, uint32_t n_re
     });
 }
 
+int cs_sim_add_cme_species(cs_sim *sim, const char *name, int32_t count)
+{
+    return guard([&] { builder(sim).add_cme_species(str(name, "name"), count); });
+}
+
+int cs_sim_add_cme_reaction(cs_sim *sim, const char *const *reactants, uint32_t n_reactants,
+                            const char *const *products, uint32_t n_products, float k)
+{
+    return guard([&] {
+        std::vector<std::string> r, p;
+        if (n_reactants)
+            need(reactants, "reactants");
+        if (n_products)
+            need(products, "products");
+        for (uint32_t i = 0; i < n_reactants; ++i)
+            r.push_back(str(reactants[i], "reactant"));
+        for (uint32_t i = 0; i < n_products; ++i)
+            p.push_back(str(products[i], "product"));
+        builder(sim).add_cme_reaction(r, p, k);
+    });
+}
+
+int cs_sim_set_cme_seed(cs_sim *sim, uint32_t seed)
+{
+    return guard([&] {
+        model_of(sim).set_cme_seed(seed);
+        if (sim->device)
+            sim->device->set_cme_seed(seed);
+    });
+}
+
+int cs_sim_read_cme_counts(cs_sim *sim, int32_t *out, size_t n)
+{
+    return guard([&] {
+        const cs::Model &m = model_of(sim);
+        if (n != m.num_cme_species())
+            throw cs::Error(CS_ERR_ARG, "buffer must hold num_cme_species entries");
+        if (n == 0)
+            return;
+        need(out, "out");
+        if (sim->device)
+            sim->device->read_cme_counts(out);
+        else
+            std::memcpy(out, m.cme_init().data(), n * sizeof(int32_t));
+    });
+}
+
+int cs_sim_cme_species_name(const cs_sim *sim, uint32_t index, char *out, size_t cap)
+{
+    return guard([&] {
+        need(out, "out");
+        const cs::Model &m = model_of(sim);
+        if (index >= m.num_cme_species())
+            throw cs::Error(CS_ERR_ARG, "well-mixed species index out of range");
+        if (cap == 0)
+            throw cs::Error(CS_ERR_ARG, "zero capacity");
+        std::snprintf(out, cap, "%s", m.cme_name(index).c_str());
+    });
+}
+
+int cs_sim_read_cme_tables(const cs_sim *sim, int32_t *reactants, size_t n_reactants, int32_t *delta,
+                           size_t n_delta, uint8_t *emit, size_t n_emit, int32_t *emit_region, size_t n_region,
+                           float *rates, size_t n_rates, uint32_t *seed)
+{
+    return guard([&] {
+        const cs::Model &m = model_of(sim);
+        auto copy = [](auto *dst, size_t n, const auto &src, const char *what) {
+            if (n != src.size())
+                throw cs::Error(CS_ERR_ARG, std::string(what) + ": size mismatch");
+            if (n) {
+                need(dst, what);
+                std::memcpy(dst, src.data(), n * sizeof(src[0]));
+            }
+        };
+        copy(reactants, n_reactants, m.cme_reactants(), "reactants");
+        copy(delta, n_delta, m.cme_delta(), "delta");
+        copy(emit, n_emit, m.cme_emit(), "emit");
+        copy(emit_region, n_region, m.cme_emit_region(), "emit_region");
+        copy(rates, n_rates, m.cme_rates(), "rates");
+        need(seed, "seed");
+        *seed = m.cme_seed();
+    });
+}
+
+int cs_sim_set_cme_audit(cs_sim *sim, int on)
+{
+    return guard([&] {
+        if (!sim || !sim->device)
+            throw cs::Error(CS_ERR_STATE, "cs_sim_prepare_device first");
+        sim->device->set_cme_audit(on != 0);
+    });
+}
+
+int cs_sim_read_cme_audit(cs_sim *sim, double *max_rel_error, uint64_t *divergences, uint64_t *events)
+{
+    return guard([&] {
+        if (!sim || !sim->device)
+            throw cs::Error(CS_ERR_STATE, "cs_sim_prepare_device first");
+        sim->device->read_cme_audit(max_rel_error, divergences, events);
+    });
+}
+
 int cs_sim_prepare_device(cs_sim *sim, int hip_device)
 {
     return guard([&] {
@@ -421,6 +523,8 @@ int cs_sim_get_info(const cs_sim *sim, cs_sim_info *out)
         out->tau = p.tau;
         out->lambda = p.lambda;
         out->device_ready = sim->device ? 1 : 0;
+        out->num_cme_species = m.num_cme_species();
+        out->num_cme_reactions = m.num_cme_reactions();
     });
 }
 

@@ -1268,6 +1268,151 @@ __global__ __launch_bounds__(kBlock) void rdme_step_tiled(StepArgs a, TileArgs t
             atomicAdd(a.counters + idx[k], (unsigned long long)vals[k]);
 }
 
+// ---- well-mixed (CME) phase --------------------------------------------------------------------------
+// No counterpart in the reference. Once per step, before the lattice step, the well-mixed counts of a
+// simulation advance by an exact SSA (Gillespie's direct method) over tau:
+//   a_r = k_r * f32(h_r), h_r = cme_h (binomials of the reactant counts);
+//   cumulative sums in f32 with Kahan compensation, read as f64 (sum - compensation);
+//   waiting time -log(u1) / a0 and the simulated time in f64; choice: first r with a_r > 0 and
+//   u2 * a0 < cumulative_r (u1 in (0, 1], u2 in [0, 1), 53 bits each, from cme_bits).
+// A chosen reaction's lattice products are placed at voxels of its region drawn with cme_place_draw,
+// up to 100 draws each, after the voxel's particles (the first draw with a free slot wins).
+constexpr uint32_t kCmeMaxEvents = 1u << 20;  // per step; more stop the step and count an exception
+
+struct CmeArgs {
+    uint32_t nc, ncr;
+    const int32_t *reactants;    // [ncr][3], -1 = none
+    const int32_t *delta;        // [ncr][nc]
+    const uint8_t *emit;         // [ncr][4] lattice species, 0 = none
+    const int32_t *emit_region;  // [ncr]
+    const float *rates;          // [ncr]
+    const uint32_t *lists;       // region voxel lists, concatenated
+    const uint32_t *list_off;    // [nreg]
+    const uint32_t *list_len;    // [nreg]
+    unsigned long long *counters;
+    unsigned long long *audit;   // [3]: max relative error (f64 bits), divergences, audited events
+    uint32_t audit_on;
+    float tau;
+};
+
+__device__ __forceinline__ void kahan_add(float &s, float &c, float x)
+{
+    const float y = x - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+template <int S>
+__device__ void cme_cell(const CmeArgs &a, int32_t *counts, uint8_t *lattice, uint32_t seed, uint32_t itime)
+{
+    const double tau = double(a.tau);
+    double t = 0.0, max_rel = 0.0;
+    uint32_t e = 0, near = 0, fails = 0, div = 0, bad = 0;
+    for (;;) {
+        float s = 0.0f, c = 0.0f;
+        double s64 = 0.0;
+        for (uint32_t r = 0; r < a.ncr; ++r) {
+            const double h = cme_h(a.reactants + 3u * r, counts);
+            kahan_add(s, c, a.rates[r] * float(h));
+            if (a.audit_on)
+                s64 = s64 + double(a.rates[r]) * h;
+        }
+        const double a0 = double(s) - double(c);
+        if (!(a0 > 0.0))
+            break;
+        if (e >= kCmeMaxEvents) {
+            bad = 1;
+            break;
+        }
+        const double u1 = double(cme_bits(itime, seed, 2u * e) + 1ull) * 0x1p-53;
+        t = t + (-log(u1)) / a0;
+        if (fabs(t - tau) <= tau * 1e-12)
+            ++near;
+        if (t >= tau)
+            break;
+        const double u2 = double(cme_bits(itime, seed, 2u * e + 1u)) * 0x1p-53;
+        const double target = u2 * a0, target64 = u2 * s64;
+        float s2 = 0.0f, c2 = 0.0f;
+        double cum64 = 0.0;
+        uint32_t chosen = a.ncr, last = a.ncr, chosen64 = a.ncr, last64 = a.ncr;
+        for (uint32_t r = 0; r < a.ncr; ++r) {
+            const double h = cme_h(a.reactants + 3u * r, counts);
+            const float ar = a.rates[r] * float(h);
+            kahan_add(s2, c2, ar);
+            if (ar > 0.0f) {
+                last = r;
+                if (chosen == a.ncr && target < double(s2) - double(c2))
+                    chosen = r;
+            }
+            if (a.audit_on) {
+                const double ar64 = double(a.rates[r]) * h;
+                cum64 = cum64 + ar64;
+                if (ar64 > 0.0) {
+                    last64 = r;
+                    if (chosen64 == a.ncr && target64 < cum64)
+                        chosen64 = r;
+                }
+            }
+        }
+        if (chosen == a.ncr)
+            chosen = last;
+        if (a.audit_on) {
+            if (chosen64 == a.ncr)
+                chosen64 = last64;
+            div += chosen64 != chosen;
+            const double rel = fabs(a0 - s64) / s64;
+            max_rel = rel > max_rel ? rel : max_rel;
+        }
+        for (uint32_t k = 0; k < a.nc; ++k)
+            counts[k] += a.delta[chosen * a.nc + k];
+        const int32_t rg = a.emit_region[chosen];
+        for (uint32_t j = 0; j < 4u; ++j) {
+            const uint32_t sp = a.emit[4u * chosen + j];
+            if (sp == 0u || rg < 0)
+                continue;
+            const uint32_t n = a.list_len[rg];
+            const uint32_t *list = a.lists + a.list_off[rg];
+            bool placed = false;
+            for (uint32_t att = 0; att < 100u && n > 0u && !placed; ++att) {
+                const uint32_t u = cme_place_draw(itime, seed, e, j, att);
+                const uint32_t v = list[uint32_t((uint64_t(u) * n) >> 32)];
+                Slots<S> sl = load_slots<S>(lattice, v);
+                const uint32_t occ = count_nonzero(sl);
+                if (occ < uint32_t(S)) {
+                    put_byte_dyn(sl, occ, sp);
+                    store_slots<S>(lattice, v, sl);
+                    placed = true;
+                }
+            }
+            fails += !placed;
+        }
+        ++e;
+    }
+    if (e)
+        atomicAdd(a.counters + CS_CNT_CME_EVENTS, (unsigned long long)e);
+    if (near)
+        atomicAdd(a.counters + CS_CNT_CME_NEAR, (unsigned long long)near);
+    if (fails)
+        atomicAdd(a.counters + CS_CNT_EMIT_FAIL, (unsigned long long)fails);
+    if (bad)
+        atomicAdd(a.counters + CS_CNT_EXCEPTION, 1ull);
+    if (a.audit_on && e) {
+        atomicMax(a.audit, (unsigned long long)__double_as_longlong(max_rel));  // non-negative: bits order like values
+        atomicAdd(a.audit + 1, (unsigned long long)div);
+        atomicAdd(a.audit + 2, (unsigned long long)e);
+    }
+}
+
+// One simulation's CME phase: a single lane (the pool is one well-mixed compartment).
+template <int S>
+__global__ __launch_bounds__(64) void cme_kernel(CmeArgs a, int32_t *counts, uint8_t *lattice, uint32_t seed,
+                                                 uint32_t itime)
+{
+    if (threadIdx.x == 0)
+        cme_cell<S>(a, counts, lattice, seed, itime);
+}
+
 // Per-region species histogram of the lattice (statistics; the reference keeps global counts only,
 // src/simulation.rs:361-388 and cme.wgsl:87,105).
 template <int S>
@@ -1479,6 +1624,49 @@ Device::Device(const Model &m, int hip_device) : dev_(hip_device)
     hip_check(hipMalloc(&fix_O_, size_t(8 * kMaxTight) * sizeof(uint32_t)), "hipMalloc fix_O");
     hip_check(hipMemsetAsync(fix_ctl_, 0, 4 * sizeof(uint32_t), st), "clear fix_ctl");
 
+    // Well-mixed (CME) pool and its tables in one blob (8-byte aligned sections), plus the voxel lists of
+    // the regions that receive lattice products.
+    nc_ = m.num_cme_species();
+    ncr_ = m.num_cme_reactions();
+    cme_seed_ = m.cme_seed();
+    has_cme_ = ncr_ > 0u;
+    hip_check(hipMalloc(&cme_counts_, size_t(nc_ > 0u ? nc_ : 1u) * sizeof(int32_t)), "hipMalloc cme counts");
+    hip_check(hipMalloc(&audit_, 3 * sizeof(unsigned long long)), "hipMalloc audit");
+    hip_check(hipMemsetAsync(audit_, 0, 3 * sizeof(unsigned long long), st), "clear audit");
+    if (nc_ > 0u)
+        hip_check(hipMemcpyAsync(cme_counts_, m.cme_init().data(), nc_ * sizeof(int32_t), hipMemcpyHostToDevice, st),
+                  "upload cme counts");
+    std::vector<uint8_t> cblob;
+    if (has_cme_) {
+        std::vector<uint32_t> lists, loff(nreg_, 0u), llen(nreg_, 0u);
+        for (uint32_t r = 0; r < ncr_; ++r) {
+            const int32_t rg = m.cme_emit_region()[r];
+            if (rg < 0 || llen[uint32_t(rg)] > 0u)
+                continue;
+            const std::vector<uint32_t> &vox = m.region_voxels(rg);
+            loff[uint32_t(rg)] = uint32_t(lists.size());
+            llen[uint32_t(rg)] = uint32_t(vox.size());
+            lists.insert(lists.end(), vox.begin(), vox.end());
+        }
+        auto put = [&](const void *src, size_t bytes) {
+            const uint32_t off = uint32_t(align8(cblob.size()));
+            cblob.resize(off + bytes);
+            if (bytes)
+                std::memcpy(cblob.data() + off, src, bytes);
+            return off;
+        };
+        off_creac_ = put(m.cme_reactants().data(), m.cme_reactants().size() * sizeof(int32_t));
+        off_cdelta_ = put(m.cme_delta().data(), m.cme_delta().size() * sizeof(int32_t));
+        off_cemit_ = put(m.cme_emit().data(), m.cme_emit().size());
+        off_cregion_ = put(m.cme_emit_region().data(), m.cme_emit_region().size() * sizeof(int32_t));
+        off_crates_ = put(m.cme_rates().data(), m.cme_rates().size() * sizeof(float));
+        off_clists_ = put(lists.data(), lists.size() * sizeof(uint32_t));
+        off_cloff_ = put(loff.data(), loff.size() * sizeof(uint32_t));
+        off_cllen_ = put(llen.data(), llen.size() * sizeof(uint32_t));
+        hip_check(hipMalloc(&cme_tab_, align8(cblob.size())), "hipMalloc cme tables");
+        hip_check(hipMemcpyAsync(cme_tab_, cblob.data(), cblob.size(), hipMemcpyHostToDevice, st), "upload cme tables");
+    }
+
     hip_check(hipMemcpyAsync(lat_[0], m.lattice().data(), N_ * S_, hipMemcpyHostToDevice, st), "upload lattice");
     hip_check(hipMemcpyAsync(reg_, reg8.data(), N_, hipMemcpyHostToDevice, st), "upload regions");
     hip_check(hipMemcpyAsync(tab_, blob.data(), blob_bytes, hipMemcpyHostToDevice, st), "upload tables");
@@ -1504,6 +1692,9 @@ Device::~Device()
     hipFree(fix_W_);
     hipFree(fix_L_);
     hipFree(fix_O_);
+    hipFree(cme_counts_);
+    hipFree(cme_tab_);
+    hipFree(audit_);
     if (stream_)
         hipStreamDestroy(static_cast<hipStream_t>(stream_));
 }
@@ -1541,6 +1732,30 @@ void Device::step(uint32_t itime)
     a.mod_zero = mod_zero_;
     a.tau = tau_;
     hipStream_t st = static_cast<hipStream_t>(stream_);
+    if (has_cme_) {
+        const uint8_t *cb = static_cast<const uint8_t *>(cme_tab_);
+        CmeArgs c;
+        c.nc = nc_;
+        c.ncr = ncr_;
+        c.reactants = reinterpret_cast<const int32_t *>(cb + off_creac_);
+        c.delta = reinterpret_cast<const int32_t *>(cb + off_cdelta_);
+        c.emit = cb + off_cemit_;
+        c.emit_region = reinterpret_cast<const int32_t *>(cb + off_cregion_);
+        c.rates = reinterpret_cast<const float *>(cb + off_crates_);
+        c.lists = reinterpret_cast<const uint32_t *>(cb + off_clists_);
+        c.list_off = reinterpret_cast<const uint32_t *>(cb + off_cloff_);
+        c.list_len = reinterpret_cast<const uint32_t *>(cb + off_cllen_);
+        c.counters = counters_;
+        c.audit = audit_;
+        c.audit_on = audit_on_ ? 1u : 0u;
+        c.tau = tau_;
+        switch (S_) {
+        case 4: hipLaunchKernelGGL(cme_kernel<4>, dim3(1), dim3(64), 0, st, c, cme_counts_, lat_[cur_], cme_seed_, itime); break;
+        case 8: hipLaunchKernelGGL(cme_kernel<8>, dim3(1), dim3(64), 0, st, c, cme_counts_, lat_[cur_], cme_seed_, itime); break;
+        default: hipLaunchKernelGGL(cme_kernel<16>, dim3(1), dim3(64), 0, st, c, cme_counts_, lat_[cur_], cme_seed_, itime); break;
+        }
+        hip_check(hipGetLastError(), "cme launch");
+    }
     if (simple_) {
         const uint32_t blocks = uint32_t((N_ + kBlock - 1) / kBlock);
         switch (S_) {
@@ -1627,6 +1842,32 @@ void Device::write_lattice(const uint8_t *in)
     hipStream_t st = static_cast<hipStream_t>(stream_);
     hip_check(hipMemcpyAsync(lat_[cur_], in, N_ * S_, hipMemcpyHostToDevice, st), "write lattice");
     hip_check(hipStreamSynchronize(st), "write lattice");
+}
+
+void Device::read_cme_counts(int32_t *out)
+{
+    hip_check(hipSetDevice(dev_), "hipSetDevice");
+    hipStream_t st = static_cast<hipStream_t>(stream_);
+    if (nc_ > 0u)
+        hip_check(hipMemcpyAsync(out, cme_counts_, nc_ * sizeof(int32_t), hipMemcpyDeviceToHost, st), "read cme counts");
+    hip_check(hipStreamSynchronize(st), "read cme counts");
+}
+
+void Device::read_cme_audit(double *max_rel_error, uint64_t *divergences, uint64_t *events)
+{
+    hip_check(hipSetDevice(dev_), "hipSetDevice");
+    hipStream_t st = static_cast<hipStream_t>(stream_);
+    unsigned long long h[3] = {0, 0, 0};
+    hip_check(hipMemcpyAsync(h, audit_, sizeof(h), hipMemcpyDeviceToHost, st), "read audit");
+    hip_check(hipStreamSynchronize(st), "read audit");
+    double rel;
+    std::memcpy(&rel, &h[0], sizeof(rel));
+    if (max_rel_error)
+        *max_rel_error = rel;
+    if (divergences)
+        *divergences = h[1];
+    if (events)
+        *events = h[2];
 }
 
 void Device::region_counts(int32_t *out)

@@ -38,6 +38,12 @@ public:
     uint64_t n_voxels() const { return N_; }
     uint32_t slots() const { return S_; }
 
+    // Well-mixed (CME) pool: counts, seed of the draws and the f64 precision audit.
+    void read_cme_counts(int32_t *out);
+    void set_cme_seed(uint32_t seed) { cme_seed_ = seed; }
+    void set_cme_audit(bool on) { audit_on_ = on; }
+    void read_cme_audit(double *max_rel_error, uint64_t *divergences, uint64_t *events);
+
 private:
     int dev_ = 0;
     void *stream_ = nullptr;       // hipStream_t
@@ -70,6 +76,15 @@ private:
     uint32_t *fix_W_ = nullptr;
     uint8_t *fix_L_ = nullptr;
     uint32_t *fix_O_ = nullptr;
+
+    bool has_cme_ = false;         // some well-mixed reaction exists
+    uint32_t nc_ = 0, ncr_ = 0, cme_seed_ = 0;
+    bool audit_on_ = false;
+    int32_t *cme_counts_ = nullptr;
+    void *cme_tab_ = nullptr;      // reactants, delta, emit, emit_region, rates, region voxel lists
+    uint32_t off_creac_ = 0, off_cdelta_ = 0, off_cemit_ = 0, off_cregion_ = 0, off_crates_ = 0;
+    uint32_t off_clists_ = 0, off_cloff_ = 0, off_cllen_ = 0;
+    unsigned long long *audit_ = nullptr;  // [3]
 };
 
 }  // namespace cs

@@ -399,6 +399,8 @@ bool Model::add_reservoir_site(uint64_t v, uint32_t sp)
 // stoichiometry matrix gains a zero column.
 uint32_t Model::new_species(const std::string &name, bool logging)
 {
+    if (find_cme(name) >= 0)
+        throw Error(CS_ERR_ARG, "name already used by a well-mixed species: " + name);
     const uint32_t nsp_old = static_cast<uint32_t>(particle_names_.size());
     if (nsp_old >= 255)
         throw Error(CS_ERR_UNSUPPORTED, "at most 254 species fit the one-byte slot encoding");
@@ -584,6 +586,95 @@ void Model::add_reaction(const std::vector<std::string> &reactants, const std::v
         ri.push_back(0);
     reactants_.insert(reactants_.end(), ri.begin(), ri.end());
     rates_.push_back(k);
+}
+
+// ---------------------------------------------------------------------------------------------------
+// Well-mixed (CME) species and reactions. No counterpart in the reference: the semantics are the
+// build's own (see the CME phase in device.hip).
+
+int Model::find_cme(const std::string &name) const
+{
+    for (size_t i = 0; i < cme_names_.size(); ++i)
+        if (cme_names_[i] == name)
+            return static_cast<int>(i);
+    return -1;
+}
+
+void Model::add_cme_species(const std::string &name, int32_t count)
+{
+    if (name.empty() || name.find('@') != std::string::npos)
+        throw Error(CS_ERR_ARG, "invalid well-mixed species name: " + name);
+    if (count < 0)
+        throw Error(CS_ERR_ARG, "well-mixed species counts are non-negative");
+    if (find_particle(name) >= 0)
+        throw Error(CS_ERR_ARG, "name already used by a lattice species: " + name);
+    const int i = find_cme(name);
+    if (i >= 0) {  // adding again raises the initial count, like add_particle_count on a known species
+        cme_init_[size_t(i)] += count;
+        return;
+    }
+    if (cme_names_.size() >= 255)
+        throw Error(CS_ERR_UNSUPPORTED, "at most 255 well-mixed species");
+    const size_t nc = cme_names_.size(), ncr = cme_rates_.size();
+    std::vector<int32_t> d(ncr * (nc + 1), 0);
+    for (size_t r = 0; r < ncr; ++r)
+        for (size_t k = 0; k < nc; ++k)
+            d[r * (nc + 1) + k] = cme_delta_[r * nc + k];
+    cme_delta_.swap(d);
+    cme_names_.push_back(name);
+    cme_init_.push_back(count);
+}
+
+void Model::add_cme_reaction(const std::vector<std::string> &reactants, const std::vector<std::string> &products,
+                             float k)
+{
+    if (!(k >= 0.0f) || !std::isfinite(k))
+        throw Error(CS_ERR_ARG, "well-mixed reaction rates must be finite and non-negative");
+    if (reactants.size() > 3)
+        throw Error(CS_ERR_ARG, "at most three reactants");
+    if (cme_rates_.size() >= 255)
+        throw Error(CS_ERR_UNSUPPORTED, "at most 255 well-mixed reactions");
+    const size_t nc = cme_names_.size();
+    std::vector<int32_t> ri;
+    std::vector<int32_t> delta(nc, 0);
+    for (const auto &r : reactants) {
+        const int i = find_cme(r);
+        if (i < 0)
+            throw Error(CS_ERR_NOT_FOUND, "well-mixed reactant not found: " + r);
+        ri.push_back(i);
+        delta[size_t(i)] -= 1;
+    }
+    uint8_t emit[4] = {0, 0, 0, 0};
+    int n_emit = 0, region = -1;
+    for (const auto &p : products) {
+        const size_t at = p.find('@');
+        if (at == std::string::npos) {
+            const int i = find_cme(p);
+            if (i < 0)
+                throw Error(CS_ERR_NOT_FOUND, "well-mixed product not found (lattice products are name@region): " + p);
+            delta[size_t(i)] += 1;
+            continue;
+        }
+        const int sp = find_particle(p.substr(0, at));
+        if (sp < 0)
+            throw Error(CS_ERR_NOT_FOUND, "lattice species not found: " + p.substr(0, at));
+        const int rg = find_region_index(p.substr(at + 1));
+        if (rg < 0)
+            throw Error(CS_ERR_NOT_FOUND, "region not found: " + p.substr(at + 1));
+        if (region >= 0 && rg != region)
+            throw Error(CS_ERR_UNSUPPORTED, "the lattice products of one reaction go to one region");
+        if (n_emit == 4)
+            throw Error(CS_ERR_UNSUPPORTED, "at most four lattice products per reaction");
+        region = rg;
+        emit[n_emit++] = uint8_t(sp);
+    }
+    while (ri.size() < 3)
+        ri.push_back(-1);
+    cme_reactants_.insert(cme_reactants_.end(), ri.begin(), ri.end());
+    cme_delta_.insert(cme_delta_.end(), delta.begin(), delta.end());
+    cme_emit_.insert(cme_emit_.end(), emit, emit + 4);
+    cme_emit_region_.push_back(region);
+    cme_rates_.push_back(k);
 }
 
 // ---------------------------------------------------------------------------------------------------

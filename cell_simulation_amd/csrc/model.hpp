@@ -82,7 +82,11 @@ public:
     uint32_t num_reactions() const { return static_cast<uint32_t>(rates_.size()) - 1u; }
     uint32_t num_regions() const { return static_cast<uint32_t>(region_types_.size()); }
 
-    void set_seed(uint64_t seed) { rng_ = SplitMix64(seed); }
+    void set_seed(uint64_t seed)
+    {
+        rng_ = SplitMix64(seed);
+        cme_seed_ = uint32_t(SplitMix64(seed ^ 0x434D4553ull).next());  // independent of the placement stream
+    }
 
     // ---- regions (simulation.rs:428-790) ----
     void add_region_cube(const std::string &name, const float p0[3], const float pf[3], float diffusion_rate);
@@ -122,6 +126,29 @@ public:
 
     // ---- reactions (simulation.rs:904-947) ----
     void add_reaction(const std::vector<std::string> &reactants, const std::vector<std::string> &products, float k);
+
+    // ---- well-mixed (CME) species: an extension without a counterpart in the reference ----
+    // Counts per simulation (one well-mixed pool), advanced once per step by an exact SSA over tau before
+    // the lattice step. Reactants are CME species; a product is a CME species ("name") or an RDME
+    // particle emitted into a region ("name@region"), placed at a random voxel of the region that has a
+    // free slot.
+    void add_cme_species(const std::string &name, int32_t count);
+    void add_cme_reaction(const std::vector<std::string> &reactants, const std::vector<std::string> &products,
+                          float k);
+    int find_cme(const std::string &name) const;  // -1 if absent
+    uint32_t num_cme_species() const { return static_cast<uint32_t>(cme_names_.size()); }
+    uint32_t num_cme_reactions() const { return static_cast<uint32_t>(cme_rates_.size()); }
+    const std::string &cme_name(uint32_t i) const { return cme_names_.at(i); }
+    const std::vector<int32_t> &cme_init() const { return cme_init_; }            // [nc]
+    const std::vector<int32_t> &cme_reactants() const { return cme_reactants_; }  // [ncr][3], -1 = none
+    const std::vector<int32_t> &cme_delta() const { return cme_delta_; }          // [ncr][nc]
+    const std::vector<uint8_t> &cme_emit() const { return cme_emit_; }            // [ncr][4], RDME species, 0 = none
+    const std::vector<int32_t> &cme_emit_region() const { return cme_emit_region_; }  // [ncr], -1 = none
+    const std::vector<float> &cme_rates() const { return cme_rates_; }            // [ncr]
+    uint32_t cme_seed() const { return cme_seed_; }
+    void set_cme_seed(uint32_t s) { cme_seed_ = s; }
+    // Voxels of a region in ascending order (the placement lists of region.rs:129-148).
+    const std::vector<uint32_t> &region_voxels(int region_idx) const { return index_list(region_idx); }
 
     // ---- tables in the reference's device layout ----
     // diffusion: [nreg][nreg][ns+1] (src, dst, species); stoich: [nr+1][ns+1]; reactants: [nr+1][3];
@@ -173,6 +200,15 @@ private:
     std::vector<int32_t> stoich_;       // [nr+1][nsp]
     std::vector<int32_t> reactants_;    // [nr+1][3]
     std::vector<float> rates_;          // [nr+1]
+
+    std::vector<std::string> cme_names_;
+    std::vector<int32_t> cme_init_;
+    std::vector<int32_t> cme_reactants_;
+    std::vector<int32_t> cme_delta_;
+    std::vector<uint8_t> cme_emit_;
+    std::vector<int32_t> cme_emit_region_;
+    std::vector<float> cme_rates_;
+    uint32_t cme_seed_ = uint32_t(SplitMix64(0x434D4553ull).next());
 
     std::vector<uint8_t> lattice_;
     std::vector<uint8_t> occupancy_;

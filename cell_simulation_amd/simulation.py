@@ -231,6 +231,55 @@ class Simulation:
         p = (C.c_char_p * max(1, len(products)))(*[s.encode() for s in products])
         check(lib().cs_sim_add_reaction(self._h, r, len(reactants), p, len(products), float(k)))
 
+    # ---- well-mixed (CME) species: no reference counterpart ----
+    def add_cme_species(self, name: str, count: int) -> None:
+        """A well-mixed species with an initial count (one pool per simulation)."""
+        check(lib().cs_sim_add_cme_species(self._h, name.encode(), int(count)))
+
+    def add_cme_reaction(self, reactants: Sequence[str], products: Sequence[str], k: float) -> None:
+        """A well-mixed reaction; products are well-mixed names or lattice particles as "name@region"."""
+        r = (C.c_char_p * max(1, len(reactants)))(*[s.encode() for s in reactants])
+        p = (C.c_char_p * max(1, len(products)))(*[s.encode() for s in products])
+        check(lib().cs_sim_add_cme_reaction(self._h, r, len(reactants), p, len(products), float(k)))
+
+    def set_cme_seed(self, seed: int) -> None:
+        check(lib().cs_sim_set_cme_seed(self._h, seed & 0xFFFFFFFF))
+
+    def cme_counts(self) -> np.ndarray:
+        n = self.info()["num_cme_species"]
+        out = np.zeros(max(n, 1), dtype=np.int32)
+        check(lib().cs_sim_read_cme_counts(self._h, out.ctypes.data, n))
+        return out[:n]
+
+    def cme_species_name(self, index: int) -> str:
+        buf = C.create_string_buffer(256)
+        check(lib().cs_sim_cme_species_name(self._h, index, buf, len(buf)))
+        return buf.value.decode()
+
+    def cme_tables(self) -> dict:
+        inf = self.info()
+        nc, ncr = inf["num_cme_species"], inf["num_cme_reactions"]
+        reac = np.zeros(max(3 * ncr, 1), dtype=np.int32)
+        delta = np.zeros(max(ncr * nc, 1), dtype=np.int32)
+        emit = np.zeros(max(4 * ncr, 1), dtype=np.uint8)
+        region = np.zeros(max(ncr, 1), dtype=np.int32)
+        rates = np.zeros(max(ncr, 1), dtype=np.float32)
+        seed = C.c_uint32(0)
+        check(lib().cs_sim_read_cme_tables(self._h, reac.ctypes.data, 3 * ncr, delta.ctypes.data, ncr * nc,
+                                           emit.ctypes.data, 4 * ncr, region.ctypes.data, ncr, rates.ctypes.data,
+                                           ncr, C.byref(seed)))
+        return {"reactants": reac[:3 * ncr].reshape(ncr, 3), "delta": delta[:ncr * nc].reshape(ncr, nc),
+                "emit": emit[:4 * ncr].reshape(ncr, 4), "emit_region": region[:ncr], "rates": rates[:ncr],
+                "seed": int(seed.value), "init": self.cme_counts() if not inf["device_ready"] else None}
+
+    def set_cme_audit(self, on: bool) -> None:
+        check(lib().cs_sim_set_cme_audit(self._h, 1 if on else 0))
+
+    def cme_audit(self) -> dict:
+        rel, div, ev = C.c_double(0.0), C.c_uint64(0), C.c_uint64(0)
+        check(lib().cs_sim_read_cme_audit(self._h, C.byref(rel), C.byref(div), C.byref(ev)))
+        return {"max_rel_error": float(rel.value), "divergences": int(div.value), "events": int(ev.value)}
+
     # ---- device ----
     def prepare_for_gpu(self, device: int = 0) -> None:
         """Simulation::prepare_for_gpu (src/simulation.rs:152-208) on HIP device `device`."""
@@ -272,6 +321,7 @@ class Simulation:
             "num_species": int(i.num_species), "num_reactions": int(i.num_reactions),
             "num_regions": int(i.num_regions), "tau": float(i.tau), "lambda": float(i.lambda_),
             "dims": tuple(i.dims), "device_ready": bool(i.device_ready),
+            "num_cme_species": int(i.num_cme_species), "num_cme_reactions": int(i.num_cme_reactions),
         }
 
     def lattice(self) -> np.ndarray:

@@ -107,6 +107,34 @@ int cs_sim_set_transition_rate_particle(cs_sim *sim, const char *particle, const
 int cs_sim_add_reaction(cs_sim *sim, const char *const *reactants, uint32_t n_reactants,
                         const char *const *products, uint32_t n_products, float k);
 
+/* ---- well-mixed (CME) species: NO reference counterpart ---------------------------------------------
+ * The reference has only lattice (RDME) species. These entry points add a well-mixed pool per
+ * simulation, advanced once per step, before the lattice step, by an exact SSA (Gillespie's direct
+ * method) over tau: propensities k*binomial(counts) in f32 with Kahan-compensated cumulative sums,
+ * simulated time and the total-propensity scale in f64. Reactants are well-mixed species; a product is a
+ * well-mixed species ("name") or a lattice particle emitted into a region ("name@region"), placed at a
+ * random voxel of that region that has a free slot (after the voxel's particles). Counts are per
+ * simulation; rates in 1/s (per-molecule for first order). For models that need gene switching and
+ * transcription into a compartment (a gene_on -> gene_on + mRNA@nucleoid row). */
+int cs_sim_add_cme_species(cs_sim *sim, const char *name, int32_t count);
+int cs_sim_add_cme_reaction(cs_sim *sim, const char *const *reactants, uint32_t n_reactants,
+                            const char *const *products, uint32_t n_products, float k);
+/* Seed of the CME draws (cs_sim_set_seed also derives one). */
+int cs_sim_set_cme_seed(cs_sim *sim, uint32_t seed);
+/* Current well-mixed counts, int32[num_cme_species]. */
+int cs_sim_read_cme_counts(cs_sim *sim, int32_t *out, size_t n);
+int cs_sim_cme_species_name(const cs_sim *sim, uint32_t index, char *out, size_t cap);
+/* Precision audit: when on, every CME event also evaluates the propensities and the choice in plain f64;
+ * the largest relative error of the f32 compensated total, the number of events whose f64 choice differs
+ * and the number of audited events accumulate until read. Off by default. */
+/* Well-mixed tables: reactants i32[ncr][3] (-1 = none), count changes i32[ncr][nc], lattice products
+ * u8[ncr][4] (0 = none) and their region i32[ncr] (-1 = none), rates f32[ncr]; and the draw seed. */
+int cs_sim_read_cme_tables(const cs_sim *sim, int32_t *reactants, size_t n_reactants, int32_t *delta,
+                           size_t n_delta, uint8_t *emit, size_t n_emit, int32_t *emit_region,
+                           size_t n_region, float *rates, size_t n_rates, uint32_t *seed);
+int cs_sim_set_cme_audit(cs_sim *sim, int on);
+int cs_sim_read_cme_audit(cs_sim *sim, double *max_rel_error, uint64_t *divergences, uint64_t *events);
+
 /* ---- device: Setup::new_nowindow (src/setup.rs:94-151) + Simulation::prepare_for_gpu
  *      (src/simulation.rs:152-208): uploads lattice, regions and tables to `hip_device`. ---------- */
 int cs_sim_prepare_device(cs_sim *sim, int hip_device);
@@ -137,6 +165,8 @@ typedef struct {
     float lambda;
     float dims[3];
     int device_ready;
+    uint32_t num_cme_species;    /* well-mixed species (no reference counterpart) */
+    uint32_t num_cme_reactions;
 } cs_sim_info;
 int cs_sim_get_info(const cs_sim *sim, cs_sim_info *out);
 
@@ -174,6 +204,9 @@ int cs_sim_pop_stat(cs_sim *sim, cs_stat_sample *out);
 #define CS_CNT_NEAR_THRESHOLD 2  /* voxel-steps with |r1 - (1-exp(-a0*tau))| <= 2^-24 */
 #define CS_CNT_FIRED 3           /* voxel-steps where a reaction fired */
 #define CS_CNT_ROLLBACK 4        /* fired reactions undone because S slots would fill */
+#define CS_CNT_CME_EVENTS 5      /* well-mixed reactions fired */
+#define CS_CNT_CME_NEAR 6        /* CME steps whose last waiting time ended within 1e-12*tau of tau */
+#define CS_CNT_EMIT_FAIL 7       /* lattice products of CME reactions dropped: no free slot in 100 draws */
 #define CS_NUM_COUNTERS 8
 int cs_sim_read_counters(cs_sim *sim, uint64_t *out, size_t n);
 int cs_sim_reset_counters(cs_sim *sim);

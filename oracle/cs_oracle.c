@@ -330,3 +330,161 @@ int cso_run(const cso_model *m, cso_state *st, const uint32_t *itime, uint32_t n
     }
     return 0;
 }
+
+/* ---- well-mixed (CME) phase: the build's own definition (cs_oracle.h), no reference counterpart ---- */
+
+double cso_cme_h(const int32_t *row, const int32_t *counts)
+{
+    double h = 1.0;
+    for (int j = 0; j < 3; ++j) {
+        int32_t id = row[j];
+        if (id < 0)
+            continue;
+        int seen = 0, m = 0;
+        for (int q = 0; q < 3; ++q) {
+            if (row[q] == id) {
+                ++m;
+                if (q < j)
+                    seen = 1;
+            }
+        }
+        if (seen)
+            continue;
+        double n = (double)counts[id];
+        if (m == 1)
+            h = h * n;
+        else if (m == 2)
+            h = h * (n * (n - 1.0) * 0.5);
+        else
+            h = h * (n * (n - 1.0) * (n - 2.0) / 6.0);
+        if (!(h > 0.0))
+            return 0.0;
+    }
+    return h;
+}
+
+uint64_t cso_cme_bits(uint32_t itime, uint32_t seed, uint32_t k)
+{
+    uint32_t a = cso_pcg(itime + cso_pcg(seed + 2u * k));
+    uint32_t b = cso_pcg(itime + cso_pcg(seed + 2u * k + 1u));
+    return (((uint64_t)a << 32) | b) >> 11;
+}
+
+static void kahan_add(float *s, float *c, float x)
+{
+    float y = x - *c;
+    float t = *s + y;
+    *c = (t - *s) - y;
+    *s = t;
+}
+
+double cso_cme_total32(const cso_cme_model *c, const int32_t *counts)
+{
+    float s = 0.0f, k = 0.0f;
+    for (uint32_t r = 0; r < c->ncr; ++r)
+        kahan_add(&s, &k, c->rates[r] * (float)cso_cme_h(c->reactants + 3u * r, counts));
+    return (double)s - (double)k;
+}
+
+uint32_t cso_cme_choice32(const cso_cme_model *c, const int32_t *counts, double u2)
+{
+    double target = u2 * cso_cme_total32(c, counts);
+    float s = 0.0f, k = 0.0f;
+    uint32_t chosen = c->ncr, last = c->ncr;
+    for (uint32_t r = 0; r < c->ncr; ++r) {
+        float ar = c->rates[r] * (float)cso_cme_h(c->reactants + 3u * r, counts);
+        kahan_add(&s, &k, ar);
+        if (ar > 0.0f) {
+            last = r;
+            if (chosen == c->ncr && target < (double)s - (double)k)
+                chosen = r;
+        }
+    }
+    return chosen == c->ncr ? last : chosen;
+}
+
+double cso_cme_total64(const cso_cme_model *c, const int32_t *counts)
+{
+    double s = 0.0;
+    for (uint32_t r = 0; r < c->ncr; ++r)
+        s = s + (double)c->rates[r] * cso_cme_h(c->reactants + 3u * r, counts);
+    return s;
+}
+
+uint32_t cso_cme_choice64(const cso_cme_model *c, const int32_t *counts, double u2)
+{
+    double target = u2 * cso_cme_total64(c, counts), cum = 0.0;
+    uint32_t chosen = c->ncr, last = c->ncr;
+    for (uint32_t r = 0; r < c->ncr; ++r) {
+        double ar = (double)c->rates[r] * cso_cme_h(c->reactants + 3u * r, counts);
+        cum = cum + ar;
+        if (ar > 0.0) {
+            last = r;
+            if (chosen == c->ncr && target < cum)
+                chosen = r;
+        }
+    }
+    return chosen == c->ncr ? last : chosen;
+}
+
+#define CSO_CME_MAX_EVENTS (1u << 20)
+
+void cso_cme_phase(const cso_model *m, const cso_cme_model *c, cso_state *st, int32_t *counts, uint32_t itime,
+                   cso_cme_counters *cnt)
+{
+    const double tau = (double)m->tau;
+    double t = 0.0;
+    uint32_t e = 0;
+    for (;;) {
+        double a0 = cso_cme_total32(c, counts);
+        if (!(a0 > 0.0))
+            break;
+        if (e >= CSO_CME_MAX_EVENTS) {
+            cnt->exceptions += 1;
+            break;
+        }
+        double u1 = (double)(cso_cme_bits(itime, c->seed, 2u * e) + 1ull) * 0x1p-53;
+        t = t + (-log(u1)) / a0;
+        if (fabs(t - tau) <= tau * 1e-12)
+            cnt->near += 1;
+        if (t >= tau)
+            break;
+        double u2 = (double)cso_cme_bits(itime, c->seed, 2u * e + 1u) * 0x1p-53;
+        uint32_t r = cso_cme_choice32(c, counts, u2);
+        for (uint32_t k = 0; k < c->nc; ++k)
+            counts[k] += c->delta[r * c->nc + k];
+        int32_t rg = c->emit_region[r];
+        for (uint32_t j = 0; j < 4u; ++j) {
+            uint32_t sp = c->emit[4u * r + j];
+            if (sp == 0u || rg < 0)
+                continue;
+            uint32_t n = c->region_len[rg];
+            const uint32_t *list = c->region_voxels[rg];
+            int placed = 0;
+            for (uint32_t att = 0; att < 100u && n > 0u && !placed; ++att) {
+                uint32_t u = cso_pcg(itime + cso_pcg((c->seed ^ 0x9E3779B9u) + ((e * 4u + j) << 7) + att));
+                uint32_t v = list[(uint32_t)(((uint64_t)u * n) >> 32)];
+                uint32_t occ = st->occupancy[v];
+                if (occ < m->S) {
+                    st->lattice[(uint64_t)v * m->S + occ] = sp;
+                    st->occupancy[v] = occ + 1u;
+                    st->counts[(uint64_t)v * (m->ns + 1u) + sp] += 1;
+                    st->stat[sp] += 1;
+                    placed = 1;
+                }
+            }
+            if (!placed)
+                cnt->emit_fail += 1;
+        }
+        ++e;
+    }
+    cnt->events += e;
+}
+
+int cso_step_hybrid(const cso_model *m, const cso_cme_model *c, cso_state *st, int32_t *counts, uint32_t itime,
+                    cso_counters *cnt, cso_cme_counters *ccnt)
+{
+    if (c && c->ncr > 0u)
+        cso_cme_phase(m, c, st, counts, itime, ccnt);
+    return cso_step(m, st, itime, cnt);
+}

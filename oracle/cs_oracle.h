@@ -83,6 +83,55 @@ void cso_probability_vector(const cso_model *m, uint32_t x, uint32_t y, uint32_t
 /* Multi-step driver: steps t = 0..nsteps-1 use itime[t]. */
 int cso_run(const cso_model *m, cso_state *st, const uint32_t *itime, uint32_t nsteps, cso_counters *cnt);
 
+/* ---- well-mixed (CME) phase: the build's own extension, NO reference counterpart -------------------
+ * Once per step, before the lattice step, the well-mixed counts advance by an exact SSA (Gillespie's
+ * direct method) over tau. The arithmetic below is the definition the HIP path follows bit for bit:
+ *   h_r = product over distinct reactant species of binomial(count, multiplicity), in double;
+ *   a_r = k_r * (float)h_r (f32); cumulative sums in f32 with Kahan compensation, read in double as
+ *   (double)sum - (double)compensation; a0 = that total;
+ *   draw k: bits = ((u64)pcg(itime + pcg(seed + 2k)) << 32 | pcg(itime + pcg(seed + 2k + 1))) >> 11;
+ *   event e: u1 = (bits(2e) + 1) * 2^-53, t += (-log(u1)) / a0, stop when t >= tau;
+ *            u2 = bits(2e + 1) * 2^-53, choice = first r with a_r > 0 and u2 * a0 < cumulative_r
+ *            (the last r with a_r > 0 if rounding leaves none);
+ *   lattice product j of event e goes to region voxel list[(u64)u * len >> 32] with
+ *            u = pcg(itime + pcg((seed ^ 0x9E3779B9) + ((4e + j) << 7) + attempt)), attempts 0..99,
+ *            appended after the voxel's particles when it has a free slot. */
+typedef struct {
+    uint32_t nc, ncr;                      /* well-mixed species, reactions */
+    const int32_t *reactants;              /* [ncr][3], -1 = none */
+    const int32_t *delta;                  /* [ncr][nc] */
+    const uint8_t *emit;                   /* [ncr][4] lattice species, 0 = none */
+    const int32_t *emit_region;            /* [ncr], -1 = none */
+    const float *rates;                    /* [ncr] */
+    const uint32_t *const *region_voxels;  /* [nreg] ascending voxel lists (NULL where unused) */
+    const uint32_t *region_len;            /* [nreg] */
+    uint32_t seed;
+} cso_cme_model;
+
+typedef struct {
+    uint64_t events;      /* reactions fired */
+    uint64_t near;        /* steps whose last waiting time ended within 1e-12*tau of tau */
+    uint64_t emit_fail;   /* lattice products dropped (no free slot in 100 draws) */
+    uint64_t exceptions;  /* steps stopped at the event cap */
+} cso_cme_counters;
+
+/* Combinatorial factor h_r and the 53-bit draw, as defined above. */
+double cso_cme_h(const int32_t *row3, const int32_t *counts);
+uint64_t cso_cme_bits(uint32_t itime, uint32_t seed, uint32_t k);
+/* The CME phase on the oracle state (counts[nc] updated in place; placements update lattice,
+ * occupancy, per-voxel counts and stat). */
+void cso_cme_phase(const cso_model *m, const cso_cme_model *c, cso_state *st, int32_t *counts, uint32_t itime,
+                   cso_cme_counters *cnt);
+/* Plain f64 reference of one event's total propensity and choice for counts and u2, for precision reports. */
+double cso_cme_total64(const cso_cme_model *c, const int32_t *counts);
+uint32_t cso_cme_choice64(const cso_cme_model *c, const int32_t *counts, double u2);
+/* The f32-compensated total and choice (the definition above), for the same comparison. */
+double cso_cme_total32(const cso_cme_model *c, const int32_t *counts);
+uint32_t cso_cme_choice32(const cso_cme_model *c, const int32_t *counts, double u2);
+/* A full hybrid step: CME phase, then cso_step. */
+int cso_step_hybrid(const cso_model *m, const cso_cme_model *c, cso_state *st, int32_t *counts, uint32_t itime,
+                    cso_counters *cnt, cso_cme_counters *ccnt);
+
 #ifdef __cplusplus
 }
 #endif

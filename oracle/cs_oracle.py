@@ -41,6 +41,21 @@ class Counters(C.Structure):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
 
 
+class CmeModel(C.Structure):
+    _fields_ = [
+        ("nc", C.c_uint32), ("ncr", C.c_uint32), ("reactants", C.c_void_p), ("delta", C.c_void_p),
+        ("emit", C.c_void_p), ("emit_region", C.c_void_p), ("rates", C.c_void_p), ("region_voxels", C.c_void_p),
+        ("region_len", C.c_void_p), ("seed", C.c_uint32),
+    ]
+
+
+class CmeCounters(C.Structure):
+    _fields_ = [("events", C.c_uint64), ("near", C.c_uint64), ("emit_fail", C.c_uint64), ("exceptions", C.c_uint64)]
+
+    def as_dict(self) -> dict:
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
 _lib = None
 
 
@@ -74,6 +89,19 @@ def lib() -> C.CDLL:
         d.cso_phase_c.argtypes = [C.POINTER(Model), C.POINTER(State), C.c_uint32, C.POINTER(Counters)]
         d.cso_phase_f.restype = None
         d.cso_phase_f.argtypes = [C.POINTER(Model), C.POINTER(State)]
+        d.cso_cme_h.restype = C.c_double
+        d.cso_cme_h.argtypes = [C.c_void_p, C.c_void_p]
+        d.cso_cme_bits.restype = C.c_uint64
+        d.cso_cme_bits.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
+        d.cso_cme_phase.restype = None
+        d.cso_cme_phase.argtypes = [C.POINTER(Model), C.POINTER(CmeModel), C.POINTER(State), C.c_void_p,
+                                    C.c_uint32, C.POINTER(CmeCounters)]
+        for nm in ("cso_cme_total64", "cso_cme_total32"):
+            getattr(d, nm).restype = C.c_double
+            getattr(d, nm).argtypes = [C.POINTER(CmeModel), C.c_void_p]
+        for nm in ("cso_cme_choice64", "cso_cme_choice32"):
+            getattr(d, nm).restype = C.c_uint32
+            getattr(d, nm).argtypes = [C.POINTER(CmeModel), C.c_void_p, C.c_double]
         _lib = d
     return _lib
 
@@ -133,21 +161,64 @@ class Oracle:
         st.stat = self.stat.ctypes.data
         self.state = st
         self.counters = Counters()
+        self.cme = None
+        self.cme_counters = CmeCounters()
+
+    def set_cme(self, counts, reactants, delta, emit, emit_region, rates, seed) -> None:
+        """Attach a well-mixed pool (the build's extension; see cso_cme_phase). Region voxel lists are the
+        ascending voxel indices of each label."""
+        self.cme_counts = np.ascontiguousarray(counts, dtype=np.int32).copy()
+        self.cme_reactants = np.ascontiguousarray(reactants, dtype=np.int32).ravel()
+        self.cme_delta = np.ascontiguousarray(delta, dtype=np.int32).ravel()
+        self.cme_emit = np.ascontiguousarray(emit, dtype=np.uint8).ravel()
+        self.cme_emit_region = np.ascontiguousarray(emit_region, dtype=np.int32).ravel()
+        self.cme_rates = np.ascontiguousarray(rates, dtype=np.float32).ravel()
+        ncr = self.cme_rates.size
+        self._lists = [np.ascontiguousarray(np.nonzero(self.regions == r)[0], dtype=np.uint32)
+                       for r in range(self.nreg)]
+        self._list_ptrs = (C.c_void_p * self.nreg)(*[l.ctypes.data for l in self._lists])
+        self._list_len = np.array([l.size for l in self._lists], dtype=np.uint32)
+        cm = CmeModel()
+        cm.nc, cm.ncr = self.cme_counts.size, ncr
+        cm.reactants = self.cme_reactants.ctypes.data
+        cm.delta = self.cme_delta.ctypes.data
+        cm.emit = self.cme_emit.ctypes.data
+        cm.emit_region = self.cme_emit_region.ctypes.data
+        cm.rates = self.cme_rates.ctypes.data
+        cm.region_voxels = C.cast(self._list_ptrs, C.c_void_p)
+        cm.region_len = self._list_len.ctypes.data
+        cm.seed = int(seed) & 0xFFFFFFFF
+        self.cme = cm
+
+    def cme_phase(self, itime: int) -> None:
+        lib().cso_cme_phase(C.byref(self.model), C.byref(self.cme), C.byref(self.state), self.cme_counts.ctypes.data,
+                            int(itime) & 0xFFFFFFFF, C.byref(self.cme_counters))
 
     @classmethod
     def from_simulation(cls, sim, mod_zero=0xFFFFFFFF) -> "Oracle":
         inf = sim.info()
         t = sim.tables()
-        return cls(inf["res"], inf["max_particles_site"], inf["tau"], inf["lambda"], t["diffusion"], t["stoich"],
-                   t["reactants"], t["rates"], sim.regions(), sim.lattice(), sim.reservoir(), mod_zero)
+        o = cls(inf["res"], inf["max_particles_site"], inf["tau"], inf["lambda"], t["diffusion"], t["stoich"],
+                t["reactants"], t["rates"], sim.regions(), sim.lattice(), sim.reservoir(), mod_zero)
+        if inf.get("num_cme_reactions", 0) > 0:
+            c = sim.cme_tables()
+            o.set_cme(sim.cme_counts(), c["reactants"], c["delta"], c["emit"], c["emit_region"], c["rates"],
+                      c["seed"])
+        return o
 
     def step(self, itime: int) -> None:
+        if self.cme is not None and self.cme.ncr > 0:
+            self.cme_phase(itime)
         rc = lib().cso_step(C.byref(self.model), C.byref(self.state), int(itime) & 0xFFFFFFFF,
                             C.byref(self.counters))
         if rc:
             raise RuntimeError(f"cso_step failed ({rc})")
 
     def run(self, itimes) -> None:
+        if self.cme is not None and self.cme.ncr > 0:
+            for t in np.asarray(itimes, dtype=np.uint32):
+                self.step(int(t))
+            return
         arr = np.ascontiguousarray(np.asarray(itimes, dtype=np.uint32))
         rc = lib().cso_run(C.byref(self.model), C.byref(self.state), arr.ctypes.data, arr.size,
                            C.byref(self.counters))
