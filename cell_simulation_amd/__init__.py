@@ -4,6 +4,7 @@ The hot path is libcellsim.so (HIP kernels + C ABI, include/cell_sim.h); this pa
 mirror of the reference's host API.
 """
 from ._lib import CellSimError, lib  # noqa: F401
+from .ensemble import Ensemble  # noqa: F401
 from .simulation import (  # noqa: F401
     LatticeParams,
     RegionType,

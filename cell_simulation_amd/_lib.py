@@ -57,6 +57,15 @@ class SimInfoC(C.Structure):
     ]
 
 
+class EnsTilingC(C.Structure):
+    _fields_ = [("xc", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("batch", C.c_uint32)]
+
+
+class EnsInfoC(C.Structure):
+    _fields_ = [("n_cells", C.c_uint32), ("max_particles_site", C.c_uint32), ("total_voxels", C.c_uint64),
+                ("steps_done", C.c_uint64), ("items", C.c_uint32)]
+
+
 class StatSampleC(C.Structure):
     _fields_ = [("name", C.c_char * 64), ("value", C.c_int32), ("iteration_count", C.c_uint32)]
 
@@ -128,6 +137,21 @@ SIGNATURES = {
                                          C.c_size_t, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t,
                                          C.POINTER(C.c_uint32)]),
     "cs_sim_set_cme_audit": (C.c_int, [_SIM, C.c_int]),
+    "cs_ens_create": (C.c_int, [C.POINTER(_SIM), C.c_uint32, C.POINTER(C.c_uint32), C.c_int, C.POINTER(_SIM)]),
+    "cs_ens_destroy": (None, [_SIM]),
+    "cs_ens_run": (C.c_int, [_SIM, C.c_uint32]),
+    "cs_ens_run_graph": (C.c_int, [_SIM, C.c_uint32, C.c_uint32]),
+    "cs_ens_run_timed": (C.c_int, [_SIM, C.c_uint32, C.c_uint32, C.POINTER(C.c_float)]),
+    "cs_ens_synchronize": (C.c_int, [_SIM]),
+    "cs_ens_set_tiling": (C.c_int, [_SIM, C.POINTER(EnsTilingC)]),
+    "cs_ens_get_tiling": (C.c_int, [_SIM, C.POINTER(EnsTilingC)]),
+    "cs_ens_autotune": (C.c_int, [_SIM, C.c_uint32, C.POINTER(C.c_float)]),
+    "cs_ens_get_info": (C.c_int, [_SIM, C.POINTER(EnsInfoC)]),
+    "cs_ens_cell_res": (C.c_int, [_SIM, C.c_uint32, C.POINTER(C.c_uint32)]),
+    "cs_ens_read_lattice": (C.c_int, [_SIM, C.c_uint32, C.c_void_p, C.c_size_t]),
+    "cs_ens_read_cme_counts": (C.c_int, [_SIM, C.c_uint32, C.c_void_p, C.c_size_t]),
+    "cs_ens_read_counters": (C.c_int, [_SIM, C.c_void_p, C.c_size_t]),
+    "cs_ens_reset_counters": (C.c_int, [_SIM]),
     "cs_sim_read_cme_audit": (C.c_int, [_SIM, C.POINTER(C.c_double), C.POINTER(C.c_uint64),
                                         C.POINTER(C.c_uint64)]),
 }

@@ -13,6 +13,7 @@
 
 #include "../../include/cell_sim.h"
 #include "device.hpp"
+#include "ensemble.hpp"
 #include "model.hpp"
 
 struct cs_sim {
@@ -690,3 +691,156 @@ int cs_sim_reset_counters(cs_sim *sim)
 }
 
 }  // extern "C"
+
+// ---- batched ensembles ---------------------------------------------------------------------------------
+struct cs_ens {
+    std::unique_ptr<cs::Ensemble> ens;
+};
+
+namespace {
+cs::Ensemble &ens_of(cs_ens *e)
+{
+    need(e, "ensemble");
+    return *e->ens;
+}
+const cs::Ensemble &ens_of(const cs_ens *e)
+{
+    need(e, "ensemble");
+    return *e->ens;
+}
+}  // namespace
+
+int cs_ens_create(cs_sim *const *cells, uint32_t n_cells, const uint32_t *itime_seeds, int hip_device, cs_ens **out)
+{
+    return guard([&] {
+        need(out, "out");
+        *out = nullptr;
+        if (n_cells == 0)
+            throw cs::Error(CS_ERR_ARG, "an ensemble needs at least one cell");
+        need(cells, "cells");
+        need(itime_seeds, "itime_seeds");
+        std::vector<const cs::Model *> models;
+        for (uint32_t i = 0; i < n_cells; ++i)
+            models.push_back(&model_of(cells[i]));
+        std::vector<uint32_t> seeds(itime_seeds, itime_seeds + n_cells);
+        auto e = std::make_unique<cs_ens>();
+        e->ens = std::make_unique<cs::Ensemble>(models, seeds, hip_device);
+        *out = e.release();
+    });
+}
+
+void cs_ens_destroy(cs_ens *ens) { delete ens; }
+
+int cs_ens_run(cs_ens *ens, uint32_t nsteps)
+{
+    return guard([&] { ens_of(ens).run(nsteps); });
+}
+
+int cs_ens_run_graph(cs_ens *ens, uint32_t nsteps, uint32_t block)
+{
+    return guard([&] { ens_of(ens).run_graph(nsteps, block); });
+}
+
+int cs_ens_run_timed(cs_ens *ens, uint32_t nsteps, uint32_t block, float *ms)
+{
+    return guard([&] {
+        need(ms, "ms");
+        *ms = ens_of(ens).run_timed(nsteps, block);
+    });
+}
+
+int cs_ens_synchronize(cs_ens *ens)
+{
+    return guard([&] { ens_of(ens).synchronize(); });
+}
+
+int cs_ens_set_tiling(cs_ens *ens, const cs_ens_tiling *t)
+{
+    return guard([&] {
+        need(t, "tiling");
+        ens_of(ens).set_tiling(cs::EnsembleTiling{t->xc, t->blocks_per_cu, t->batch});
+    });
+}
+
+int cs_ens_get_tiling(const cs_ens *ens, cs_ens_tiling *t)
+{
+    return guard([&] {
+        need(t, "tiling");
+        const cs::EnsembleTiling &x = ens_of(ens).tiling();
+        t->xc = x.xc;
+        t->blocks_per_cu = x.blocks_per_cu;
+        t->batch = x.batch;
+    });
+}
+
+int cs_ens_autotune(cs_ens *ens, uint32_t trial_steps, float *ms_per_step)
+{
+    return guard([&] {
+        if (trial_steps == 0)
+            throw cs::Error(CS_ERR_ARG, "trial_steps must be positive");
+        const float ms = ens_of(ens).autotune(trial_steps);
+        if (ms_per_step)
+            *ms_per_step = ms;
+    });
+}
+
+int cs_ens_get_info(const cs_ens *ens, cs_ens_info *out)
+{
+    return guard([&] {
+        need(out, "out");
+        const cs::Ensemble &e = ens_of(ens);
+        out->n_cells = e.n_cells();
+        out->max_particles_site = e.slots();
+        out->total_voxels = e.total_voxels();
+        out->steps_done = e.steps_done();
+        out->items = e.items();
+    });
+}
+
+int cs_ens_cell_res(const cs_ens *ens, uint32_t cell, uint32_t res[3])
+{
+    return guard([&] {
+        need(res, "res");
+        ens_of(ens).cell_res(cell, res);
+    });
+}
+
+int cs_ens_read_lattice(cs_ens *ens, uint32_t cell, uint8_t *out, size_t nbytes)
+{
+    return guard([&] {
+        need(out, "out");
+        cs::Ensemble &e = ens_of(ens);
+        if (nbytes != e.cell_voxels(cell) * e.slots())
+            throw cs::Error(CS_ERR_ARG, "buffer must hold the cell's N*S bytes");
+        e.read_lattice(cell, out);
+    });
+}
+
+int cs_ens_read_cme_counts(cs_ens *ens, uint32_t cell, int32_t *out, size_t n)
+{
+    return guard([&] {
+        cs::Ensemble &e = ens_of(ens);
+        if (cell >= e.n_cells())
+            throw cs::Error(CS_ERR_ARG, "cell index out of range");
+        if (n != e.cell_cme_species(cell))
+            throw cs::Error(CS_ERR_ARG, "buffer must hold the cell's well-mixed species");
+        if (n)
+            need(out, "out");
+        e.read_cme_counts(cell, out);
+    });
+}
+
+int cs_ens_read_counters(cs_ens *ens, uint64_t *out, size_t n)
+{
+    return guard([&] {
+        need(out, "out");
+        if (n != CS_NUM_COUNTERS)
+            throw cs::Error(CS_ERR_ARG, "counter buffer must hold CS_NUM_COUNTERS entries");
+        ens_of(ens).read_counters(out);
+    });
+}
+
+int cs_ens_reset_counters(cs_ens *ens)
+{
+    return guard([&] { ens_of(ens).reset_counters(); });
+}

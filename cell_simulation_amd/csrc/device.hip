@@ -28,6 +28,7 @@
 #include "../../include/cell_sim.h"
 #include "cs_rng.h"
 #include "device.hpp"
+#include "ensemble.hpp"
 
 namespace cs {
 
@@ -61,6 +62,7 @@ struct StepArgs {
     uint8_t *fix_L;       // slots of the affected voxels during the replay
     uint32_t *fix_O;      // their occupancy during the replay
     uint32_t max_tight;
+    uint32_t vox_base;    // index of this lattice's voxel 0 among all voxels on the device (ensembles)
     uint32_t R0, R1, R2, R12;
     uint32_t N;
     uint32_t ns, nr, nreg, nrw;
@@ -622,11 +624,11 @@ __device__ __forceinline__ void counter_add(unsigned long long *c, int idx, long
         atomicAdd(c + idx, (unsigned long long)d);
 }
 
+// Replays the `nt_all` tight voxels listed at a.fix_list (voxel indices of the lattice in `a`).
 template <int S>
-__device__ __noinline__ void fixup_tight(const StepArgs &a)
+__device__ __noinline__ void fixup_tight(const StepArgs &a, uint32_t nt_all)
 {
     const Tab tb = global_tab(a);
-    const uint32_t nt_all = a.fix_ctl[0];
     if (nt_all > a.max_tight) {
         counter_add(a.counters, CS_CNT_EXCEPTION, (long long)nt_all);  // too many to replay
         return;
@@ -743,7 +745,7 @@ __device__ __forceinline__ void note_tight(const StepArgs &a, uint32_t v)
 {
     const uint32_t i = atomicAdd(a.fix_ctl, 1u);
     if (i < a.max_tight)
-        a.fix_list[i] = v;
+        a.fix_list[i] = a.vox_base + v;
 }
 
 // One block launched after every step kernel on the same stream: replays the tight neighbourhoods noted
@@ -755,7 +757,7 @@ __global__ __launch_bounds__(64) void tight_replay_kernel(StepArgs a)
 {
     if (threadIdx.x != 0 || a.fix_ctl[0] == 0u)
         return;
-    fixup_tight<S>(a);
+    fixup_tight<S>(a, a.fix_ctl[0]);
     a.fix_ctl[0] = 0u;
 }
 
@@ -973,15 +975,17 @@ __device__ __forceinline__ uint32_t take_arrivals(const typename SlotVec<S>::T &
     return n;
 }
 
+// Event counts a block accumulates over the tiles it marches.
+struct TileCounts {
+    uint32_t tight = 0, bad = 0, near = 0, fired = 0, rolled = 0;
+};
+
+// One tile (tz, ty) of the lattice described by (a, t), marched through x-chunk `chunk`. Every lane of the
+// block calls it; it starts and ends with the block's LDS free for the next tile.
 template <int S>
-__global__ __launch_bounds__(kBlock) void rdme_step_tiled(StepArgs a, TileArgs t)
+__device__ __forceinline__ void tile_march(TileLds<S> &L, uint32_t *tab_lds, const StepArgs &a, const TileArgs &t,
+                                           uint32_t tz, uint32_t ty, uint32_t chunk, TileCounts &tc)
 {
-    __shared__ TileLds<S> L;
-    extern __shared__ uint32_t tab_lds[];
-    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
-    const uint32_t tz = blk % t.tiles_z;
-    const uint32_t ty = (blk / t.tiles_z) % t.tiles_y;
-    const uint32_t chunk = blk / (t.tiles_z * t.tiles_y);
     const int y0 = int(ty) * kTY, z0 = int(tz) * kTZ;
     const int xb = int(chunk * t.xc);
     const int xe = min(int(a.R0), xb + int(t.xc));
@@ -1073,6 +1077,7 @@ __global__ __launch_bounds__(kBlock) void rdme_step_tiled(StepArgs a, TileArgs t
     fetch_labels(xb, st0.lab);
 
     uint32_t n_tight = 0, n_bad = 0, n_near = 0, n_fired = 0, n_rolled = 0;
+    __syncthreads();  // the tables and the first labels are in LDS
 
     auto plane = [&](int p, Staged &use, Staged &fill) __attribute__((always_inline)) {
         const int rp = ring(p);
@@ -1260,12 +1265,37 @@ __global__ __launch_bounds__(kBlock) void rdme_step_tiled(StepArgs a, TileArgs t
     const uint32_t nt = min(L.n_tight, uint32_t(kTightLds));
     for (uint32_t i = uint32_t(lane); i < nt; i += kBlock)
         note_tight(a, L.tight[i]);
-    const uint32_t vals[5] = {n_tight, n_bad, n_near, n_fired, n_rolled};
+    tc.tight += n_tight;
+    tc.bad += n_bad;
+    tc.near += n_near;
+    tc.fired += n_fired;
+    tc.rolled += n_rolled;
+    __syncthreads();  // every lane is done with the LDS before the next tile reuses it
+}
+
+__device__ __forceinline__ void flush_counts(unsigned long long *counters, const TileCounts &tc)
+{
+    const uint32_t vals[5] = {tc.tight, tc.bad, tc.near, tc.fired, tc.rolled};
     const int idx[5] = {CS_CNT_TIGHT, CS_CNT_EXCEPTION, CS_CNT_NEAR_THRESHOLD, CS_CNT_FIRED, CS_CNT_ROLLBACK};
 #pragma unroll
-    for (int k = 0; k < 5; ++k)
+    for (int k = 0; k < 5; ++k)  // events are rare: one atomic per lane that saw any
         if (vals[k])
-            atomicAdd(a.counters + idx[k], (unsigned long long)vals[k]);
+            atomicAdd(counters + idx[k], (unsigned long long)vals[k]);
+}
+
+// The plain step: one block per (tile, x-chunk) of one lattice.
+template <int S>
+__global__ __launch_bounds__(kBlock) void rdme_step_tiled(StepArgs a, TileArgs t)
+{
+    __shared__ TileLds<S> L;
+    extern __shared__ uint32_t tab_lds[];
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t tz = blk % t.tiles_z;
+    const uint32_t ty = (blk / t.tiles_z) % t.tiles_y;
+    const uint32_t chunk = blk / (t.tiles_z * t.tiles_y);
+    TileCounts tc;
+    tile_march<S>(L, tab_lds, a, t, tz, ty, chunk, tc);
+    flush_counts(a.counters, tc);
 }
 
 // ---- well-mixed (CME) phase --------------------------------------------------------------------------
@@ -1442,6 +1472,209 @@ __global__ __launch_bounds__(kBlock) void region_count_kernel(const uint8_t *lat
             atomicAdd(&hist[i], sh[i]);
 }
 
+
+// ---- batched ensembles (see ensemble.hpp) ------------------------------------------------------------
+struct EnsCell {
+    uint64_t lat_off;          // byte offset of the cell's slots in each lattice buffer
+    uint32_t vox_off;          // index of the cell's voxel 0 among all voxels (labels, tight list)
+    uint32_t R0, R1, R2;
+    uint32_t tiles_z, tiles_y, chunks, first_item;
+    uint32_t tab_off, tab_words;  // the cell's table blob, in 32-bit words of the tables buffer
+    uint32_t off_p2, off_rates, off_ridx, off_stoich, off_rmask, off_bound;
+    uint32_t ns, nr, nreg, nrw, has_zero_order, mod_zero;
+    float tau;
+    uint32_t itime_seed;       // itime(step) = low 32 bits of SplitMix64((itime_seed << 32) + step)
+    int32_t cme;               // the cell's well-mixed pool (index of its CmeArgs), -1 = none
+    uint32_t cme_counts_off, cme_seed;
+};
+
+struct EnsArgs {
+    const EnsCell *cells;
+    uint32_t n_cells, total_items, xc, batch;
+    uint8_t *src, *dst;
+    const uint8_t *reg;
+    const uint32_t *tab;
+    uint32_t *itime;           // [n_cells], written by the prologue of each step
+    uint32_t *queue;           // [0] next item, [1] step counter
+    unsigned long long *counters;
+    uint32_t *fix_ctl, *fix_list, *fix_F, *fix_W, *fix_O;
+    uint8_t *fix_L;
+    uint32_t max_tight;
+};
+
+__device__ __forceinline__ uint32_t splitmix_low32(uint64_t x)
+{
+    uint64_t z = x + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return uint32_t(z ^ (z >> 31));
+}
+
+__device__ __forceinline__ StepArgs cell_step_args(const EnsArgs &e, const EnsCell &c, uint32_t itime)
+{
+    StepArgs a;
+    a.src = e.src + c.lat_off;
+    a.dst = e.dst + c.lat_off;
+    a.reg = e.reg + c.vox_off;
+    const uint8_t *tb = reinterpret_cast<const uint8_t *>(e.tab + c.tab_off);
+    a.p2 = reinterpret_cast<const float *>(tb + c.off_p2);
+    a.rates = reinterpret_cast<const float *>(tb + c.off_rates);
+    a.ridx = tb + c.off_ridx;
+    a.stoich = reinterpret_cast<const int8_t *>(tb + c.off_stoich);
+    a.rmask = reinterpret_cast<const uint64_t *>(tb + c.off_rmask);
+    a.bound = reinterpret_cast<const float *>(tb + c.off_bound);
+    a.counters = e.counters;
+    a.fix_ctl = e.fix_ctl;
+    a.fix_list = e.fix_list;
+    a.fix_F = e.fix_F;
+    a.fix_W = e.fix_W;
+    a.fix_L = e.fix_L;
+    a.fix_O = e.fix_O;
+    a.max_tight = e.max_tight;
+    a.vox_base = c.vox_off;
+    a.R0 = c.R0;
+    a.R1 = c.R1;
+    a.R2 = c.R2;
+    a.R12 = c.R1 * c.R2;
+    a.N = c.R0 * c.R1 * c.R2;
+    a.ns = c.ns;
+    a.nr = c.nr;
+    a.nreg = c.nreg;
+    a.nrw = c.nrw;
+    a.itime = itime;
+    a.mod_zero = c.mod_zero;
+    a.tau = c.tau;
+    return a;
+}
+
+__device__ __forceinline__ TileArgs cell_tile_args(const EnsArgs &e, const EnsCell &c)
+{
+    TileArgs t;
+    t.tiles_z = c.tiles_z;
+    t.tiles_y = c.tiles_y;
+    t.chunks = c.chunks;
+    t.xc = e.xc;
+    t.has_zero_order = c.has_zero_order;
+    t.tab_words = c.tab_words;
+    t.off_p2 = c.off_p2;
+    t.off_rates = c.off_rates;
+    t.off_ridx = c.off_ridx;
+    t.off_stoich = c.off_stoich;
+    t.off_rmask = c.off_rmask;
+    t.off_bound = c.off_bound;
+    t.tab = e.tab + c.tab_off;
+    return t;
+}
+
+// Per step, one lane per cell: the cell's itime from the device step counter, then its well-mixed phase.
+template <int S>
+__global__ __launch_bounds__(64) void ens_prologue(EnsArgs e, const CmeArgs *cme, int32_t *cme_counts)
+{
+    const uint32_t c = blockIdx.x * 64u + threadIdx.x;
+    if (c >= e.n_cells)
+        return;
+    const EnsCell &cell = e.cells[c];
+    const uint32_t it = splitmix_low32((uint64_t(cell.itime_seed) << 32) + e.queue[1]);
+    e.itime[c] = it;
+    if (cell.cme >= 0)
+        cme_cell<S>(cme[cell.cme], cme_counts + cell.cme_counts_off, e.src + cell.lat_off, cell.cme_seed, it);
+}
+
+// The lattice step of every cell: persistent blocks take (cell, tile, x-chunk) items from the queue
+// (`batch` consecutive items per access) or, with batch 0, in a fixed round-robin.
+template <int S>
+__global__ __launch_bounds__(kBlock) void rdme_step_ensemble(EnsArgs e)
+{
+    __shared__ TileLds<S> L;
+    __shared__ uint32_t next;
+    extern __shared__ uint32_t tab_lds[];
+    TileCounts tc;
+    const uint32_t per = e.batch ? e.batch : 1u;
+    uint32_t rr = blockIdx.x;  // round-robin position when there is no queue
+    for (;;) {
+        if (e.batch) {
+            if (threadIdx.x == 0)
+                next = atomicAdd(e.queue, e.batch);
+            __syncthreads();
+        } else if (threadIdx.x == 0) {
+            next = rr;
+        }
+        if (!e.batch)
+            __syncthreads();
+        const uint32_t first = __builtin_amdgcn_readfirstlane(next);
+        __syncthreads();  // every lane has read `next` before it is written again
+        if (first >= e.total_items)
+            break;
+        rr += gridDim.x;
+        const uint32_t last = min(first + per, e.total_items);
+        for (uint32_t item = first; item < last; ++item) {
+            uint32_t lo = 0, hi = e.n_cells;  // the cell whose items contain `item`
+            while (hi - lo > 1u) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (e.cells[mid].first_item <= item)
+                    lo = mid;
+                else
+                    hi = mid;
+            }
+            const EnsCell &c = e.cells[lo];
+            const StepArgs a = cell_step_args(e, c, e.itime[lo]);
+            const TileArgs t = cell_tile_args(e, c);
+            const uint32_t local = item - c.first_item;
+            const uint32_t tz = local % c.tiles_z, ty = (local / c.tiles_z) % c.tiles_y;
+            const uint32_t chunk = local / (c.tiles_z * c.tiles_y);
+            tile_march<S>(L, tab_lds, a, t, tz, ty, chunk, tc);
+        }
+    }
+    flush_counts(e.counters, tc);
+}
+
+// After each step: replays the tight voxels cell by cell, re-arms the queue, advances the step counter.
+template <int S>
+__global__ __launch_bounds__(64) void ens_epilogue(EnsArgs e)
+{
+    if (threadIdx.x != 0)
+        return;
+    const uint32_t nt = e.fix_ctl[0];
+    if (nt > e.max_tight) {
+        counter_add(e.counters, CS_CNT_EXCEPTION, (long long)nt);
+    } else if (nt > 0u) {
+        uint32_t *T = e.fix_list;
+        for (uint32_t i = 1; i < nt; ++i) {  // insertion sort: cells' voxel ranges become contiguous
+            const uint32_t x = T[i];
+            uint32_t j = i;
+            while (j > 0u && T[j - 1u] > x) {
+                T[j] = T[j - 1u];
+                --j;
+            }
+            T[j] = x;
+        }
+        uint32_t i = 0;
+        while (i < nt) {
+            uint32_t lo = 0, hi = e.n_cells;
+            while (hi - lo > 1u) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (e.cells[mid].vox_off <= T[i])
+                    lo = mid;
+                else
+                    hi = mid;
+            }
+            const EnsCell &c = e.cells[lo];
+            const uint32_t end = c.vox_off + c.R0 * c.R1 * c.R2;
+            uint32_t j = i;
+            while (j < nt && T[j] < end) {
+                T[j] -= c.vox_off;
+                ++j;
+            }
+            StepArgs a = cell_step_args(e, c, e.itime[lo]);
+            a.fix_list = T + i;
+            fixup_tight<S>(a, j - i);
+            i = j;
+        }
+    }
+    e.fix_ctl[0] = 0u;
+    e.queue[0] = 0u;
+    e.queue[1] += 1u;
+}
 }  // namespace
 
 // Upper bound of a voxel's total propensity sum_r k_r * prod(counts of r's reactants) over every voxel
@@ -1507,6 +1740,149 @@ static std::vector<float> propensity_bounds(const Model &m, uint32_t S)
     return out;
 }
 
+// Checks that the model fits the device encodings (one-byte labels and slots, int8 stoichiometry).
+static void check_device_model(const Model &m)
+{
+    if (m.has_reservoir())
+        throw Error(CS_ERR_UNSUPPORTED, "reservoir particles are not supported by the HIP path");
+    if (m.num_regions() > 255)
+        throw Error(CS_ERR_UNSUPPORTED, "at most 255 regions fit the one-byte label");
+    if (m.num_reactions() > 255)
+        throw Error(CS_ERR_UNSUPPORTED, "at most 255 reactions");
+    if (m.n_voxels() * m.slots() >= (uint64_t(1) << 32))
+        throw Error(CS_ERR_UNSUPPORTED, "lattice too large for u32 slot indices");
+    for (float k : m.rates())
+        if (!std::isfinite(k))
+            throw Error(CS_ERR_UNSUPPORTED, "reaction rates must be finite");
+    for (int32_t st : m.stoich())
+        if (st < -127 || st > 127)
+            throw Error(CS_ERR_UNSUPPORTED, "stoichiometry entries must fit int8");
+}
+
+static size_t align8(size_t n) { return (n + 7u) & ~size_t(7); }
+
+// The lattice-step tables of a model in one blob (8-byte aligned sections) that the tiled kernel copies
+// into LDS as is: hop probabilities (D*tau)/(lambda*lambda) in f32 as rdme.wgsl:135 evaluates them,
+// rates, reactant bytes, candidate masks, int8 stoichiometry and the propensity bounds.
+struct TableBlob {
+    std::vector<uint8_t> bytes;
+    uint32_t off_p2 = 0, off_rates = 0, off_ridx = 0, off_rmask = 0, off_stoich = 0, off_bound = 0;
+    uint32_t nrw = 0;
+    bool has_zero_order = false;
+};
+
+static TableBlob make_table_blob(const Model &m)
+{
+    const LatticeParams &lp = m.params();
+    const uint32_t ns = m.num_species(), nr = m.num_reactions(), n1 = ns + 1u;
+    TableBlob b;
+    b.nrw = (nr + 1u + 63u) / 64u;
+    std::vector<float> p2(m.diffusion().size());
+    const float den = lp.lambda * lp.lambda;
+    for (size_t i = 0; i < p2.size(); ++i) {
+        const float num = m.diffusion()[i] * lp.tau;
+        p2[i] = num / den;
+    }
+    std::vector<uint8_t> ridx(size_t(nr + 1u) * 4u, 0);
+    std::vector<int8_t> st8(size_t(nr + 1u) * n1, 0);
+    std::vector<uint64_t> rmask(size_t(n1) * b.nrw, 0ull);
+    for (uint32_t r = 0; r <= nr; ++r) {
+        bool any = false;
+        for (int k = 0; k < 3; ++k) {
+            const int32_t sp = m.reactants()[size_t(r) * 3u + k];
+            ridx[size_t(r) * 4u + k] = uint8_t(sp);
+            if (sp > 0) {
+                any = true;
+                rmask[size_t(sp) * b.nrw + r / 64u] |= 1ull << (r % 64u);
+            }
+        }
+        if (!any && r > 0)
+            rmask[r / 64u] |= 1ull << (r % 64u);  // zero-order: a candidate in every voxel
+        for (uint32_t k = 0; k < n1; ++k)
+            st8[size_t(r) * n1 + k] = int8_t(m.stoich()[size_t(r) * n1 + k]);
+    }
+    for (uint32_t w = 0; w < b.nrw; ++w)
+        b.has_zero_order = b.has_zero_order || rmask[w] != 0ull;
+    const std::vector<float> bound = propensity_bounds(m, m.slots());
+    b.off_p2 = 0;
+    b.off_rates = uint32_t(align8(b.off_p2 + p2.size() * sizeof(float)));
+    b.off_ridx = uint32_t(align8(b.off_rates + m.rates().size() * sizeof(float)));
+    b.off_rmask = uint32_t(align8(b.off_ridx + ridx.size()));
+    b.off_stoich = uint32_t(align8(b.off_rmask + rmask.size() * sizeof(uint64_t)));
+    b.off_bound = uint32_t(align8(b.off_stoich + st8.size()));
+    b.bytes.assign(align8(b.off_bound + bound.size() * sizeof(float)), 0);
+    std::memcpy(b.bytes.data() + b.off_p2, p2.data(), p2.size() * sizeof(float));
+    std::memcpy(b.bytes.data() + b.off_rates, m.rates().data(), m.rates().size() * sizeof(float));
+    std::memcpy(b.bytes.data() + b.off_ridx, ridx.data(), ridx.size());
+    std::memcpy(b.bytes.data() + b.off_rmask, rmask.data(), rmask.size() * sizeof(uint64_t));
+    std::memcpy(b.bytes.data() + b.off_stoich, st8.data(), st8.size());
+    std::memcpy(b.bytes.data() + b.off_bound, bound.data(), bound.size() * sizeof(float));
+    return b;
+}
+
+// The well-mixed tables of a model in one blob, plus the voxel lists of the regions that receive
+// lattice products (offsets and lengths indexed by region).
+struct CmeBlob {
+    std::vector<uint8_t> bytes;
+    uint32_t off_reac = 0, off_delta = 0, off_emit = 0, off_region = 0, off_rates = 0;
+    uint32_t off_lists = 0, off_loff = 0, off_llen = 0;
+};
+
+static CmeBlob make_cme_blob(const Model &m)
+{
+    CmeBlob b;
+    const uint32_t ncr = m.num_cme_reactions(), nreg = m.num_regions();
+    std::vector<uint32_t> lists, loff(nreg, 0u), llen(nreg, 0u);
+    for (uint32_t r = 0; r < ncr; ++r) {
+        const int32_t rg = m.cme_emit_region()[r];
+        if (rg < 0 || llen[uint32_t(rg)] > 0u)
+            continue;
+        const std::vector<uint32_t> &vox = m.region_voxels(rg);
+        loff[uint32_t(rg)] = uint32_t(lists.size());
+        llen[uint32_t(rg)] = uint32_t(vox.size());
+        lists.insert(lists.end(), vox.begin(), vox.end());
+    }
+    auto put = [&](const void *src, size_t bytes) {
+        const uint32_t off = uint32_t(align8(b.bytes.size()));
+        b.bytes.resize(off + bytes);
+        if (bytes)
+            std::memcpy(b.bytes.data() + off, src, bytes);
+        return off;
+    };
+    b.off_reac = put(m.cme_reactants().data(), m.cme_reactants().size() * sizeof(int32_t));
+    b.off_delta = put(m.cme_delta().data(), m.cme_delta().size() * sizeof(int32_t));
+    b.off_emit = put(m.cme_emit().data(), m.cme_emit().size());
+    b.off_region = put(m.cme_emit_region().data(), m.cme_emit_region().size() * sizeof(int32_t));
+    b.off_rates = put(m.cme_rates().data(), m.cme_rates().size() * sizeof(float));
+    b.off_lists = put(lists.data(), lists.size() * sizeof(uint32_t));
+    b.off_loff = put(loff.data(), loff.size() * sizeof(uint32_t));
+    b.off_llen = put(llen.data(), llen.size() * sizeof(uint32_t));
+    b.bytes.resize(align8(b.bytes.size()));
+    return b;
+}
+
+// Device view of a CME blob at `base`.
+static CmeArgs cme_args_at(const uint8_t *base, const CmeBlob &b, const Model &m, unsigned long long *counters,
+                           unsigned long long *audit, bool audit_on)
+{
+    CmeArgs c;
+    c.nc = m.num_cme_species();
+    c.ncr = m.num_cme_reactions();
+    c.reactants = reinterpret_cast<const int32_t *>(base + b.off_reac);
+    c.delta = reinterpret_cast<const int32_t *>(base + b.off_delta);
+    c.emit = base + b.off_emit;
+    c.emit_region = reinterpret_cast<const int32_t *>(base + b.off_region);
+    c.rates = reinterpret_cast<const float *>(base + b.off_rates);
+    c.lists = reinterpret_cast<const uint32_t *>(base + b.off_lists);
+    c.list_off = reinterpret_cast<const uint32_t *>(base + b.off_loff);
+    c.list_len = reinterpret_cast<const uint32_t *>(base + b.off_llen);
+    c.counters = counters;
+    c.audit = audit;
+    c.audit_on = audit_on ? 1u : 0u;
+    c.tau = m.params().tau;
+    return c;
+}
+
 Device::Device(const Model &m, int hip_device) : dev_(hip_device)
 {
     const LatticeParams &lp = m.params();
@@ -1518,21 +1894,7 @@ Device::Device(const Model &m, int hip_device) : dev_(hip_device)
     nr_ = m.num_reactions();
     nreg_ = m.num_regions();
     tau_ = lp.tau;
-    if (m.has_reservoir())
-        throw Error(CS_ERR_UNSUPPORTED, "reservoir particles are not supported by the HIP path");
-    if (nreg_ > 255)
-        throw Error(CS_ERR_UNSUPPORTED, "at most 255 regions fit the one-byte label");
-    if (nr_ > 255)
-        throw Error(CS_ERR_UNSUPPORTED, "at most 255 reactions");
-    if (N_ * S_ >= (uint64_t(1) << 32))
-        throw Error(CS_ERR_UNSUPPORTED, "lattice too large for u32 slot indices");
-    for (float k : m.rates())
-        if (!std::isfinite(k))
-            throw Error(CS_ERR_UNSUPPORTED, "reaction rates must be finite");
-    for (int32_t st : m.stoich())
-        if (st < -127 || st > 127)
-            throw Error(CS_ERR_UNSUPPORTED, "stoichiometry entries must fit int8");
-    nrw_ = (nr_ + 1u + 63u) / 64u;
+    check_device_model(m);
 
     int ndev = 0;
     hip_check(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
@@ -1543,37 +1905,13 @@ Device::Device(const Model &m, int hip_device) : dev_(hip_device)
     hip_check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
     stream_ = st;
 
-    // Hop probabilities (D*tau)/(lambda*lambda) in f32, as rdme.wgsl:135 evaluates them.
     const uint32_t n1 = ns_ + 1u;
-    std::vector<float> p2(m.diffusion().size());
-    const float den = lp.lambda * lp.lambda;
-    for (size_t i = 0; i < p2.size(); ++i) {
-        const float num = m.diffusion()[i] * lp.tau;
-        p2[i] = num / den;
-    }
-    std::vector<uint8_t> ridx(size_t(nr_ + 1u) * 4u, 0);
-    std::vector<int8_t> st8(size_t(nr_ + 1u) * n1, 0);
-    std::vector<uint64_t> rmask(size_t(n1) * nrw_, 0ull);
-    for (uint32_t r = 0; r <= nr_; ++r) {
-        bool any = false;
-        for (int k = 0; k < 3; ++k) {
-            const int32_t sp = m.reactants()[size_t(r) * 3u + k];
-            ridx[size_t(r) * 4u + k] = uint8_t(sp);
-            if (sp > 0) {
-                any = true;
-                rmask[size_t(sp) * nrw_ + r / 64u] |= 1ull << (r % 64u);
-            }
-        }
-        if (!any && r > 0)
-            rmask[r / 64u] |= 1ull << (r % 64u);  // zero-order: a candidate in every voxel
-        for (uint32_t k = 0; k < n1; ++k)
-            st8[size_t(r) * n1 + k] = int8_t(m.stoich()[size_t(r) * n1 + k]);
-    }
+    const TableBlob blob = make_table_blob(m);
+    nrw_ = blob.nrw;
+    has_zero_order_ = blob.has_zero_order;
     std::vector<uint8_t> reg8(N_);
     for (uint64_t v = 0; v < N_; ++v)
         reg8[v] = uint8_t(m.regions()[v]);
-    for (uint32_t w = 0; w < nrw_; ++w)
-        has_zero_order_ = has_zero_order_ || rmask[w] != 0ull;
     if (const char *k = std::getenv("CS_STEP_KERNEL"))
         simple_ = std::string(k) == "simple";
     if (const char *x = std::getenv("CS_TILE_PLANES")) {
@@ -1586,24 +1924,14 @@ Device::Device(const Model &m, int hip_device) : dev_(hip_device)
     hip_check(hipMalloc(&lat_[1], N_ * S_), "hipMalloc lattice");
     hip_check(hipMalloc(&reg_, N_), "hipMalloc regions");
 
-    // All tables in one blob (8-byte aligned sections): the tiled kernel copies it into LDS as is.
-    auto align8 = [](size_t n) { return (n + 7u) & ~size_t(7); };
-    off_p2_ = 0;
-    off_rates_ = uint32_t(align8(off_p2_ + p2.size() * sizeof(float)));
-    off_ridx_ = uint32_t(align8(off_rates_ + m.rates().size() * sizeof(float)));
-    off_rmask_ = uint32_t(align8(off_ridx_ + ridx.size()));
-    off_stoich_ = uint32_t(align8(off_rmask_ + rmask.size() * sizeof(uint64_t)));
-    const std::vector<float> bound = propensity_bounds(m, S_);
-    off_bound_ = uint32_t(align8(off_stoich_ + st8.size()));
-    const size_t blob_bytes = align8(off_bound_ + bound.size() * sizeof(float));
+    off_p2_ = blob.off_p2;
+    off_rates_ = blob.off_rates;
+    off_ridx_ = blob.off_ridx;
+    off_rmask_ = blob.off_rmask;
+    off_stoich_ = blob.off_stoich;
+    off_bound_ = blob.off_bound;
+    const size_t blob_bytes = blob.bytes.size();
     tab_words_ = uint32_t(blob_bytes / 4u);
-    std::vector<uint8_t> blob(blob_bytes, 0);
-    std::memcpy(blob.data() + off_p2_, p2.data(), p2.size() * sizeof(float));
-    std::memcpy(blob.data() + off_rates_, m.rates().data(), m.rates().size() * sizeof(float));
-    std::memcpy(blob.data() + off_ridx_, ridx.data(), ridx.size());
-    std::memcpy(blob.data() + off_rmask_, rmask.data(), rmask.size() * sizeof(uint64_t));
-    std::memcpy(blob.data() + off_stoich_, st8.data(), st8.size());
-    std::memcpy(blob.data() + off_bound_, bound.data(), bound.size() * sizeof(float));
     hip_check(hipMalloc(&tab_, blob_bytes), "hipMalloc tables");
     uint8_t *tb = reinterpret_cast<uint8_t *>(tab_);
     p2_ = reinterpret_cast<float *>(tb + off_p2_);
@@ -1636,40 +1964,25 @@ Device::Device(const Model &m, int hip_device) : dev_(hip_device)
     if (nc_ > 0u)
         hip_check(hipMemcpyAsync(cme_counts_, m.cme_init().data(), nc_ * sizeof(int32_t), hipMemcpyHostToDevice, st),
                   "upload cme counts");
-    std::vector<uint8_t> cblob;
+    CmeBlob cb;  // lives until the uploads below have completed (the stream is synchronised at the end)
     if (has_cme_) {
-        std::vector<uint32_t> lists, loff(nreg_, 0u), llen(nreg_, 0u);
-        for (uint32_t r = 0; r < ncr_; ++r) {
-            const int32_t rg = m.cme_emit_region()[r];
-            if (rg < 0 || llen[uint32_t(rg)] > 0u)
-                continue;
-            const std::vector<uint32_t> &vox = m.region_voxels(rg);
-            loff[uint32_t(rg)] = uint32_t(lists.size());
-            llen[uint32_t(rg)] = uint32_t(vox.size());
-            lists.insert(lists.end(), vox.begin(), vox.end());
-        }
-        auto put = [&](const void *src, size_t bytes) {
-            const uint32_t off = uint32_t(align8(cblob.size()));
-            cblob.resize(off + bytes);
-            if (bytes)
-                std::memcpy(cblob.data() + off, src, bytes);
-            return off;
-        };
-        off_creac_ = put(m.cme_reactants().data(), m.cme_reactants().size() * sizeof(int32_t));
-        off_cdelta_ = put(m.cme_delta().data(), m.cme_delta().size() * sizeof(int32_t));
-        off_cemit_ = put(m.cme_emit().data(), m.cme_emit().size());
-        off_cregion_ = put(m.cme_emit_region().data(), m.cme_emit_region().size() * sizeof(int32_t));
-        off_crates_ = put(m.cme_rates().data(), m.cme_rates().size() * sizeof(float));
-        off_clists_ = put(lists.data(), lists.size() * sizeof(uint32_t));
-        off_cloff_ = put(loff.data(), loff.size() * sizeof(uint32_t));
-        off_cllen_ = put(llen.data(), llen.size() * sizeof(uint32_t));
-        hip_check(hipMalloc(&cme_tab_, align8(cblob.size())), "hipMalloc cme tables");
-        hip_check(hipMemcpyAsync(cme_tab_, cblob.data(), cblob.size(), hipMemcpyHostToDevice, st), "upload cme tables");
+        cb = make_cme_blob(m);
+        off_creac_ = cb.off_reac;
+        off_cdelta_ = cb.off_delta;
+        off_cemit_ = cb.off_emit;
+        off_cregion_ = cb.off_region;
+        off_crates_ = cb.off_rates;
+        off_clists_ = cb.off_lists;
+        off_cloff_ = cb.off_loff;
+        off_cllen_ = cb.off_llen;
+        hip_check(hipMalloc(&cme_tab_, cb.bytes.size()), "hipMalloc cme tables");
+        hip_check(hipMemcpyAsync(cme_tab_, cb.bytes.data(), cb.bytes.size(), hipMemcpyHostToDevice, st),
+                  "upload cme tables");
     }
 
     hip_check(hipMemcpyAsync(lat_[0], m.lattice().data(), N_ * S_, hipMemcpyHostToDevice, st), "upload lattice");
     hip_check(hipMemcpyAsync(reg_, reg8.data(), N_, hipMemcpyHostToDevice, st), "upload regions");
-    hip_check(hipMemcpyAsync(tab_, blob.data(), blob_bytes, hipMemcpyHostToDevice, st), "upload tables");
+    hip_check(hipMemcpyAsync(tab_, blob.bytes.data(), blob_bytes, hipMemcpyHostToDevice, st), "upload tables");
     hip_check(hipMemsetAsync(counters_, 0, kNumCounters * sizeof(unsigned long long), st), "clear counters");
     hip_check(hipStreamSynchronize(st), "upload");
     cur_ = 0;
@@ -1719,6 +2032,7 @@ void Device::step(uint32_t itime)
     a.fix_L = fix_L_;
     a.fix_O = fix_O_;
     a.max_tight = kMaxTight;
+    a.vox_base = 0;
     a.R0 = res_[0];
     a.R1 = res_[1];
     a.R2 = res_[2];
@@ -1907,6 +2221,466 @@ void Device::reset_counters()
     hipStream_t st = static_cast<hipStream_t>(stream_);
     hip_check(hipMemsetAsync(counters_, 0, kNumCounters * sizeof(unsigned long long), st), "clear counters");
     hip_check(hipStreamSynchronize(st), "clear counters");
+}
+
+
+// ---- Ensemble (host) -------------------------------------------------------------------------------------
+// Descriptor fields that do not depend on the tiling are filled once; tiles, chunks and item offsets are
+// refilled by build_descriptors() whenever the tiling changes.
+
+Ensemble::Ensemble(const std::vector<const Model *> &cells, const std::vector<uint32_t> &itime_seeds,
+                   int hip_device)
+    : dev_(hip_device)
+{
+    if (cells.empty())
+        throw Error(CS_ERR_ARG, "an ensemble needs at least one cell");
+    if (itime_seeds.size() != cells.size())
+        throw Error(CS_ERR_ARG, "one itime seed per cell");
+    S_ = cells[0]->slots();
+    for (const Model *m : cells) {
+        if (!m)
+            throw Error(CS_ERR_ARG, "null cell");
+        if (m->slots() != S_)
+            throw Error(CS_ERR_ARG, "all cells of an ensemble have the same slots per voxel");
+        check_device_model(*m);
+    }
+    int ndev = 0;
+    hip_check(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
+    if (hip_device < 0 || hip_device >= ndev)
+        throw Error(CS_ERR_ARG, "no such HIP device");
+    hip_check(hipSetDevice(dev_), "hipSetDevice");
+    int cus = 0;
+    hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev_), "CU count");
+    n_cu_ = uint32_t(cus > 0 ? cus : 256);
+    hipStream_t st;
+    hip_check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
+    stream_ = st;
+
+    const uint32_t n = uint32_t(cells.size());
+    n_cells_ = n;
+    std::vector<uint32_t> tab_words;  // all table blobs, 8-byte aligned
+    std::vector<uint8_t> cme_bytes;
+    std::vector<CmeBlob> cme_blobs;
+    std::vector<uint32_t> cme_blob_off;
+    std::vector<int32_t> cme_init;
+    std::vector<EnsCell> desc(n);
+    lat_off_.resize(n);
+    vox_off_.resize(n);
+    res_.resize(3 * size_t(n));
+    cell_nc_.resize(n);
+    cme_counts_off_.resize(n);
+    uint64_t vox = 0;
+    int32_t n_cme = 0;
+    for (uint32_t c = 0; c < n; ++c) {
+        const Model &m = *cells[c];
+        const LatticeParams &lp = m.params();
+        EnsCell &d = desc[c];
+        std::memset(&d, 0, sizeof(d));
+        lat_off_[c] = vox * S_;
+        vox_off_[c] = vox;
+        d.lat_off = vox * S_;
+        d.vox_off = uint32_t(vox);
+        d.R0 = lp.res[0];
+        d.R1 = lp.res[1];
+        d.R2 = lp.res[2];
+        for (int i = 0; i < 3; ++i)
+            res_[3 * size_t(c) + i] = lp.res[i];
+        vox += m.n_voxels();
+        if (vox >= (uint64_t(1) << 32) / S_)
+            throw Error(CS_ERR_UNSUPPORTED, "ensemble too large for u32 voxel indices");
+        const TableBlob b = make_table_blob(m);
+        if (b.bytes.size() > kTabLdsMax)
+            throw Error(CS_ERR_UNSUPPORTED, "cell tables too large to stage in LDS");
+        d.tab_off = uint32_t(tab_words.size());
+        d.tab_words = uint32_t(b.bytes.size() / 4u);
+        tab_words.resize(tab_words.size() + d.tab_words);
+        std::memcpy(tab_words.data() + d.tab_off, b.bytes.data(), b.bytes.size());
+        tab_words_max_ = std::max(tab_words_max_, d.tab_words);
+        d.off_p2 = b.off_p2;
+        d.off_rates = b.off_rates;
+        d.off_ridx = b.off_ridx;
+        d.off_stoich = b.off_stoich;
+        d.off_rmask = b.off_rmask;
+        d.off_bound = b.off_bound;
+        d.ns = m.num_species();
+        d.nr = m.num_reactions();
+        d.nreg = m.num_regions();
+        d.nrw = b.nrw;
+        d.has_zero_order = b.has_zero_order ? 1u : 0u;
+        d.mod_zero = 0xFFFFFFFFu;
+        d.tau = lp.tau;
+        d.itime_seed = itime_seeds[c];
+        cell_nc_[c] = m.num_cme_species();
+        cme_counts_off_[c] = uint32_t(cme_init.size());
+        d.cme_counts_off = cme_counts_off_[c];
+        cme_init.insert(cme_init.end(), m.cme_init().begin(), m.cme_init().end());
+        d.cme = -1;
+        d.cme_seed = m.cme_seed();
+        if (m.num_cme_reactions() > 0) {
+            d.cme = n_cme++;
+            cme_blobs.push_back(make_cme_blob(m));
+            cme_blob_off.push_back(uint32_t(cme_bytes.size()));
+            cme_bytes.insert(cme_bytes.end(), cme_blobs.back().bytes.begin(), cme_blobs.back().bytes.end());
+        }
+        if (tab_words.size() % 2u)
+            tab_words.push_back(0u);  // keep every blob 8-byte aligned
+    }
+    total_vox_ = vox;
+    cells_.resize(sizeof(EnsCell) * n);
+    std::memcpy(cells_.data(), desc.data(), cells_.size());
+
+    hip_check(hipMalloc(&lat_[0], total_vox_ * S_), "hipMalloc ensemble lattice");
+    hip_check(hipMalloc(&lat_[1], total_vox_ * S_), "hipMalloc ensemble lattice");
+    hip_check(hipMalloc(&reg_, total_vox_), "hipMalloc ensemble regions");
+    hip_check(hipMalloc(&tab_, tab_words.size() * sizeof(uint32_t)), "hipMalloc ensemble tables");
+    hip_check(hipMalloc(&cells_dev_, cells_.size()), "hipMalloc ensemble cells");
+    hip_check(hipMalloc(&itime_, size_t(n) * sizeof(uint32_t)), "hipMalloc itime");
+    hip_check(hipMalloc(&queue_, 2 * sizeof(uint32_t)), "hipMalloc queue");
+    hip_check(hipMalloc(&counters_, kNumCounters * sizeof(unsigned long long)), "hipMalloc counters");
+    hip_check(hipMalloc(&audit_, 3 * sizeof(unsigned long long)), "hipMalloc audit");
+    hip_check(hipMalloc(&cme_counts_, std::max<size_t>(cme_init.size(), 1) * sizeof(int32_t)), "hipMalloc cme counts");
+    hip_check(hipMalloc(&fix_ctl_, 4 * sizeof(uint32_t)), "hipMalloc fix_ctl");
+    hip_check(hipMalloc(&fix_list_, size_t(kMaxTight) * sizeof(uint32_t)), "hipMalloc fix_list");
+    hip_check(hipMalloc(&fix_F_, size_t(8 * kMaxTight) * sizeof(uint32_t)), "hipMalloc fix_F");
+    hip_check(hipMalloc(&fix_W_, size_t(56 * kMaxTight) * sizeof(uint32_t)), "hipMalloc fix_W");
+    hip_check(hipMalloc(&fix_L_, size_t(8 * kMaxTight) * S_), "hipMalloc fix_L");
+    hip_check(hipMalloc(&fix_O_, size_t(8 * kMaxTight) * sizeof(uint32_t)), "hipMalloc fix_O");
+    if (!cme_bytes.empty())
+        hip_check(hipMalloc(&cme_tab_, cme_bytes.size()), "hipMalloc ensemble cme tables");
+    if (n_cme > 0)
+        hip_check(hipMalloc(&cme_args_dev_, size_t(n_cme) * sizeof(CmeArgs)), "hipMalloc cme args");
+
+    std::vector<CmeArgs> cargs;
+    for (uint32_t c = 0, k = 0; c < n; ++c) {
+        if (desc[c].cme < 0)
+            continue;
+        cargs.push_back(cme_args_at(static_cast<const uint8_t *>(cme_tab_) + cme_blob_off[k], cme_blobs[k], *cells[c],
+                                    counters_, audit_, false));
+        ++k;
+    }
+    for (uint32_t c = 0; c < n; ++c) {
+        const Model &m = *cells[c];
+        hip_check(hipMemcpyAsync(lat_[0] + lat_off_[c], m.lattice().data(), m.n_voxels() * S_, hipMemcpyHostToDevice, st),
+                  "upload ensemble lattice");
+        std::vector<uint8_t> reg8(m.n_voxels());
+        for (uint64_t v = 0; v < m.n_voxels(); ++v)
+            reg8[v] = uint8_t(m.regions()[v]);
+        hip_check(hipMemcpyAsync(reg_ + vox_off_[c], reg8.data(), reg8.size(), hipMemcpyHostToDevice, st),
+                  "upload ensemble regions");
+        hip_check(hipStreamSynchronize(st), "upload ensemble cell");  // reg8 is freed next iteration
+    }
+    hip_check(hipMemcpyAsync(tab_, tab_words.data(), tab_words.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st),
+              "upload ensemble tables");
+    if (!cme_bytes.empty())
+        hip_check(hipMemcpyAsync(cme_tab_, cme_bytes.data(), cme_bytes.size(), hipMemcpyHostToDevice, st),
+                  "upload ensemble cme tables");
+    if (!cargs.empty())
+        hip_check(hipMemcpyAsync(cme_args_dev_, cargs.data(), cargs.size() * sizeof(CmeArgs), hipMemcpyHostToDevice, st),
+                  "upload cme args");
+    if (!cme_init.empty())
+        hip_check(hipMemcpyAsync(cme_counts_, cme_init.data(), cme_init.size() * sizeof(int32_t), hipMemcpyHostToDevice, st),
+                  "upload cme counts");
+    hip_check(hipMemsetAsync(queue_, 0, 2 * sizeof(uint32_t), st), "clear queue");
+    hip_check(hipMemsetAsync(counters_, 0, kNumCounters * sizeof(unsigned long long), st), "clear counters");
+    hip_check(hipMemsetAsync(audit_, 0, 3 * sizeof(unsigned long long), st), "clear audit");
+    hip_check(hipMemsetAsync(fix_ctl_, 0, 4 * sizeof(uint32_t), st), "clear fix_ctl");
+    build_descriptors();
+    hip_check(hipStreamSynchronize(st), "ensemble upload");
+}
+
+Ensemble::~Ensemble()
+{
+    hipSetDevice(dev_);
+    if (stream_)
+        hipStreamSynchronize(static_cast<hipStream_t>(stream_));
+    destroy_graph();
+    void *bufs[] = {lat_[0], lat_[1], reg_, tab_, cells_dev_, cme_args_dev_, cme_tab_, cme_counts_, itime_, queue_,
+                    counters_, audit_, fix_ctl_, fix_list_, fix_F_, fix_W_, fix_L_, fix_O_};
+    for (void *b : bufs)
+        if (b)
+            (void)hipFree(b);
+    if (stream_)
+        (void)hipStreamDestroy(static_cast<hipStream_t>(stream_));
+}
+
+void Ensemble::build_descriptors()
+{
+    const uint32_t n = n_cells();
+    EnsCell *d = reinterpret_cast<EnsCell *>(cells_.data());
+    uint64_t items = 0;
+    for (uint32_t c = 0; c < n; ++c) {
+        d[c].tiles_z = (d[c].R2 + kTZ - 1) / kTZ;
+        d[c].tiles_y = (d[c].R1 + kTY - 1) / kTY;
+        d[c].chunks = (d[c].R0 + tiling_.xc - 1) / tiling_.xc;
+        d[c].first_item = uint32_t(items);
+        items += uint64_t(d[c].tiles_z) * d[c].tiles_y * d[c].chunks;
+    }
+    if (items >= (uint64_t(1) << 31))
+        throw Error(CS_ERR_UNSUPPORTED, "too many work items");
+    total_items_ = uint32_t(items);
+    hipStream_t st = static_cast<hipStream_t>(stream_);
+    hip_check(hipMemcpyAsync(cells_dev_, cells_.data(), cells_.size(), hipMemcpyHostToDevice, st), "upload cells");
+    hip_check(hipStreamSynchronize(st), "upload cells");  // cells_ may change again before the copy ran
+}
+
+void Ensemble::set_tiling(const EnsembleTiling &t)
+{
+    if (t.xc < 1 || t.xc > 4096 || t.blocks_per_cu < 1 || t.blocks_per_cu > 16 || t.batch > 64)
+        throw Error(CS_ERR_ARG, "tiling out of range");
+    hip_check(hipSetDevice(dev_), "hipSetDevice");
+    synchronize();
+    tiling_ = t;
+    destroy_graph();
+    build_descriptors();
+}
+
+void Ensemble::destroy_graph()
+{
+    if (graph_exec_) {
+        (void)hipGraphExecDestroy(static_cast<hipGraphExec_t>(graph_exec_));
+        graph_exec_ = nullptr;
+        graph_block_ = 0;
+    }
+}
+
+void Ensemble::enqueue_step()
+{
+    hipStream_t st = static_cast<hipStream_t>(stream_);
+    EnsArgs e;
+    e.cells = static_cast<const EnsCell *>(cells_dev_);
+    e.n_cells = n_cells();
+    e.total_items = total_items_;
+    e.xc = tiling_.xc;
+    e.batch = tiling_.batch;
+    e.src = lat_[cur_];
+    e.dst = lat_[cur_ ^ 1];
+    e.reg = reg_;
+    e.tab = tab_;
+    e.itime = itime_;
+    e.queue = queue_;
+    e.counters = counters_;
+    e.fix_ctl = fix_ctl_;
+    e.fix_list = fix_list_;
+    e.fix_F = fix_F_;
+    e.fix_W = fix_W_;
+    e.fix_O = fix_O_;
+    e.fix_L = fix_L_;
+    e.max_tight = kMaxTight;
+    const uint32_t pro_blocks = (n_cells() + 63u) / 64u;
+    const CmeArgs *ca = static_cast<const CmeArgs *>(cme_args_dev_);
+    const uint32_t grid = std::min(tiling_.blocks_per_cu * n_cu_, std::max(total_items_, 1u));
+    const size_t shm = size_t(tab_words_max_) * 4u;
+    switch (S_) {
+    case 4:
+        hipLaunchKernelGGL(ens_prologue<4>, dim3(pro_blocks), dim3(64), 0, st, e, ca, cme_counts_);
+        hipLaunchKernelGGL(rdme_step_ensemble<4>, dim3(grid), dim3(kBlock), shm, st, e);
+        hipLaunchKernelGGL(ens_epilogue<4>, dim3(1), dim3(64), 0, st, e);
+        break;
+    case 8:
+        hipLaunchKernelGGL(ens_prologue<8>, dim3(pro_blocks), dim3(64), 0, st, e, ca, cme_counts_);
+        hipLaunchKernelGGL(rdme_step_ensemble<8>, dim3(grid), dim3(kBlock), shm, st, e);
+        hipLaunchKernelGGL(ens_epilogue<8>, dim3(1), dim3(64), 0, st, e);
+        break;
+    default:
+        hipLaunchKernelGGL(ens_prologue<16>, dim3(pro_blocks), dim3(64), 0, st, e, ca, cme_counts_);
+        hipLaunchKernelGGL(rdme_step_ensemble<16>, dim3(grid), dim3(kBlock), shm, st, e);
+        hipLaunchKernelGGL(ens_epilogue<16>, dim3(1), dim3(64), 0, st, e);
+        break;
+    }
+    hip_check(hipGetLastError(), "ensemble step launch");
+    cur_ ^= 1;
+}
+
+void Ensemble::run(uint32_t nsteps)
+{
+    hip_check(hipSetDevice(dev_), "hipSetDevice");
+    for (uint32_t t = 0; t < nsteps; ++t)
+        enqueue_step();
+    steps_ += nsteps;
+}
+
+void Ensemble::run_graph(uint32_t nsteps, uint32_t block)
+{
+    hip_check(hipSetDevice(dev_), "hipSetDevice");
+    if (block < 2)
+        return run(nsteps);
+    block += block & 1u;  // even, so the ping-pong parity is the same at the start of every replay
+    hipStream_t st = static_cast<hipStream_t>(stream_);
+    uint32_t done = 0;
+    if (nsteps >= block)
+        capture(block);
+    for (; done + block <= nsteps; done += block)
+        hip_check(hipGraphLaunch(static_cast<hipGraphExec_t>(graph_exec_), st), "graph launch");
+    steps_ += done;
+    run(nsteps - done);
+}
+
+void Ensemble::capture(uint32_t block)
+{
+    hipStream_t st = static_cast<hipStream_t>(stream_);
+    if (graph_exec_ && (graph_block_ != block || graph_parity_ != cur_))
+        destroy_graph();
+    if (!graph_exec_) {
+        hipGraph_t g = nullptr;
+        const int parity = cur_;
+        hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "begin capture");
+        try {
+            for (uint32_t k = 0; k < block; ++k)
+                enqueue_step();
+        } catch (...) {
+            // leave capture mode before reporting, so the stream stays usable
+            hipGraph_t partial = nullptr;
+            if (hipStreamEndCapture(st, &partial) == hipSuccess && partial)
+                (void)hipGraphDestroy(partial);
+            cur_ = parity;
+            throw;
+        }
+        hip_check(hipStreamEndCapture(st, &g), "end capture");
+        cur_ = parity;  // capture recorded the launches; nothing ran
+        hipGraphExec_t ex;
+        hip_check(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0), "graph instantiate");
+        (void)hipGraphDestroy(g);
+        graph_exec_ = ex;
+        graph_block_ = block;
+        graph_parity_ = parity;
+    }
+}
+
+float Ensemble::run_timed(uint32_t nsteps, uint32_t block)
+{
+    hip_check(hipSetDevice(dev_), "hipSetDevice");
+    hipStream_t st = static_cast<hipStream_t>(stream_);
+    if (block >= 2 && nsteps >= block + (block & 1u))
+        capture(block + (block & 1u));  // outside the timed window
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "hipEventCreate");
+    hip_check(hipEventCreate(&e1), "hipEventCreate");
+    hip_check(hipEventRecord(e0, st), "hipEventRecord");
+    if (block >= 2)
+        run_graph(nsteps, block);
+    else
+        run(nsteps);
+    hip_check(hipEventRecord(e1, st), "hipEventRecord");
+    hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+    float ms = 0.f;
+    hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return ms;
+}
+
+float Ensemble::autotune(uint32_t trial_steps)
+{
+    hip_check(hipSetDevice(dev_), "hipSetDevice");
+    hipStream_t st = static_cast<hipStream_t>(stream_);
+    synchronize();
+    // snapshot: current lattice, well-mixed counts, step counter, event counters
+    const size_t lat_bytes = total_vox_ * S_;
+    const size_t nc_total = std::max<size_t>(1, cme_counts_off_.empty() ? 1 : cme_counts_off_.back() + cell_nc_.back());
+    uint8_t *lat_snap = nullptr;
+    int32_t *cme_snap = nullptr;
+    uint32_t *q_snap = nullptr;
+    unsigned long long *cnt_snap = nullptr;
+    hip_check(hipMalloc(&lat_snap, lat_bytes), "hipMalloc snapshot");
+    hip_check(hipMalloc(&cme_snap, nc_total * sizeof(int32_t)), "hipMalloc snapshot");
+    hip_check(hipMalloc(&q_snap, 2 * sizeof(uint32_t)), "hipMalloc snapshot");
+    hip_check(hipMalloc(&cnt_snap, kNumCounters * sizeof(unsigned long long)), "hipMalloc snapshot");
+    const int cur0 = cur_;
+    const uint64_t steps0 = steps_;
+    auto save = [&] {
+        hip_check(hipMemcpyAsync(lat_snap, lat_[cur0], lat_bytes, hipMemcpyDeviceToDevice, st), "snapshot");
+        hip_check(hipMemcpyAsync(cme_snap, cme_counts_, nc_total * sizeof(int32_t), hipMemcpyDeviceToDevice, st), "snapshot");
+        hip_check(hipMemcpyAsync(q_snap, queue_, 2 * sizeof(uint32_t), hipMemcpyDeviceToDevice, st), "snapshot");
+        hip_check(hipMemcpyAsync(cnt_snap, counters_, kNumCounters * sizeof(unsigned long long), hipMemcpyDeviceToDevice, st),
+                  "snapshot");
+    };
+    auto restore = [&] {
+        synchronize();
+        cur_ = cur0;
+        steps_ = steps0;
+        hip_check(hipMemcpyAsync(lat_[cur0], lat_snap, lat_bytes, hipMemcpyDeviceToDevice, st), "restore");
+        hip_check(hipMemcpyAsync(cme_counts_, cme_snap, nc_total * sizeof(int32_t), hipMemcpyDeviceToDevice, st), "restore");
+        hip_check(hipMemcpyAsync(queue_, q_snap, 2 * sizeof(uint32_t), hipMemcpyDeviceToDevice, st), "restore");
+        hip_check(hipMemcpyAsync(counters_, cnt_snap, kNumCounters * sizeof(unsigned long long), hipMemcpyDeviceToDevice, st),
+                  "restore");
+        synchronize();
+    };
+    save();
+    const EnsembleTiling before = tiling_;
+    EnsembleTiling best = before;
+    float best_ms = -1.f;
+    const uint32_t xcs[] = {8, 16, 32};
+    const uint32_t bpcs[] = {2, 3, 4};
+    const uint32_t batches[] = {0, 1, 4};
+    for (uint32_t xc : xcs)
+        for (uint32_t bpc : bpcs)
+            for (uint32_t batch : batches) {
+                set_tiling(EnsembleTiling{xc, bpc, batch});
+                run(1);  // warm-up
+                const float ms = run_timed(trial_steps, 0) / float(trial_steps);
+                restore();
+                if (best_ms < 0.f || ms < best_ms) {
+                    best_ms = ms;
+                    best = tiling_;
+                }
+            }
+    set_tiling(best);
+    (void)hipFree(lat_snap);
+    (void)hipFree(cme_snap);
+    (void)hipFree(q_snap);
+    (void)hipFree(cnt_snap);
+    return best_ms;
+}
+
+void Ensemble::synchronize()
+{
+    hip_check(hipSetDevice(dev_), "hipSetDevice");
+    hip_check(hipStreamSynchronize(static_cast<hipStream_t>(stream_)), "hipStreamSynchronize");
+}
+
+uint64_t Ensemble::cell_voxels(uint32_t c) const
+{
+    if (c >= n_cells())
+        throw Error(CS_ERR_ARG, "cell index out of range");
+    return uint64_t(res_[3 * size_t(c)]) * res_[3 * size_t(c) + 1] * res_[3 * size_t(c) + 2];
+}
+
+void Ensemble::cell_res(uint32_t c, uint32_t res[3]) const
+{
+    if (c >= n_cells())
+        throw Error(CS_ERR_ARG, "cell index out of range");
+    for (int i = 0; i < 3; ++i)
+        res[i] = res_[3 * size_t(c) + i];
+}
+
+void Ensemble::read_lattice(uint32_t cell, uint8_t *out)
+{
+    const uint64_t nv = cell_voxels(cell);
+    synchronize();
+    hip_check(hipMemcpy(out, lat_[cur_] + lat_off_[cell], nv * S_, hipMemcpyDeviceToHost), "read cell lattice");
+}
+
+void Ensemble::read_cme_counts(uint32_t cell, int32_t *out)
+{
+    if (cell >= n_cells())
+        throw Error(CS_ERR_ARG, "cell index out of range");
+    synchronize();
+    if (cell_nc_[cell])
+        hip_check(hipMemcpy(out, cme_counts_ + cme_counts_off_[cell], cell_nc_[cell] * sizeof(int32_t),
+                            hipMemcpyDeviceToHost),
+                  "read cell cme counts");
+}
+
+void Ensemble::read_counters(uint64_t *out)
+{
+    synchronize();
+    hip_check(hipMemcpy(out, counters_, kNumCounters * sizeof(uint64_t), hipMemcpyDeviceToHost), "read counters");
+}
+
+void Ensemble::reset_counters()
+{
+    synchronize();
+    hip_check(hipMemset(counters_, 0, kNumCounters * sizeof(unsigned long long)), "clear counters");
 }
 
 }  // namespace cs

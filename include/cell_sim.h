@@ -214,6 +214,51 @@ int cs_sim_reset_counters(cs_sim *sim);
 /* Name of species `index` (0 = "void"); copies at most `cap` bytes including the terminator. */
 int cs_sim_particle_name(const cs_sim *sim, uint32_t index, char *out, size_t cap);
 
+/* ---- batched ensembles: NO reference counterpart -----------------------------------------------------
+ * The reference steps one Simulation per dispatch chain. An ensemble holds many independent cells
+ * (lattices of any size with their own tables, regions and well-mixed pools; one slot count) in one
+ * device state and steps them together: per step, a prologue launch derives each cell's itime from a
+ * device-side step counter and runs the well-mixed phases, one persistent launch takes (cell, tile,
+ * x-chunk) items from a device-side queue, and an epilogue replays tight voxels and re-arms the queue.
+ * Blocks of steps are captured as a hipGraph and replayed. Cell c's itime at step t is the low 32 bits of
+ * SplitMix64((itime_seeds[c] << 32) + t) -- what default_itime_schedule(itime_seeds[c]) gives a plain
+ * simulation -- so every cell evolves exactly as that simulation stepped alone would. The cells are
+ * copied from the handles' host models (their state as built); the handles stay independent. */
+typedef struct cs_ens cs_ens;
+typedef struct {
+    uint32_t xc;             /* x planes per work item */
+    uint32_t blocks_per_cu;  /* persistent blocks per compute unit */
+    uint32_t batch;          /* items taken per queue access; 0 = fixed round-robin without the queue */
+} cs_ens_tiling;
+typedef struct {
+    uint32_t n_cells;
+    uint32_t max_particles_site;
+    uint64_t total_voxels;
+    uint64_t steps_done;
+    uint32_t items;          /* work items per step under the current tiling */
+} cs_ens_info;
+int cs_ens_create(cs_sim *const *cells, uint32_t n_cells, const uint32_t *itime_seeds, int hip_device,
+                  cs_ens **out);
+void cs_ens_destroy(cs_ens *ens);
+/* nsteps steps with launches per step / through a graph of `block` steps (rounded up to even). */
+int cs_ens_run(cs_ens *ens, uint32_t nsteps);
+int cs_ens_run_graph(cs_ens *ens, uint32_t nsteps, uint32_t block);
+/* As cs_ens_run_graph (block >= 2) or cs_ens_run (block < 2), timed with events on the ensemble's
+ * stream (graph capture outside the timed window). Synchronises. */
+int cs_ens_run_timed(cs_ens *ens, uint32_t nsteps, uint32_t block, float *ms);
+int cs_ens_synchronize(cs_ens *ens);
+int cs_ens_set_tiling(cs_ens *ens, const cs_ens_tiling *t);
+int cs_ens_get_tiling(const cs_ens *ens, cs_ens_tiling *t);
+/* Tries tilings (planes per item 8/16/32, 2-4 blocks per CU, queue batch 0/1/4) for `trial_steps` steps
+ * each from a snapshot of the state (restored afterwards) and keeps the fastest. */
+int cs_ens_autotune(cs_ens *ens, uint32_t trial_steps, float *ms_per_step);
+int cs_ens_get_info(const cs_ens *ens, cs_ens_info *out);
+int cs_ens_cell_res(const cs_ens *ens, uint32_t cell, uint32_t res[3]);
+int cs_ens_read_lattice(cs_ens *ens, uint32_t cell, uint8_t *out, size_t nbytes);
+int cs_ens_read_cme_counts(cs_ens *ens, uint32_t cell, int32_t *out, size_t n);
+int cs_ens_read_counters(cs_ens *ens, uint64_t *out, size_t n);
+int cs_ens_reset_counters(cs_ens *ens);
+
 #ifdef __cplusplus
 }
 #endif
