@@ -45,6 +45,59 @@ def calibrate_copy(mib: int, steps: int) -> int:
     return 0
 
 
+def ensemble_leg(args) -> dict:
+    """Replicate cells (gene-expression network, edges log-uniform 32..96, rates x LogNormal(0.2), S=8,
+    seed 99): one batched Ensemble (autotuned, hipGraph blocks) against the same cells stepped one
+    plain launch sequence per cell. Values are replicate-voxel-steps per second."""
+    from cell_simulation_amd import Ensemble, configs, default_itime_schedule
+
+    log(f"ensemble: building {args.ensemble_cells} cells")
+    cells, seeds = configs.ensemble_cells(args.ensemble_cells, seed=99)
+    ens = Ensemble(cells, seeds)
+    vox = ens.info()["total_voxels"]
+    ens.run(2)  # warm-up (first launches, page-in)
+    ens.synchronize()
+    tune_ms = ens.autotune(3)
+    log(f"ensemble: {vox} voxels per step, autotuned tiling {ens.tiling} ({tune_ms:.3f} ms/step in trials)")
+    ens_ms = ens.run_timed(args.ensemble_steps, args.graph_block)
+    batched_value = vox * args.ensemble_steps / (ens_ms / 1e3)
+    c = ens.counters()
+    tiling = ens.tiling
+    ens.close()
+
+    log(f"plain: preparing {len(cells)} simulations")
+    for sim in cells:
+        sim.prepare_for_gpu(0)
+    its = [default_itime_schedule(s, args.plain_steps + 1) for s in seeds]
+    for sim, it in zip(cells, its):  # warm-up step
+        sim.step(0, int(it[0]), 0)
+    for sim in cells:
+        sim.synchronize()
+    t0 = time.perf_counter()
+    for k in range(1, args.plain_steps + 1):
+        for sim, it in zip(cells, its):
+            sim.step(k, int(it[k]), 0)
+    for sim in cells:
+        sim.synchronize()
+    plain_s = time.perf_counter() - t0
+    plain_value = vox * args.plain_steps / plain_s
+    for sim in cells:
+        sim.close()
+    b_alg = 2 * 8 + 1
+    return {
+        "workload": f"{len(cells)} replicate cells, gene-expression network (well-mixed gene switching, lattice "
+                    "mRNA/protein), edges log-uniform 32..96, rates x LogNormal(sigma 0.2), S=8, seed 99",
+        "voxels_per_step": vox, "unit": "replicate-voxel-steps/s",
+        "batched_value": batched_value, "batched_ms_per_step": ens_ms / args.ensemble_steps,
+        "batched_steps": args.ensemble_steps, "graph_block": args.graph_block, "tiling": tiling,
+        "plain_value": plain_value, "plain_ms_per_step": plain_s * 1e3 / args.plain_steps,
+        "plain_steps": args.plain_steps, "plain_path": "one cs_sim per cell, its launches per step (own stream)",
+        "speedup": batched_value / plain_value,
+        "batched_hbm_frac": batched_value * b_alg / (HBM_PEAK_GBS * 1e9), "bytes_per_voxel_step": b_alg,
+        "cme_events": int(c[5]), "exceptions": int(c[1]),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -54,6 +107,11 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps for the CPU baseline (0 = skip)")
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "large_cell_256_traffic.json"),
                     help="per-launch HBM bytes measured with rocprofv3 --pmc (FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--ensemble-cells", type=int, default=512,
+                    help="replicate cells of the batched-vs-plain comparison (0 = skip it)")
+    ap.add_argument("--ensemble-steps", type=int, default=200, help="timed steps of the batched ensemble")
+    ap.add_argument("--plain-steps", type=int, default=10, help="timed steps of the per-cell plain launches")
+    ap.add_argument("--graph-block", type=int, default=100, help="steps per captured hipGraph")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="instead of the step: copy a MIB-MiB device buffer --steps times (a kernel of known "
                          "bytes for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
@@ -120,8 +178,12 @@ def main() -> int:
                "sample": f"{n_cpu} step(s) of the {args.edge}^3 S={S} large-cell workload after one warm-up step, "
                          "serial C oracle (canonical order of the reference shaders)"}
 
+    batched = None
+    if args.ensemble_cells > 0:
+        batched = ensemble_leg(args)
+
     out = {
-        "metric": "voxel-steps/s (RDME diff+react, 256^3)",
+        "metric": "voxel-steps/s (RDME diff+react, 256^3) batched vs plain; % of HBM peak",
         "value": value,
         "unit": "voxel-steps/s",
         "n_gpus": 1,
@@ -143,6 +205,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "diagnostics": {"tight_voxel_steps": int(counters[0]), "exceptions": int(counters[1]),
                         "near_threshold": int(counters[2]), "fired": int(counters[3])},
+        "batched": batched,
     }
     print(json.dumps(out), flush=True)
     return 0
