@@ -98,6 +98,35 @@ def ensemble_leg(args) -> dict:
     }
 
 
+def precision_leg(steps: int) -> dict:
+    """512^3 precision-stress workload (12 lattice + 40 well-mixed species, 120 reactions with rates over
+    1e-6..1e4 /s, ~10M particles, S=8): `steps` steps with the device's f64 audit of every well-mixed event
+    (compensated f32 total vs plain f64 total; choice under f64 vs the f32 choice)."""
+    from cell_simulation_amd import configs, default_itime_schedule
+
+    log("precision: building the 512^3 workload")
+    sim = configs.precision_stress()
+    sim.prepare_for_gpu(0)
+    sim.set_cme_audit(True)
+    its = default_itime_schedule(2024, steps)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.run(0, its)
+    sim.synchronize()
+    wall = time.perf_counter() - t0
+    a = sim.cme_audit()
+    c = sim.counters()
+    inf = sim.info()
+    return {"workload": "precision_stress 512^3 S=8, 12 lattice + 40 well-mixed species, 120 reactions, "
+                        "rates log-uniform 1e-6..1e4 /s, ~10M particles, dt=5us, seed 2024",
+            "steps": steps, "ms_per_step": wall * 1e3 / steps,
+            "voxel_steps_per_s": inf["n_voxels"] * steps / wall,
+            "cme_events": a["events"], "max_rel_propensity_error_f32_kahan_vs_f64": a["max_rel_error"],
+            "selection_divergences": a["divergences"],
+            "selection_divergence_rate": a["divergences"] / a["events"] if a["events"] else None,
+            "lattice_reactions_fired": int(c[3]), "near_threshold": int(c[2]), "exceptions": int(c[1])}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,6 +141,9 @@ def main() -> int:
     ap.add_argument("--ensemble-steps", type=int, default=200, help="timed steps of the batched ensemble")
     ap.add_argument("--plain-steps", type=int, default=10, help="timed steps of the per-cell plain launches")
     ap.add_argument("--graph-block", type=int, default=100, help="steps per captured hipGraph")
+    ap.add_argument("--precision", type=int, default=0, metavar="STEPS",
+                    help="also run the 512^3 precision-stress workload for STEPS steps with the f64 audit of the "
+                         "well-mixed propensities and report it (off by default: building 512^3 takes a while)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="instead of the step: copy a MIB-MiB device buffer --steps times (a kernel of known "
                          "bytes for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
@@ -181,6 +213,7 @@ def main() -> int:
     batched = None
     if args.ensemble_cells > 0:
         batched = ensemble_leg(args)
+    precision = precision_leg(args.precision) if args.precision > 0 else None
 
     out = {
         "metric": "voxel-steps/s (RDME diff+react, 256^3) batched vs plain; % of HBM peak",
@@ -206,6 +239,7 @@ def main() -> int:
         "diagnostics": {"tight_voxel_steps": int(counters[0]), "exceptions": int(counters[1]),
                         "near_threshold": int(counters[2]), "fired": int(counters[3])},
         "batched": batched,
+        "precision": precision,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -12,6 +12,10 @@
 * `ensemble_cells` -- replicate cells of the gene-expression network with edges drawn log-uniform in
   32..96 voxels, every rate constant (and the diffusion constants) perturbed per replicate by a
   log-normal factor (sigma 0.2), S=8, seed 99; per-cell itime seeds for an Ensemble.
+* `precision_stress` -- 512^3 lattice, S=8, 32 nm voxels, dt = 5 us; the large-cell regions; 12 lattice
+  species (about 10M particles in the cytoplasm and nucleoid) with region-dependent D, 40 well-mixed species; 120 reactions in total
+  (40 per-voxel lattice reactions, 72 well-mixed reactions, 8 well-mixed reactions that emit lattice
+  particles) with rates log-uniform over 1e-6..1e4 /s; seed 2024. `edge` scales it down for tests.
 * `gene_expression` -- 128^3 lattice, 16 nm voxels, S=8, dt = 20 us; extracellular space, a membrane shell
   one voxel thick, cytoplasm and a nucleoid sphere of radius 20 voxels; well-mixed gene_off / gene_on
   switching (0.01 /s on, 0.05 /s off) with transcription of mRNA into the nucleoid at 0.5 /s while on;
@@ -165,3 +169,50 @@ def ensemble_cells(n_cells: int = 512, seed: int = 99, edge_range=(32, 96), sigm
                                      rate_factors=factors))
         seeds.append(int(rng.integers(0, 2**32)))
     return cells, seeds
+
+
+def precision_stress(seed: int = 2024, edge: int = 512, slots: int = 8, n_particles: int = 10_000_000,
+                     n_rdme: int = 12, n_cme: int = 40, n_reactions: int = 120, tau: float = 5e-6,
+                     lam: float = 32e-9) -> Simulation:
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sim = Simulation(LatticeParams([edge * lam] * 3, [edge] * 3, tau, lam, slots))
+    sim.set_seed(seed)
+    c = [edge / 2 * lam] * 3
+    r_ext = 0.47 * edge * lam
+    sim.add_region(RegionType.SphericalShell("membrane", "cytoplasm", c, r_ext - 2 * lam, r_ext), 1e-12)
+    sim.add_region(RegionType.Sphere("nucleoid", c, 0.2 * edge * lam), 1e-12)
+    sim.prepare_regions()
+
+    def loguniform(lo, hi):
+        return float(np.exp(rng.uniform(np.log(lo), np.log(hi))))
+
+    names = [f"R{i:02d}" for i in range(n_rdme)]
+    per = n_particles // n_rdme
+    for i, nm in enumerate(names):  # cytoplasm and nucleoid: ~0.2 particles per voxel
+        sim.add_particle_count(nm, "nucleoid" if i % 3 == 2 else "cytoplasm", per, i < 4, False)
+    for nm in names:
+        for reg in LARGE_CELL_REGIONS:
+            sim.set_diffusion_rate_particle(nm, reg, loguniform(1e-14, 1e-11))
+    cme = [f"W{i:02d}" for i in range(n_cme)]
+    for nm in cme:
+        sim.add_cme_species(nm, int(rng.integers(10, 2000)))
+    n_lattice = n_reactions // 3
+    n_emit = n_reactions // 15
+    n_wm = n_reactions - n_lattice - n_emit
+    for _ in range(n_lattice):
+        a = names[int(rng.integers(n_rdme))]
+        if rng.uniform() < 0.5:
+            sim.add_reaction([a], [names[int(rng.integers(n_rdme))]], loguniform(1e-6, 1e1))
+        else:  # two distinct species (k*n^2 of a homodimer row fires on a single particle)
+            b = names[(names.index(a) + 1 + int(rng.integers(n_rdme - 1))) % n_rdme]
+            sim.add_reaction([a, b], [names[int(rng.integers(n_rdme))]], loguniform(1e-6, 1e1))
+    for _ in range(n_wm):
+        order = 1 if rng.uniform() < 0.7 else 2
+        reac = [cme[int(i)] for i in rng.integers(0, n_cme, size=order)]
+        prods = [cme[int(i)] for i in rng.integers(0, n_cme, size=int(rng.integers(0, 3)))]
+        k = loguniform(1e-6, 1e4)
+        sim.add_cme_reaction(reac, prods, k / 1000.0 if order == 2 else k / 100.0)
+    for _ in range(n_emit):
+        src = cme[int(rng.integers(n_cme))]
+        sim.add_cme_reaction([src], [src, f"{names[int(rng.integers(n_rdme))]}@cytoplasm"], loguniform(1e-6, 1e2))
+    return sim

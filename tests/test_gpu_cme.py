@@ -86,3 +86,51 @@ def test_precision_network_parity_and_audit(oracle_mod):
     assert a["events"] == c[CS_CNT_CME_EVENTS]
     assert a["max_rel_error"] < 2.5e-7, a
     assert a["divergences"] <= max(1, a["events"] // 1000), a
+
+
+def test_precision_stress_small_parity(oracle_mod):
+    announce("precision_stress 48^3 S=8 (12 lattice + 40 well-mixed species, 120 reactions), 6000 particles, 40 steps")
+    sim = configs.precision_stress(edge=48, n_particles=6000)
+    orc = oracle_mod.Oracle.from_simulation(sim)
+    sim.prepare_for_gpu(0)
+    sim.set_cme_audit(True)
+    its = default_itime_schedule(2024, 40)
+    sim.run(0, its)
+    orc.run(its)
+    # a slot-S alias (rdme.wgsl:30-40) is not representable in one-byte slots: compare without any
+    assert orc.counters.alias_writes == 0
+    _compare(sim, orc, "precision_stress 48^3 after 40 steps")
+    c = sim.counters()
+    assert c[CS_CNT_CME_EVENTS] == orc.cme_counters.events > 0
+    a = sim.cme_audit()
+    assert a["events"] == c[CS_CNT_CME_EVENTS] and a["max_rel_error"] < 2.5e-7, a
+
+
+def test_ensemble_gene_switching_matches_closed_form():
+    """400 replicate cells of a two-state gene: the fraction switched on after T is
+    k_on / (k_on + k_off) * (1 - exp(-(k_on + k_off) T)) within 3 binomial sigma (an f64 closed form)."""
+    import math
+
+    from cell_simulation_amd import Ensemble
+
+    announce("ensemble of 400 two-state gene cells, 40 steps of 1 ms, against the closed form")
+    lam = 32e-9
+    cells, seeds = [], []
+    k_on, k_off, n_steps, tau = 50.0, 30.0, 40, 1e-3
+    for i in range(400):
+        sim = Simulation(LatticeParams([4 * lam] * 3, [4] * 3, tau, lam, 4))
+        sim.set_seed(1000 + i)
+        sim.add_region(RegionType.Sphere("cell", [2 * lam] * 3, 1.5 * lam), 0.0)
+        sim.prepare_regions()
+        sim.add_cme_species("off", 1)
+        sim.add_cme_species("on", 0)
+        sim.add_cme_reaction(["off"], ["on"], k_on)
+        sim.add_cme_reaction(["on"], ["off"], k_off)
+        cells.append(sim)
+        seeds.append(77 + 13 * i)
+    ens = Ensemble(cells, seeds)
+    ens.run_graph(n_steps, block=20)
+    on = sum(int(ens.cme_counts(i)[1]) for i in range(len(cells)))
+    p = k_on / (k_on + k_off) * (1.0 - math.exp(-(k_on + k_off) * n_steps * tau))
+    sigma = math.sqrt(p * (1 - p) / len(cells))
+    assert abs(on / len(cells) - p) < 3 * sigma, (on / len(cells), p, sigma)
