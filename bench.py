@@ -186,11 +186,14 @@ def main() -> int:
 
     b_alg = 2 * S + 1
     achieved_gbs = b_alg * N / avg_kernel_s / 1e9
+    kernel = {"simple": "rdme_step_kernel", "tiled": "rdme_step_tiled"}.get(os.environ.get("CS_STEP_KERNEL", ""),
+                                                                          "rdme_step_wave")
     traffic = None
     if args.traffic_json and os.path.exists(args.traffic_json):
         with open(args.traffic_json) as f:
             tj = json.load(f)
-        if tj.get("edge") == args.edge:
+        # measured bytes apply only to the kernel and lattice they were measured on
+        if tj.get("edge") == args.edge and str(tj.get("kernel", "")).startswith(kernel):
             traffic = tj.get("bytes_per_launch")
 
     value = N * args.steps / wall
@@ -233,7 +236,7 @@ def main() -> int:
                    "voxels": N, "slots_per_voxel": S, "path": "plain"},
         "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "rdme_step_tiled", "bytes_per_voxel_step": b_alg,
+                     "kernel": kernel, "bytes_per_voxel_step": b_alg,
                      "avg_kernel_us": avg_kernel_s * 1e6},
         "cpu_baseline": cpu,
         "diagnostics": {"tight_voxel_steps": int(counters[0]), "exceptions": int(counters[1]),

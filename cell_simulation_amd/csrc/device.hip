@@ -37,6 +37,8 @@ namespace {
 constexpr int kMaxRW = 4;  // reaction bitmask words: up to 255 reactions
 constexpr int kBlock = 256;
 constexpr uint32_t kMaxTight = 1024;  // tight voxels replayed exactly per step; more are counted as exceptions
+constexpr uint32_t kFixCtl = 16;      // control words: [0] tight voxels noted, [1] blocks done, [8..15] queues
+constexpr uint32_t kFixQueue = 8;
 
 void hip_check(hipError_t e, const char *what)
 {
@@ -131,15 +133,17 @@ __device__ __forceinline__ uint32_t count_nonzero(const Slots<S> &s)
     return n;
 }
 
-// number of slots holding species sp (sp >= 1)
+// number of slots holding species sp (sp >= 1); only the first `words` slot words are examined (a
+// wave-uniform bound: the slots are compacted, so words past every lane's occupancy hold no particle)
 template <int S>
-__device__ __forceinline__ uint32_t count_species(const Slots<S> &s, uint32_t sp)
+__device__ __forceinline__ uint32_t count_species(const Slots<S> &s, uint32_t sp, uint32_t words = S / 4)
 {
     const uint32_t pat = sp * 0x01010101u;
     uint32_t n = 0;
 #pragma unroll
     for (int q = 0; q < S / 4; ++q)
-        n += __builtin_popcount(nonzero_bytes(s.w[q] ^ pat) ^ 0x80808080u);
+        if (uint32_t(q) < words)
+            n += __builtin_popcount(nonzero_bytes(s.w[q] ^ pat) ^ 0x80808080u);
     return n;
 }
 
@@ -285,24 +289,46 @@ __device__ __forceinline__ void count_event(unsigned long long *counters, int id
 
 // ---- reaction (cme.wgsl:25-120) on the compacted post-diffusion slots ------------------------------
 template <int S>
-__device__ __forceinline__ uint32_t species_count(const Slots<S> &s, uint32_t sp)
+__device__ __forceinline__ uint32_t species_count(const Slots<S> &s, uint32_t sp, uint32_t words)
 {
-    return sp == 0u ? 1u : count_species(s, sp);  // C[v][0] = 1 (cme.wgsl:31)
+    return sp == 0u ? 1u : count_species(s, sp, words);  // C[v][0] = 1 (cme.wgsl:31)
 }
 
 template <int S>
-__device__ __forceinline__ float propensity(const Tab &tb, const Slots<S> &s, uint32_t r)
+__device__ __forceinline__ float propensity(const Tab &tb, const Slots<S> &s, uint32_t r, uint32_t words)
 {
     const uint32_t ix = tb.ridx[r];
-    const uint32_t prod = species_count(s, ix & 0xFFu) * species_count(s, (ix >> 8) & 0xFFu) *
-                          species_count(s, (ix >> 16) & 0xFFu);
+    const uint32_t prod = species_count(s, ix & 0xFFu, words) * species_count(s, (ix >> 8) & 0xFFu, words) *
+                          species_count(s, (ix >> 16) & 0xFFu, words);
     return tb.rates[r] * float(int32_t(prod));
+}
+
+// Slot words that hold the particles of every lane of the wave with `cnt` particles (wave-uniform).
+template <int S>
+__device__ __forceinline__ uint32_t wave_words(uint32_t cnt)
+{
+    if constexpr (S == 4)
+        return 1u;
+    else if constexpr (S == 8)
+        return __ballot(cnt > 4u) ? 2u : 1u;
+    else
+        return __ballot(cnt > 8u) ? (__ballot(cnt > 12u) ? 4u : 3u) : (__ballot(cnt > 4u) ? 2u : 1u);
 }
 
 // Applies the reaction step to `out` (cnt particles, compacted). Reactions with a reactant absent from
 // the voxel have propensity +0 and leave every partial sum unchanged, so only candidate reactions
 // (zero-order ones and those with a reactant present) are visited; the f32 sums match the reference's
 // sequential sums bit for bit.
+// The reject test that opens react(): false when the voxel's draw lies above the upper bound of its total
+// propensity, i.e. nothing can fire there this step.
+template <int S>
+__device__ __forceinline__ bool react_may_fire(const StepArgs &a, const Tab &tb, uint32_t cnt, uint32_t x,
+                                               uint32_t y, uint32_t z)
+{
+    const float r1 = uniform_float(pcg_hash(a.itime + x * y + z));
+    return !(r1 > tb.bound[cnt] * a.tau * 1.001f + 1e-6f);
+}
+
 template <int S>
 __device__ __forceinline__ void react(const StepArgs &a, const Tab &tb, Slots<S> &out, uint32_t cnt, uint32_t x,
                                       uint32_t y, uint32_t z, bool &fired, bool &rolled, bool &near, bool &bad)
@@ -314,6 +340,7 @@ __device__ __forceinline__ void react(const StepArgs &a, const Tab &tb, Slots<S>
     // above the bound's reject line is above the exact one below as well: nothing fires, nothing is near.
     if (r1 > tb.bound[cnt] * a.tau * 1.001f + 1e-6f)
         return;
+    const uint32_t words = wave_words<S>(cnt);  // slot words holding particles in any lane still here
 
     uint64_t cand[kMaxRW];
 #pragma unroll
@@ -334,7 +361,7 @@ __device__ __forceinline__ void react(const StepArgs &a, const Tab &tb, Slots<S>
         while (m) {
             const uint32_t r = uint32_t(w) * 64u + uint32_t(__ffsll((long long)m) - 1);
             m &= m - 1ull;
-            tot = tot + propensity(tb, out, r);
+            tot = tot + propensity(tb, out, r, words);
         }
     }
 
@@ -360,7 +387,7 @@ __device__ __forceinline__ void react(const StepArgs &a, const Tab &tb, Slots<S>
         while (m && !chosen) {
             const uint32_t r = uint32_t(w) * 64u + uint32_t(__ffsll((long long)m) - 1);
             m &= m - 1ull;
-            cum = cum + propensity(tb, out, r);
+            cum = cum + propensity(tb, out, r, words);
             i = r;
             if (r >= a.nr || !(r2 > cum / tot))
                 chosen = true;
@@ -375,7 +402,7 @@ __device__ __forceinline__ void react(const StepArgs &a, const Tab &tb, Slots<S>
     Slots<S> res = zero_slots<S>();
     uint32_t j = 0;
     for (uint32_t sp = 1; sp <= a.ns; ++sp) {
-        const int32_t c = int32_t(count_species(out, sp)) + int32_t(row[sp]);
+        const int32_t c = int32_t(count_species(out, sp, words)) + int32_t(row[sp]);
         if (c < 0)
             bad = true;
         if (c <= 0)
@@ -755,6 +782,8 @@ __device__ __forceinline__ void note_tight(const StepArgs &a, uint32_t v)
 template <int S>
 __global__ __launch_bounds__(64) void tight_replay_kernel(StepArgs a)
 {
+    if (threadIdx.x < 8u)
+        a.fix_ctl[kFixQueue + threadIdx.x] = 0u;  // the wave kernel's work queues, for the next step
     if (threadIdx.x != 0 || a.fix_ctl[0] == 0u)
         return;
     fixup_tight<S>(a, a.fix_ctl[0]);
@@ -1295,6 +1324,601 @@ __global__ __launch_bounds__(kBlock) void rdme_step_tiled(StepArgs a, TileArgs t
     const uint32_t chunk = blk / (t.tiles_z * t.tiles_y);
     TileCounts tc;
     tile_march<S>(L, tab_lds, a, t, tz, ty, chunk, tc);
+    flush_counts(a.counters, tc);
+}
+
+// ---- wave-march kernel --------------------------------------------------------------------------------
+// Each wave owns a tile of kWY rows (y) x kWZ columns (z) and marches along x through `xc` planes on its
+// own: no block barrier sits in the march, so the waves of a CU hide each other's latency. The slots of
+// the tile plus a one-voxel halo ("records") and the region labels with a two-voxel halo are loaded one
+// plane ahead into registers. Per plane X:
+//   T0  the occupied records of plane X are listed, with their slots, in the wave's LDS (ballot ranks);
+//   T1  one lane per listed record computes its particles' hops; a hop that ends on a voxel of the tile
+//       (planes X-1, X, X+1) appends an arrival event (destination, direction, source slot, species) to
+//       the destination plane's list, and an interior particle that leaves sets its bit in the voxel's
+//       leave mask;
+//   T2  plane X-1 is final (every hop into it came from planes X-2 .. X): voxels without particles or
+//       arrivals are written as zeros by their own lane; the others are listed and gathered one lane
+//       each: own stays from the leave mask, arrivals from the events sorted into the canonical order,
+//       then the reaction.
+// Particles are sparse, so the lists keep the lanes busy where one lane per voxel would leave most idle.
+// A plane whose event list overflows is gathered by the one-lane-per-voxel global form instead.
+constexpr int kWY = 4;                            // rows of a wave tile
+constexpr int kWZ = 64;                           // columns of a wave tile: one lane each
+constexpr int kWRZ = kWZ + 2;                     // record row with its z halo
+constexpr int kWLY = kWY + 4;                     // label rows y0-2 .. y0+kWY+1
+constexpr int kWLZ = kWZ + 4;                     // label columns z0-2 .. z0+kWZ+1 (68 bytes, 17 words)
+constexpr int kWLabWords = kWLY * kWLZ / 4;       // 136
+constexpr int kWBatches = kWY + 3;                // records per lane: interior rows, rows -1 and kWY, z halo
+constexpr int kWHop = kWY * kWZ + 2 * kWZ + 2 * kWY;  // records a plane can list (392)
+constexpr int kWEv = 96;                          // arrival events per destination plane
+constexpr int kWReact = 128;                      // deferred reactions a wave holds (processed 64 at a time)
+constexpr uint32_t kWEvEnd = 0xFFu;               // end of an event chain
+constexpr int kWaveBlock = 256;                   // four waves: four tiles stacked in y share the tables
+constexpr int kWavesPerBlock = kWaveBlock / 64;
+static_assert(kWZ == 64 && kWLZ % 4 == 0 && kWLabWords <= 3 * 64, "wave tile geometry");
+
+struct WaveArgs {
+    uint32_t tiles_z, wtiles_y, btiles_y, chunks, xc;  // wave tiles in z and y, block rows of tiles, x-chunks
+    uint32_t has_zero_order;
+    uint32_t ev_cap;        // events a plane may hold (<= kWEv); a plane with more takes the global form
+    uint32_t tab_words;
+    uint32_t off_p2, off_rates, off_ridx, off_stoich, off_rmask, off_bound;
+    const uint32_t *tab;
+    const uint32_t *regp;   // padded labels [R0+4][PY][PZ] (0xFF outside the lattice), read as words
+    uint32_t PY, PZ4;       // PY = kWY*wtiles_y + 4 rows; PZ4 = (kWZ*tiles_z + 4) / 4 words per row
+    uint32_t ngroups;       // groups of kWavesPerBlock tiles stacked in y: tiles_z * btiles_y * chunks
+    uint32_t *queue;        // [8] work counters, one per XCD; zero at the start of every step
+    const uint32_t *order;  // [ngroups] groups by decreasing particle count (the longest tiles start first)
+};
+
+template <int S>
+struct WaveLds {
+    typename SlotVec<S>::T hop_slots[kWHop];  // listed records of the plane being hopped
+    uint16_t hop_rec[kWHop];                   // their record ids: (row + 1) * kWRZ + (column + 1)
+    uint32_t ev[3][kWEv];                      // arrival events per destination plane (X % 3): direction |
+                                               // species << 3 | next event to the same voxel << 11
+    uint32_t head[3][kWY * kWZ];               // latest event into each interior voxel (kWEvEnd: none)
+    uint32_t ev_n[4];                          // events appended (may pass the cap: the plane overflowed)
+    uint64_t arrive[3][kWY];                   // bit z of row r: an event targets voxel (r, z)
+    uint32_t leave[2][kWY * kWZ / 2];          // 16-bit leave mask per interior voxel (X % 2)
+    uint32_t lab[3][kWLabWords];               // label planes (X % 3)
+    uint16_t slow[kWY * kWZ];                  // voxels of plane X-1 to gather
+    typename SlotVec<S>::T rb_slots[kWReact];  // gathered voxels whose reaction draw passed the bound test,
+    uint32_t rb_v[kWReact];                    // their voxel index,
+    uint32_t rb_xc[kWReact];                   // x | particle count << 24
+    uint32_t rb_yz[kWReact];                   // and y << 16 | z (a circular list)
+};
+
+__device__ __forceinline__ void wave_sync()
+{
+    // LDS accesses of one wave complete in order; this keeps the compiler from moving them across.
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ int ring3(int x) { return (x + 3) % 3; }
+
+template <int S>
+__device__ __forceinline__ bool vec_nonzero(const typename SlotVec<S>::T &v)
+{
+    if constexpr (S == 16)
+        return (v.x | v.y | v.z | v.w) != 0u;
+    else if constexpr (S == 8)
+        return (v.x | v.y) != 0u;
+    else
+        return v != 0u;
+}
+
+template <int S>
+__device__ __forceinline__ typename SlotVec<S>::T load_vec(const uint8_t *base, uint32_t v)
+{
+    return *reinterpret_cast<const typename SlotVec<S>::T *>(base + size_t(v) * uint32_t(S));
+}
+
+// Bit 4s set for every bit s of a 16-bit mask.
+__device__ __forceinline__ uint64_t spread_nibbles(uint32_t m)
+{
+    uint64_t x = m & 0xFFFFu;
+    x = (x | (x << 24)) & 0x000000FF000000FFull;
+    x = (x | (x << 12)) & 0x000F000F000F000Full;
+    x = (x | (x << 6)) & 0x0303030303030303ull;
+    x = (x | (x << 3)) & 0x1111111111111111ull;
+    return x;
+}
+
+// `val` inserted at byte position d < 16; the bytes from d up move one place up (the top byte is lost).
+__device__ __forceinline__ BV bv_insert(const BV &v, uint32_t d, uint32_t val)
+{
+    const BV m = bytes_below(d), sh = shl_bytes(v, 1u);
+    const BV r{(v.lo & m.lo) | (sh.lo & ~m.lo), (v.hi & m.hi) | (sh.hi & ~m.hi)};
+    return bv_set(r, d, val);
+}
+
+// The arrivals of voxel V = (x, y, z) recomputed from global memory (the neighbours' slots and labels), in
+// the canonical order: from -x, -y, -z into `lo`, from +z, +y, +x into `hi`. Loops are kept rolled: this
+// serves planes whose event list overflowed, and must not add registers to the main path.
+template <int S>
+__device__ __forceinline__ void gather_arrivals_rolled(const StepArgs &a, const Tab &tb, int x, uint32_t y,
+                                                       uint32_t z, uint32_t V, Slots<S> &lo, uint32_t &nlo,
+                                                       Slots<S> &hi, uint32_t &nhi)
+{
+#pragma unroll 1
+    for (uint32_t k = 0; k < 6u; ++k) {
+        // k: source -x, -y, -z (hopped +x, +y, +z), then +z, +y, +x (hopped -z, -y, -x)
+        const uint32_t want = (0x024531u >> (4u * k)) & 0xFu;
+        const bool exists = k == 0u ? x > 0 : k == 1u ? y > 0u : k == 2u ? z > 0u
+                          : k == 3u ? z + 1u < a.R2 : k == 4u ? y + 1u < a.R1 : uint32_t(x) + 1u < a.R0;
+        if (!exists)
+            continue;
+        const uint32_t u = k == 0u ? V - a.R12 : k == 1u ? V - a.R2 : k == 2u ? V - 1u
+                         : k == 3u ? V + 1u : k == 4u ? V + a.R2 : V + a.R12;
+        const uint32_t ux = k == 0u ? uint32_t(x) - 1u : k == 5u ? uint32_t(x) + 1u : uint32_t(x);
+        const uint32_t uy = k == 1u ? y - 1u : k == 4u ? y + 1u : y;
+        const uint32_t uz = k == 2u ? z - 1u : k == 3u ? z + 1u : z;
+        const Slots<S> su = load_slots<S>(a.src, u);
+        const uint32_t occ = count_nonzero(su);
+        if (occ == 0u)
+            continue;
+        const Hood h = make_hood(a, ux, uy, uz, u, S);
+#pragma unroll 1
+        for (uint32_t s = 0; s < occ; ++s) {
+            const uint32_t sp = get_byte_dyn(su, s);
+            if (hop(a, tb, h, sp, s) == want) {
+                if (k < 3u) {
+                    put_byte_dyn(lo, nlo, sp);
+                    ++nlo;
+                } else {
+                    put_byte_dyn(hi, nhi, sp);
+                    ++nhi;
+                }
+            }
+        }
+    }
+}
+
+template <int S>
+struct WaveStaged {
+    typename SlotVec<S>::T rec[kWBatches];
+    uint32_t lab[3];
+};
+
+// One wave tile (tz, wy) of the lattice (a, w), marched through x-chunk `chunk`.
+template <int S>
+__device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const StepArgs &a, const WaveArgs &w,
+                                           uint32_t tz, uint32_t wy, uint32_t chunk, TileCounts &tc)
+{
+    using Vec = typename SlotVec<S>::T;
+    const int l = int(__lane_id());
+    const int y0 = int(wy) * kWY, z0 = int(tz) * kWZ;
+    const int xb = int(chunk * w.xc);
+    const int xe = min(int(a.R0), xb + int(w.xc));
+    const int R0 = int(a.R0), R1 = int(a.R1), R2 = int(a.R2);
+
+    // records of this lane: k < kWY interior row k; kWY, kWY+1 halo rows -1 and kWY; the last one a z-halo
+    // record for lanes < 2*kWY (rows 0..kWY-1 at column -1, then at column kWZ)
+    uint32_t off[kWBatches];
+    uint32_t okm = 0u, inl = 0u;  // record inside the lattice; interior voxel inside the lattice
+    int zrow = 0, zcol = -1;
+#pragma unroll
+    for (int k = 0; k < kWBatches; ++k) {
+        int row = k, col = l;
+        if (k == kWY) row = -1;
+        if (k == kWY + 1) row = kWY;
+        if (k == kWY + 2) {
+            row = (l < kWY) ? l : l - kWY;
+            col = (l < kWY) ? -1 : kWZ;
+            zrow = row;
+            zcol = col;
+        }
+        const int Y = y0 + row, Z = z0 + col;
+        const bool ok = (k < kWY + 2 || l < 2 * kWY) && Y >= 0 && Y < R1 && Z >= 0 && Z < R2;
+        off[k] = ok ? uint32_t(Z + Y * R2) : 0u;
+        okm |= uint32_t(ok) << k;
+        if (k < kWY)
+            inl |= uint32_t(ok) << k;
+    }
+    const uint32_t zrec = uint32_t((zrow + 1) * kWRZ + (zcol + 1));
+    // label words of this lane: word i of the plane's 8 x 17-word grid is row i / 17, word i % 17
+    uint32_t loff[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const int wd = min(l + 64 * i, kWLabWords - 1);
+        loff[i] = uint32_t((y0 + wd / (kWLZ / 4)) * int(w.PZ4) + z0 / 4 + wd % (kWLZ / 4));
+    }
+    const uint32_t plane_words = w.PY * w.PZ4;
+    auto lab_base = [&](int X) { return uint32_t(min(max(X, -2), R0 + 1) + 2) * plane_words; };
+    auto slot_base = [&](int X) { return (X >= 0 && X < R0) ? uint32_t(X) * a.R12 : 0u; };
+
+    // X % Y of the record rows -1 .. kWY on lanes 0 .. kWY+1 (the reference's seed term, rdme.wgsl:220-226);
+    // kept for the plane being hopped, advanced by one per plane
+    const int xrow = y0 + l - 1;
+    const bool xrow_ok = l < kWY + 2 && xrow > 0 && xrow < R1;
+    uint32_t xm = 0u;
+    if (xrow_ok)
+        xm = xb - 1 >= 0 ? uint32_t(xb - 1) % uint32_t(xrow) : uint32_t(xrow - 1);
+    const uint32_t xm_zero = a.mod_zero;
+
+    // labels of planes xb-2 and xb-1; slots of plane xb-1 and labels of plane xb into the first register set
+    auto store_lab = [&](int X, const uint32_t (&r)[3]) {
+        uint32_t *dst = L.lab[ring3(X)];
+        dst[l] = r[0];
+        dst[l + 64] = r[1];
+        if (l < kWLabWords - 128)
+            dst[l + 128] = r[2];
+    };
+    WaveStaged<S> st0, st1;
+    {
+        uint32_t r[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            r[i] = w.regp[lab_base(xb - 2) + loff[i]];
+        store_lab(xb - 2, r);
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            r[i] = w.regp[lab_base(xb - 1) + loff[i]];
+        store_lab(xb - 1, r);
+    }
+#pragma unroll
+    for (int k = 0; k < kWBatches; ++k)
+        st0.rec[k] = load_vec<S>(a.src, slot_base(xb - 1) + off[k]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        st0.lab[i] = w.regp[lab_base(xb) + loff[i]];
+    if (l < 3)
+        L.ev_n[l] = 0u;
+    if (l < 3 * kWY)
+        L.arrive[l / kWY][l % kWY] = 0ull;
+#pragma unroll
+    for (int q = 0; q < 3 * kWY; ++q)
+        L.head[q / kWY][(q % kWY) * 64 + l] = kWEvEnd;
+
+    uint32_t n_tight = 0, n_bad = 0, n_near = 0, n_fired = 0, n_rolled = 0;
+    uint32_t occ_prev = 0u;  // interior occupancy bits (one per row) of the plane gathered next
+    Vec prev[kWY];           // interior slots of that plane: the gather reads no global memory, where a
+                             // load would also wait for the prefetch issued before it (loads return in order)
+#pragma unroll
+    for (int r = 0; r < kWY; ++r)
+        prev[r] = Vec{};
+    // Reactions whose draw passes the bound test are rare in sparse tiles and common in dense ones; either way
+    // only a few lanes of a gather pass have one. They wait in a circular list and run 64 at a time.
+    uint32_t rb_head = 0u, rb_n = 0u;
+    auto react_flush = [&](uint32_t k) __attribute__((always_inline)) {
+        wave_sync();
+        const bool act = uint32_t(l) < k;
+        const uint32_t idx = (rb_head + uint32_t(l)) & uint32_t(kWReact - 1);
+        Slots<S> out = from_vec<S>(act ? L.rb_slots[idx] : Vec{});
+        const uint32_t V = L.rb_v[idx], xc = L.rb_xc[idx], yz = L.rb_yz[idx];
+        if (act) {
+            bool fired, rolled, near, bad;
+            react<S>(a, tb, out, xc >> 24, xc & 0xFFFFFFu, yz >> 16, yz & 0xFFFFu, fired, rolled, near, bad);
+            n_fired += fired;
+            n_rolled += rolled;
+            n_near += near;
+            n_bad += bad;
+            store_slots<S>(a.dst, V, out);
+        }
+        rb_head = (rb_head + k) & uint32_t(kWReact - 1);
+        rb_n -= k;
+        wave_sync();
+    };
+
+    auto plane = [&](int X, WaveStaged<S> &use, WaveStaged<S> &fill) __attribute__((always_inline)) {
+        // ---- prefetch plane X+1's slots and plane X+2's labels (clamped; unused past the chunk)
+        {
+            const uint32_t sb = slot_base(X + 1), lb = lab_base(X + 2);
+#pragma unroll
+            for (int k = 0; k < kWBatches; ++k)
+                fill.rec[k] = load_vec<S>(a.src, sb + off[k]);
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+                fill.lab[i] = w.regp[lb + loff[i]];
+        }
+        store_lab(X + 1, use.lab);
+        // plane X+1's event list starts empty (it was consumed as plane X-2's); plane X's leave masks too
+        if (l == 0)
+            L.ev_n[ring3(X + 1)] = 0u;
+        if (l < kWY)
+            L.arrive[ring3(X + 1)][l] = 0ull;
+#pragma unroll
+        for (int q = 0; q < kWY; ++q)
+            L.head[ring3(X + 1)][q * 64 + l] = kWEvEnd;
+        L.leave[X & 1][l] = 0u;
+        L.leave[X & 1][l + 64] = 0u;
+        wave_sync();
+
+        // ---- T0: list the occupied records of plane X
+        uint32_t n = 0u, occ_cur = 0u;
+        const bool hop_plane = X >= 0 && X < R0 && X <= xe;
+        if (hop_plane) {
+#pragma unroll
+            for (int k = 0; k < kWBatches; ++k) {
+                const bool occ = ((okm >> k) & 1u) && vec_nonzero<S>(use.rec[k]);
+                const uint64_t m = __ballot(occ);
+                const uint32_t rank =
+                    __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
+                if (occ) {
+                    uint32_t rec;
+                    if (k < kWY) rec = uint32_t((k + 1) * kWRZ + l + 1);
+                    else if (k == kWY) rec = uint32_t(l + 1);
+                    else if (k == kWY + 1) rec = uint32_t((kWY + 1) * kWRZ + l + 1);
+                    else rec = zrec;
+                    L.hop_slots[n + rank] = use.rec[k];
+                    L.hop_rec[n + rank] = uint16_t(rec);
+                }
+                n += uint32_t(__popcll(m));
+                if (k < kWY)
+                    occ_cur |= uint32_t(occ) << k;
+            }
+        }
+        wave_sync();
+
+        // ---- T1: hops of plane X
+        if (n != 0u) {
+            const uint8_t *lprev = reinterpret_cast<const uint8_t *>(L.lab[ring3(X - 1)]);
+            const uint8_t *lcur = reinterpret_cast<const uint8_t *>(L.lab[ring3(X)]);
+            const uint8_t *lnext = reinterpret_cast<const uint8_t *>(L.lab[ring3(X + 1)]);
+            const uint32_t n1 = a.ns + 1u;
+            for (uint32_t base = 0; base < n; base += 64u) {
+                const uint32_t i = base + uint32_t(l);
+                const bool act = i < n;
+                const uint32_t rec = act ? L.hop_rec[i] : uint32_t(kWRZ + 1);
+                const Slots<S> sl = from_vec<S>(act ? L.hop_slots[i] : Vec{});
+                const int rho = int(rec / kWRZ) - 1, zeta = int(rec % kWRZ) - 1;
+                const uint32_t Y = uint32_t(y0 + rho), Z = uint32_t(z0 + zeta);
+                const int gi = (rho + 2) * kWLZ + (zeta + 2);
+                Hood h;
+                h.src_reg = lcur[gi];
+                const uint32_t rg[6] = {lprev[gi], lnext[gi], lcur[gi - kWLZ], lcur[gi + kWLZ], lcur[gi - 1],
+                                        lcur[gi + 1]};
+                h.inside = 0u;
+#pragma unroll
+                for (int d = 0; d < 6; ++d) {
+                    const bool in = rg[d] != 0xFFu;
+                    h.inside |= uint32_t(in) << d;
+                    h.rg[d] = in ? rg[d] : 0u;
+                }
+                if (h.src_reg == 0xFFu)
+                    h.src_reg = 0u;  // an empty out-of-lattice filler lane
+                const uint32_t xmr = __shfl(xm, rho + 1);
+                const uint32_t xmy = Y == 0u ? xm_zero : xmr;
+                h.seed0 = a.itime + xmy + Z + (Z + Y * a.R2 + uint32_t(X) * a.R12) * uint32_t(S);
+                const uint32_t occ = act ? count_nonzero(sl) : 0u;
+                const bool interior = rho >= 0 && rho < kWY && zeta >= 0 && zeta < kWZ;
+                uint32_t lv = 0u;
+                for (uint32_t j = 0;; ++j) {
+                    const bool pj = j < occ;
+                    if (__ballot(pj) == 0ull)
+                        break;
+                    const uint32_t sp = get_byte_dyn(sl, j);
+                    const uint32_t d = pj ? hop(a, tb, h, sp, j) : 6u;
+                    (void)n1;
+                    const bool moves = d < 6u;
+                    lv |= uint32_t(moves) << j;
+                    const int dx = d == 0u ? -1 : (d == 1u ? 1 : 0);
+                    const int r2 = rho + (d == 2u ? -1 : (d == 3u ? 1 : 0));
+                    const int z2 = zeta + (d == 4u ? -1 : (d == 5u ? 1 : 0));
+                    const int X2 = X + dx;
+                    const bool ev = moves && r2 >= 0 && r2 < kWY && z2 >= 0 && z2 < kWZ && X2 >= xb && X2 < xe;
+                    if (__ballot(ev) == 0ull)
+                        continue;
+                    const uint32_t dest = uint32_t(r2 * kWZ + z2);
+#pragma unroll
+                    for (int t = -1; t <= 1; ++t) {
+                        const bool mine = ev && dx == t;
+                        const uint64_t m = __ballot(mine);
+                        if (m == 0ull)
+                            continue;
+                        const int rg3 = ring3(X + t);
+                        const uint32_t cnt0 = L.ev_n[rg3];
+                        const uint32_t rank = __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32),
+                                                                        __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
+                        if (mine) {
+                            const uint32_t idx = cnt0 + rank;
+                            if (idx < w.ev_cap) {
+                                const uint32_t prev = atomicExch(&L.head[rg3][dest], idx);
+                                L.ev[rg3][idx] = d | (sp << 3) | (prev << 11);
+                            }
+                            atomicOr(reinterpret_cast<unsigned long long *>(&L.arrive[rg3][r2]),
+                                     (unsigned long long)(1ull << z2));
+                        }
+                        wave_sync();
+                        if (l == 0)
+                            L.ev_n[rg3] = cnt0 + uint32_t(__popcll(m));
+                        wave_sync();
+                    }
+                }
+                if (act && interior && lv != 0u)
+                    reinterpret_cast<uint16_t *>(L.leave[X & 1])[rho * kWZ + zeta] = uint16_t(lv);
+            }
+        }
+        wave_sync();
+
+        // ---- T2: gather plane Q = X-1
+        const int Q = X - 1;
+        if (Q >= xb && Q < xe) {
+            const int rq = ring3(Q);
+            // a plane whose event list overflowed gathers every interior voxel's arrivals from global memory
+            const bool over = L.ev_n[rq] > w.ev_cap;
+            uint32_t ns = 0u, needm = 0u, sidx[kWY];
+#pragma unroll
+            for (int r = 0; r < kWY; ++r) {
+                const bool in = (inl >> r) & 1u;
+                const bool occ = (occ_prev >> r) & 1u;
+                const bool arr = ((L.arrive[rq][r] >> l) & 1ull) != 0ull;
+                const bool need = in && (occ || arr || over || w.has_zero_order != 0u);
+                if (in && !need)
+                    n_fired += (a.itime + uint32_t(Q) * uint32_t(y0 + r) + uint32_t(z0 + l)) == 0u;
+                const uint64_t m = __ballot(need);
+                const uint32_t rank =
+                    __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
+                sidx[r] = ns + rank;
+                needm |= uint32_t(need) << r;
+                if (need) {
+                    L.slow[ns + rank] = uint16_t(uint32_t(r * kWZ + l) | (uint32_t(occ) << 8));
+                    L.hop_slots[ns + rank] = occ ? prev[r] : Vec{};  // the hop list is done with
+                }
+                ns += uint32_t(__popcll(m));
+            }
+            wave_sync();
+            const uint16_t *lvq = reinterpret_cast<const uint16_t *>(L.leave[Q & 1]);
+            for (uint32_t base = 0; base < ns; base += 64u) {
+                const uint32_t i = base + uint32_t(l);
+                const bool act = i < ns;
+                const uint32_t code = act ? L.slow[i] : 0u;
+                const uint32_t me = code & 0xFFu;
+                const int r = int(me) / kWZ, z = int(me) % kWZ;
+                const uint32_t Y = uint32_t(y0 + r), Z = uint32_t(z0 + z);
+                const uint32_t V = Z + Y * a.R2 + uint32_t(Q) * a.R12;
+                const Slots<S> own = from_vec<S>(act ? L.hop_slots[i] : Vec{});
+                const uint32_t occ = count_nonzero(own);
+                const uint32_t lv = act ? lvq[me] : 0u;
+                const uint64_t stay = first_nibbles(occ) & ~spread_nibbles(lv);
+                Slots<S> lo = zero_slots<S>(), hi = zero_slots<S>();
+                uint32_t nlo = 0u, nhi = 0u;
+                if (!over) {
+                    // arrivals in the canonical order: from -x, -y, -z (lower-index sources), then from +z,
+                    // +y, +x, each in source-slot order. The chain holds a voxel's events newest first, so
+                    // each event goes in front of the earlier events of its rank (rank of hop direction d
+                    // in that order = nibble d of 0x231405).
+                    BV arr{0ull, 0ull};
+                    uint64_t cnts = 0ull;  // arrivals per rank, one byte each
+                    uint32_t e = act ? L.head[rq][me] : kWEvEnd;
+                    while (__ballot(e != kWEvEnd) != 0ull) {
+                        if (e != kWEvEnd) {
+                            const uint32_t c = L.ev[rq][e];
+                            const uint32_t k = (0x231405u >> (4u * (c & 7u))) & 0xFu;
+                            const uint32_t pos =
+                                k ? uint32_t(((cnts * 0x0101010101010101ull) >> (8u * (k - 1u))) & 0xFFull) : 0u;
+                            if (pos < 16u)
+                                arr = bv_insert(arr, pos, (c >> 3) & 0xFFu);
+                            cnts += 1ull << (8u * k);
+                            e = c >> 11;
+                        }
+                    }
+                    nlo = uint32_t((cnts * 0x010101ull) >> 16) & 0xFFu;
+                    nhi = uint32_t(((cnts >> 24) * 0x010101ull) >> 16) & 0xFFu;
+                    const BV lom = bytes_below(nlo);
+                    lo = from_bv<S>(BV{arr.lo & lom.lo, arr.hi & lom.hi});
+                    hi = from_bv<S>(shr_bytes(arr, nlo));
+                } else if (act) {
+                    gather_arrivals_rolled<S>(a, tb, Q, Y, Z, V, lo, nlo, hi, nhi);
+                }
+                if (act && occ + nlo + nhi > uint32_t(S)) {
+                    ++n_tight;
+                    note_tight(a, V);
+                }
+                uint32_t cnt = 0;
+                Slots<S> out = assemble<S>(own, occ, occ ? stay : 0ull, lo, min(nlo, 16u), hi, min(nhi, 16u), cnt);
+                const bool may = act && react_may_fire<S>(a, tb, cnt, uint32_t(Q), Y, Z);
+                // the result goes back to the list entry; the voxel's own lane stores whole rows below, so no
+                // output line is written in pieces at different times (a reaction still to run is stored
+                // by its batch instead, flagged here)
+                if (act) {
+                    L.hop_slots[i] = to_vec<S>(out);
+                    L.slow[i] = uint16_t(may ? 1u : 0u);
+                }
+                const uint64_t mr = __ballot(may);
+                if (mr != 0ull) {
+                    const uint32_t rank =
+                        __builtin_amdgcn_mbcnt_hi(uint32_t(mr >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mr), 0u));
+                    if (may) {
+                        const uint32_t idx = (rb_head + rb_n + rank) & uint32_t(kWReact - 1);
+                        L.rb_slots[idx] = to_vec<S>(out);
+                        L.rb_v[idx] = V;
+                        L.rb_xc[idx] = uint32_t(Q) | (cnt << 24);
+                        L.rb_yz[idx] = (Y << 16) | Z;
+                    }
+                    rb_n += uint32_t(__popcll(mr));
+                    if (rb_n >= 64u)
+                        react_flush(64u);
+                }
+            }
+            wave_sync();
+#pragma unroll
+            for (int r = 0; r < kWY; ++r) {
+                if ((inl >> r) & 1u) {
+                    const bool need = (needm >> r) & 1u;
+                    const Vec v = need ? L.hop_slots[sidx[r]] : Vec{};
+                    const bool deferred = need && L.slow[sidx[r]] != 0u;
+                    if (!deferred)
+                        *reinterpret_cast<Vec *>(a.dst + size_t(uint32_t(z0 + l) + uint32_t(y0 + r) * a.R2 +
+                                                                uint32_t(Q) * a.R12) * uint32_t(S)) = v;
+                }
+            }
+        }
+        occ_prev = occ_cur;
+#pragma unroll
+        for (int r = 0; r < kWY; ++r)
+            prev[r] = use.rec[r];
+        if (xrow_ok) {
+            xm = xm + 1u;
+            xm = xm == uint32_t(xrow) ? 0u : xm;
+        }
+        wave_sync();
+    };
+    // planes xb-1 .. xe in pairs; an odd count ends with plane xe+1, which lists and gathers nothing
+    for (int X = xb - 1; X <= xe; X += 2) {
+        plane(X, st0, st1);
+        plane(X + 1, st1, st0);
+    }
+    while (rb_n != 0u)
+        react_flush(min(rb_n, 64u));
+    tc.tight += n_tight;
+    tc.bad += n_bad;
+    tc.near += n_near;
+    tc.fired += n_fired;
+    tc.rolled += n_rolled;
+}
+
+// The plain step on wave tiles, as a persistent kernel: every block takes groups (kWavesPerBlock tiles
+// stacked in y in one x-chunk, one per wave) from a work queue until it is empty. The waves of a group
+// march side by side on one CU, so the halo rows one wave loads are the interior rows of its neighbours,
+// found in L1/L2. A group's time grows with its particles, and a dense region holds several times the
+// particles of the average group, so groups are queued by decreasing particle count: the longest start
+// first and the short ones fill in behind them. The queue is split per XCD (XCD k takes the groups of
+// rank k mod 8, an equal share of the heavy ones); an XCD whose queue is empty helps the others.
+template <int S>
+__global__ __launch_bounds__(kWaveBlock) void rdme_step_wave(StepArgs a, WaveArgs w)
+{
+    __shared__ WaveLds<S> L[kWavesPerBlock];
+    extern __shared__ uint32_t tab_lds[];
+    for (uint32_t i = threadIdx.x; i < w.tab_words; i += kWaveBlock)
+        tab_lds[i] = w.tab[i];
+    __syncthreads();
+    const uint8_t *tbase = reinterpret_cast<const uint8_t *>(tab_lds);
+    const Tab tb{reinterpret_cast<const float *>(tbase + w.off_p2), reinterpret_cast<const float *>(tbase + w.off_rates),
+                 reinterpret_cast<const uint32_t *>(tbase + w.off_ridx),
+                 reinterpret_cast<const int8_t *>(tbase + w.off_stoich),
+                 reinterpret_cast<const uint64_t *>(tbase + w.off_rmask),
+                 reinterpret_cast<const float *>(tbase + w.off_bound)};
+    const uint32_t wv = threadIdx.x / 64u;
+    // XCC_ID hardware register (id 20, bits 3:0): the XCD this block runs on
+    uint32_t xcd = uint32_t(__builtin_amdgcn_s_getreg((31 << 11) | 20)) & 0xFu;
+    xcd = __builtin_amdgcn_readfirstlane(xcd < 8u ? xcd : blockIdx.x % 8u);
+    __shared__ uint32_t s_item;
+    TileCounts tc;
+    for (uint32_t probe = 0; probe < 8u; ++probe) {
+        const uint32_t k = (xcd + probe) & 7u;
+        const uint32_t ngk = w.ngroups > k ? (w.ngroups - k + 7u) / 8u : 0u;
+        for (;;) {
+            if (threadIdx.x == 0)
+                s_item = atomicAdd(&w.queue[k], 1u);
+            __syncthreads();
+            const uint32_t it = s_item;
+            __syncthreads();
+            if (it >= ngk)
+                break;
+            const uint32_t g = w.order[it * 8u + k];
+            const uint32_t tz = g % w.tiles_z;
+            const uint32_t by = (g / w.tiles_z) % w.btiles_y;
+            const uint32_t chunk = g / (w.tiles_z * w.btiles_y);
+            const uint32_t wy = by * kWavesPerBlock + wv;
+            if (wy < w.wtiles_y)
+                wave_march<S>(L[wv], tb, a, w, tz, wy, chunk, tc);
+        }
+    }
     flush_counts(a.counters, tc);
 }
 
@@ -1912,8 +2536,16 @@ Device::Device(const Model &m, int hip_device) : dev_(hip_device)
     std::vector<uint8_t> reg8(N_);
     for (uint64_t v = 0; v < N_; ++v)
         reg8[v] = uint8_t(m.regions()[v]);
-    if (const char *k = std::getenv("CS_STEP_KERNEL"))
-        simple_ = std::string(k) == "simple";
+    if (const char *k = std::getenv("CS_STEP_KERNEL")) {
+        const std::string kk(k);
+        simple_ = kk == "simple";
+        tiled_ = kk == "tiled";
+    }
+    if (const char *x = std::getenv("CS_WAVE_EVENT_CAP")) {  // test hook: a smaller arrival-event list
+        const long v = std::strtol(x, nullptr, 10);
+        if (v >= 0 && v <= kWEv)
+            ev_cap_ = uint32_t(v);
+    }
     if (const char *x = std::getenv("CS_TILE_PLANES")) {
         const long v = std::strtol(x, nullptr, 10);
         if (v >= 1 && v <= 4096)
@@ -1923,6 +2555,18 @@ Device::Device(const Model &m, int hip_device) : dev_(hip_device)
     hip_check(hipMalloc(&lat_[0], N_ * S_), "hipMalloc lattice");
     hip_check(hipMalloc(&lat_[1], N_ * S_), "hipMalloc lattice");
     hip_check(hipMalloc(&reg_, N_), "hipMalloc regions");
+    // Labels padded to whole wave tiles plus two voxels on every side, 0xFF outside the lattice: the wave
+    // kernel reads its label window as aligned words without bounds tests, and 0xFF marks a missing neighbour.
+    wtiles_y_ = (res_[1] + kWY - 1) / kWY;
+    wtiles_z_ = (res_[2] + kWZ - 1) / kWZ;
+    pad_y_ = kWY * wtiles_y_ + 4u;
+    pad_z_ = kWZ * wtiles_z_ + 4u;
+    std::vector<uint8_t> regp(size_t(res_[0] + 4u) * pad_y_ * pad_z_, 0xFFu);
+    for (uint32_t x = 0; x < res_[0]; ++x)
+        for (uint32_t y = 0; y < res_[1]; ++y)
+            std::memcpy(&regp[(size_t(x + 2u) * pad_y_ + (y + 2u)) * pad_z_ + 2u],
+                        &reg8[(size_t(x) * res_[1] + y) * res_[2]], res_[2]);
+    hip_check(hipMalloc(&regp_, regp.size()), "hipMalloc padded regions");
 
     off_p2_ = blob.off_p2;
     off_rates_ = blob.off_rates;
@@ -1942,15 +2586,36 @@ Device::Device(const Model &m, int hip_device) : dev_(hip_device)
     bound_ = reinterpret_cast<float *>(tb + off_bound_);
     if (blob_bytes > kTabLdsMax)
         simple_ = true;  // tables too large to stage in LDS next to the plane records
+    if (nreg_ >= 255u)
+        tiled_ = true;   // the wave kernel's padded labels use 0xFF for "outside the lattice"
+    {
+        // resident blocks of the persistent wave kernel: every CU filled to its occupancy
+        int cus = 0, per_cu = 0;
+        hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev_), "CU count");
+        const size_t shm = blob_bytes;
+        hipError_t e = hipSuccess;
+        switch (S_) {
+        case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rdme_step_wave<4>, kWaveBlock, shm); break;
+        case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rdme_step_wave<8>, kWaveBlock, shm); break;
+        default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rdme_step_wave<16>, kWaveBlock, shm); break;
+        }
+        hip_check(e, "occupancy of the wave kernel");
+        wave_blocks_ = uint32_t(std::max(cus, 1) * std::max(per_cu, 1));
+        if (const char *x = std::getenv("CS_WAVE_BLOCKS")) {  // tuning hook: resident blocks
+            const long v = std::strtol(x, nullptr, 10);
+            if (v >= 1 && v <= 65536)
+                wave_blocks_ = uint32_t(v);
+        }
+    }
     hip_check(hipMalloc(&counters_, kNumCounters * sizeof(unsigned long long)), "hipMalloc counters");
     hip_check(hipMalloc(&hist_, size_t(nreg_) * n1 * sizeof(int32_t)), "hipMalloc hist");
-    hip_check(hipMalloc(&fix_ctl_, 4 * sizeof(uint32_t)), "hipMalloc fix_ctl");
+    hip_check(hipMalloc(&fix_ctl_, kFixCtl * sizeof(uint32_t)), "hipMalloc fix_ctl");
     hip_check(hipMalloc(&fix_list_, size_t(kMaxTight) * sizeof(uint32_t)), "hipMalloc fix_list");
     hip_check(hipMalloc(&fix_F_, size_t(8 * kMaxTight) * sizeof(uint32_t)), "hipMalloc fix_F");
     hip_check(hipMalloc(&fix_W_, size_t(56 * kMaxTight) * sizeof(uint32_t)), "hipMalloc fix_W");
     hip_check(hipMalloc(&fix_L_, size_t(8 * kMaxTight) * S_), "hipMalloc fix_L");
     hip_check(hipMalloc(&fix_O_, size_t(8 * kMaxTight) * sizeof(uint32_t)), "hipMalloc fix_O");
-    hip_check(hipMemsetAsync(fix_ctl_, 0, 4 * sizeof(uint32_t), st), "clear fix_ctl");
+    hip_check(hipMemsetAsync(fix_ctl_, 0, kFixCtl * sizeof(uint32_t), st), "clear fix_ctl");
 
     // Well-mixed (CME) pool and its tables in one blob (8-byte aligned sections), plus the voxel lists of
     // the regions that receive lattice products.
@@ -1981,7 +2646,9 @@ Device::Device(const Model &m, int hip_device) : dev_(hip_device)
     }
 
     hip_check(hipMemcpyAsync(lat_[0], m.lattice().data(), N_ * S_, hipMemcpyHostToDevice, st), "upload lattice");
+    plan_wave_order(m.lattice().data());
     hip_check(hipMemcpyAsync(reg_, reg8.data(), N_, hipMemcpyHostToDevice, st), "upload regions");
+    hip_check(hipMemcpyAsync(regp_, regp.data(), regp.size(), hipMemcpyHostToDevice, st), "upload padded regions");
     hip_check(hipMemcpyAsync(tab_, blob.bytes.data(), blob_bytes, hipMemcpyHostToDevice, st), "upload tables");
     hip_check(hipMemsetAsync(counters_, 0, kNumCounters * sizeof(unsigned long long), st), "clear counters");
     hip_check(hipStreamSynchronize(st), "upload");
@@ -1996,6 +2663,8 @@ Device::~Device()
     hipFree(lat_[0]);
     hipFree(lat_[1]);
     hipFree(reg_);
+    hipFree(regp_);
+    hipFree(group_order_);
     hipFree(tab_);
     hipFree(counters_);
     hipFree(hist_);
@@ -2078,6 +2747,37 @@ void Device::step(uint32_t itime)
         case 16: hipLaunchKernelGGL(rdme_step_kernel<16>, dim3(blocks), dim3(kBlock), 0, st, a); break;
         default: throw Error(CS_ERR_UNSUPPORTED, "slots per voxel must be 4, 8 or 16");
         }
+    } else if (!tiled_) {
+        WaveArgs w;
+        w.tiles_z = wtiles_z_;
+        w.wtiles_y = wtiles_y_;
+        w.btiles_y = (wtiles_y_ + kWavesPerBlock - 1) / kWavesPerBlock;
+        w.xc = xc_;
+        w.chunks = (res_[0] + xc_ - 1) / xc_;
+        w.has_zero_order = has_zero_order_ ? 1u : 0u;
+        w.ev_cap = ev_cap_;
+        w.tab_words = tab_words_;
+        w.off_p2 = off_p2_;
+        w.off_rates = off_rates_;
+        w.off_ridx = off_ridx_;
+        w.off_stoich = off_stoich_;
+        w.off_rmask = off_rmask_;
+        w.off_bound = off_bound_;
+        w.tab = static_cast<const uint32_t *>(tab_);
+        w.regp = reinterpret_cast<const uint32_t *>(regp_);
+        w.PY = pad_y_;
+        w.PZ4 = pad_z_ / 4u;
+        w.ngroups = w.tiles_z * w.btiles_y * w.chunks;
+        w.queue = fix_ctl_ + kFixQueue;
+        w.order = group_order_;
+        const uint32_t blocks = std::min(w.ngroups, wave_blocks_);
+        const size_t shm = size_t(tab_words_) * 4u;
+        switch (S_) {
+        case 4: hipLaunchKernelGGL(rdme_step_wave<4>, dim3(blocks), dim3(kWaveBlock), shm, st, a, w); break;
+        case 8: hipLaunchKernelGGL(rdme_step_wave<8>, dim3(blocks), dim3(kWaveBlock), shm, st, a, w); break;
+        case 16: hipLaunchKernelGGL(rdme_step_wave<16>, dim3(blocks), dim3(kWaveBlock), shm, st, a, w); break;
+        default: throw Error(CS_ERR_UNSUPPORTED, "slots per voxel must be 4, 8 or 16");
+        }
     } else {
         TileArgs t;
         t.tiles_z = (res_[2] + kTZ - 1) / kTZ;
@@ -2155,7 +2855,46 @@ void Device::write_lattice(const uint8_t *in)
     hip_check(hipSetDevice(dev_), "hipSetDevice");
     hipStream_t st = static_cast<hipStream_t>(stream_);
     hip_check(hipMemcpyAsync(lat_[cur_], in, N_ * S_, hipMemcpyHostToDevice, st), "write lattice");
+    plan_wave_order(in);
     hip_check(hipStreamSynchronize(st), "write lattice");
+}
+
+// Queue order of the wave kernel's tile groups: by decreasing particle count in the given lattice (the
+// spatial density changes slowly, so the order of one state serves the steps after it).
+void Device::plan_wave_order(const uint8_t *lat)
+{
+    const uint32_t tz_n = wtiles_z_, by_n = (wtiles_y_ + kWavesPerBlock - 1) / kWavesPerBlock;
+    const uint32_t chunks = (res_[0] + xc_ - 1) / xc_;
+    const uint32_t ng = tz_n * by_n * chunks;
+    std::vector<uint64_t> cost(ng, 0);
+    const uint32_t gy = uint32_t(kWY * kWavesPerBlock);
+    for (uint32_t x = 0; x < res_[0]; ++x)
+        for (uint32_t y = 0; y < res_[1]; ++y) {
+            const uint32_t gbase = tz_n * (y / gy + by_n * (x / xc_));
+            const uint8_t *row = lat + (size_t(x) * res_[1] + y) * res_[2] * S_;
+            for (uint32_t z = 0; z < res_[2]; ++z) {
+                const uint8_t *v = row + size_t(z) * S_;
+                uint32_t c = 0;
+                for (uint32_t k = 0; k < S_ && v[k]; ++k)
+                    ++c;
+                cost[gbase + z / uint32_t(kWZ)] += c;
+            }
+        }
+    std::vector<uint32_t> order(ng);
+    for (uint32_t g = 0; g < ng; ++g)
+        order[g] = g;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t p, uint32_t q) { return cost[p] > cost[q]; });
+    if (!group_order_ || group_order_n_ != ng) {
+        if (group_order_)
+            hipFree(group_order_);
+        group_order_ = nullptr;
+        hip_check(hipMalloc(&group_order_, std::max<size_t>(ng, 1) * sizeof(uint32_t)), "hipMalloc group order");
+        group_order_n_ = ng;
+    }
+    hipStream_t st = static_cast<hipStream_t>(stream_);
+    hip_check(hipMemcpyAsync(group_order_, order.data(), ng * sizeof(uint32_t), hipMemcpyHostToDevice, st),
+              "upload group order");
+    hip_check(hipStreamSynchronize(st), "upload group order");
 }
 
 void Device::read_cme_counts(int32_t *out)
@@ -2339,7 +3078,7 @@ Ensemble::Ensemble(const std::vector<const Model *> &cells, const std::vector<ui
     hip_check(hipMalloc(&counters_, kNumCounters * sizeof(unsigned long long)), "hipMalloc counters");
     hip_check(hipMalloc(&audit_, 3 * sizeof(unsigned long long)), "hipMalloc audit");
     hip_check(hipMalloc(&cme_counts_, std::max<size_t>(cme_init.size(), 1) * sizeof(int32_t)), "hipMalloc cme counts");
-    hip_check(hipMalloc(&fix_ctl_, 4 * sizeof(uint32_t)), "hipMalloc fix_ctl");
+    hip_check(hipMalloc(&fix_ctl_, kFixCtl * sizeof(uint32_t)), "hipMalloc fix_ctl");
     hip_check(hipMalloc(&fix_list_, size_t(kMaxTight) * sizeof(uint32_t)), "hipMalloc fix_list");
     hip_check(hipMalloc(&fix_F_, size_t(8 * kMaxTight) * sizeof(uint32_t)), "hipMalloc fix_F");
     hip_check(hipMalloc(&fix_W_, size_t(56 * kMaxTight) * sizeof(uint32_t)), "hipMalloc fix_W");
@@ -2383,7 +3122,7 @@ Ensemble::Ensemble(const std::vector<const Model *> &cells, const std::vector<ui
     hip_check(hipMemsetAsync(queue_, 0, 2 * sizeof(uint32_t), st), "clear queue");
     hip_check(hipMemsetAsync(counters_, 0, kNumCounters * sizeof(unsigned long long), st), "clear counters");
     hip_check(hipMemsetAsync(audit_, 0, 3 * sizeof(unsigned long long), st), "clear audit");
-    hip_check(hipMemsetAsync(fix_ctl_, 0, 4 * sizeof(uint32_t), st), "clear fix_ctl");
+    hip_check(hipMemsetAsync(fix_ctl_, 0, kFixCtl * sizeof(uint32_t), st), "clear fix_ctl");
     build_descriptors();
     hip_check(hipStreamSynchronize(st), "ensemble upload");
 }

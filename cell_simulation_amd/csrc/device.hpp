@@ -45,6 +45,8 @@ public:
     void read_cme_audit(double *max_rel_error, uint64_t *divergences, uint64_t *events);
 
 private:
+    void plan_wave_order(const uint8_t *lat);
+
     int dev_ = 0;
     void *stream_ = nullptr;       // hipStream_t
     uint32_t res_[3] = {0, 0, 0};
@@ -55,10 +57,17 @@ private:
     int cur_ = 0;                  // index of the buffer holding the current lattice
     bool has_zero_order_ = false;  // some reaction has no reactants
     bool simple_ = false;          // one-lane-per-voxel kernel without LDS staging
-    uint32_t xc_ = 16;             // planes marched per block by the tiled kernel
+    bool tiled_ = false;           // block-shared plane-march kernel instead of the wave-march kernel
+    uint32_t xc_ = 8;              // planes marched per tile
+    uint32_t wtiles_y_ = 0, wtiles_z_ = 0, pad_y_ = 0, pad_z_ = 0;  // wave tiles; padded label volume
+    uint32_t ev_cap_ = 96;         // arrival events a wave-tile plane may hold
+    uint32_t wave_blocks_ = 256;   // resident blocks of the persistent wave kernel
+    uint32_t *group_order_ = nullptr;  // wave kernel: tile groups by decreasing particle count
+    uint32_t group_order_n_ = 0;
 
     uint8_t *lat_[2] = {nullptr, nullptr};
     uint8_t *reg_ = nullptr;
+    uint8_t *regp_ = nullptr;      // labels padded by two voxels (and to whole wave tiles), 0xFF outside
     void *tab_ = nullptr;          // one blob with every table below, at the byte offsets off_*
     uint32_t tab_words_ = 0;
     uint32_t off_p2_ = 0, off_rates_ = 0, off_ridx_ = 0, off_rmask_ = 0, off_stoich_ = 0, off_bound_ = 0;
