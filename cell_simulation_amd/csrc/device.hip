@@ -2102,7 +2102,8 @@ struct EnsCell {
     uint64_t lat_off;          // byte offset of the cell's slots in each lattice buffer
     uint32_t vox_off;          // index of the cell's voxel 0 among all voxels (labels, tight list)
     uint32_t R0, R1, R2;
-    uint32_t tiles_z, tiles_y, chunks, first_item;
+    uint32_t tiles_z, wtiles_y, btiles_y, chunks, first_item;  // wave tiles in z and y, groups in y, x-chunks
+    uint32_t regp_off, PY, PZ4;  // the cell's padded labels: word offset, rows per plane, words per row
     uint32_t tab_off, tab_words;  // the cell's table blob, in 32-bit words of the tables buffer
     uint32_t off_p2, off_rates, off_ridx, off_stoich, off_rmask, off_bound;
     uint32_t ns, nr, nreg, nrw, has_zero_order, mod_zero;
@@ -2117,6 +2118,7 @@ struct EnsArgs {
     uint32_t n_cells, total_items, xc, batch;
     uint8_t *src, *dst;
     const uint8_t *reg;
+    const uint32_t *regp;      // padded labels of every cell (words)
     const uint32_t *tab;
     uint32_t *itime;           // [n_cells], written by the prologue of each step
     uint32_t *queue;           // [0] next item, [1] step counter
@@ -2171,23 +2173,31 @@ __device__ __forceinline__ StepArgs cell_step_args(const EnsArgs &e, const EnsCe
     return a;
 }
 
-__device__ __forceinline__ TileArgs cell_tile_args(const EnsArgs &e, const EnsCell &c)
+__device__ __forceinline__ WaveArgs cell_wave_args(const EnsArgs &e, const EnsCell &c)
 {
-    TileArgs t;
-    t.tiles_z = c.tiles_z;
-    t.tiles_y = c.tiles_y;
-    t.chunks = c.chunks;
-    t.xc = e.xc;
-    t.has_zero_order = c.has_zero_order;
-    t.tab_words = c.tab_words;
-    t.off_p2 = c.off_p2;
-    t.off_rates = c.off_rates;
-    t.off_ridx = c.off_ridx;
-    t.off_stoich = c.off_stoich;
-    t.off_rmask = c.off_rmask;
-    t.off_bound = c.off_bound;
-    t.tab = e.tab + c.tab_off;
-    return t;
+    WaveArgs w;
+    w.tiles_z = c.tiles_z;
+    w.wtiles_y = c.wtiles_y;
+    w.btiles_y = c.btiles_y;
+    w.chunks = c.chunks;
+    w.xc = e.xc;
+    w.has_zero_order = c.has_zero_order;
+    w.ev_cap = uint32_t(kWEv);
+    w.tab_words = c.tab_words;
+    w.off_p2 = c.off_p2;
+    w.off_rates = c.off_rates;
+    w.off_ridx = c.off_ridx;
+    w.off_stoich = c.off_stoich;
+    w.off_rmask = c.off_rmask;
+    w.off_bound = c.off_bound;
+    w.tab = e.tab + c.tab_off;
+    w.regp = e.regp + c.regp_off;
+    w.PY = c.PY;
+    w.PZ4 = c.PZ4;
+    w.ngroups = 0u;
+    w.queue = nullptr;
+    w.order = nullptr;
+    return w;
 }
 
 // Per step, one lane per cell: the cell's itime from the device step counter, then its well-mixed phase.
@@ -2204,15 +2214,18 @@ __global__ __launch_bounds__(64) void ens_prologue(EnsArgs e, const CmeArgs *cme
         cme_cell<S>(cme[cell.cme], cme_counts + cell.cme_counts_off, e.src + cell.lat_off, cell.cme_seed, it);
 }
 
-// The lattice step of every cell: persistent blocks take (cell, tile, x-chunk) items from the queue
-// (`batch` consecutive items per access) or, with batch 0, in a fixed round-robin.
+// The lattice step of every cell: persistent blocks take (cell, group, x-chunk) items from the queue
+// (`batch` consecutive items per access) or, with batch 0, in a fixed round-robin. A group is
+// kWavesPerBlock wave tiles stacked in y; the block stages the cell's tables in LDS and each wave marches
+// its tile.
 template <int S>
-__global__ __launch_bounds__(kBlock) void rdme_step_ensemble(EnsArgs e)
+__global__ __launch_bounds__(kWaveBlock) void rdme_step_ensemble(EnsArgs e)
 {
-    __shared__ TileLds<S> L;
+    __shared__ WaveLds<S> L[kWavesPerBlock];
     __shared__ uint32_t next;
     extern __shared__ uint32_t tab_lds[];
     TileCounts tc;
+    const uint32_t wv = threadIdx.x / 64u;
     const uint32_t per = e.batch ? e.batch : 1u;
     uint32_t rr = blockIdx.x;  // round-robin position when there is no queue
     for (;;) {
@@ -2242,11 +2255,24 @@ __global__ __launch_bounds__(kBlock) void rdme_step_ensemble(EnsArgs e)
             }
             const EnsCell &c = e.cells[lo];
             const StepArgs a = cell_step_args(e, c, e.itime[lo]);
-            const TileArgs t = cell_tile_args(e, c);
+            const WaveArgs w = cell_wave_args(e, c);
+            for (uint32_t i = threadIdx.x; i < w.tab_words; i += kWaveBlock)
+                tab_lds[i] = w.tab[i];
+            __syncthreads();
+            const uint8_t *tbase = reinterpret_cast<const uint8_t *>(tab_lds);
+            const Tab tb{reinterpret_cast<const float *>(tbase + w.off_p2),
+                         reinterpret_cast<const float *>(tbase + w.off_rates),
+                         reinterpret_cast<const uint32_t *>(tbase + w.off_ridx),
+                         reinterpret_cast<const int8_t *>(tbase + w.off_stoich),
+                         reinterpret_cast<const uint64_t *>(tbase + w.off_rmask),
+                         reinterpret_cast<const float *>(tbase + w.off_bound)};
             const uint32_t local = item - c.first_item;
-            const uint32_t tz = local % c.tiles_z, ty = (local / c.tiles_z) % c.tiles_y;
-            const uint32_t chunk = local / (c.tiles_z * c.tiles_y);
-            tile_march<S>(L, tab_lds, a, t, tz, ty, chunk, tc);
+            const uint32_t tz = local % c.tiles_z, by = (local / c.tiles_z) % c.btiles_y;
+            const uint32_t chunk = local / (c.tiles_z * c.btiles_y);
+            const uint32_t wy = by * kWavesPerBlock + wv;
+            if (wy < c.wtiles_y)
+                wave_march<S>(L[wv], tb, a, w, tz, wy, chunk, tc);
+            __syncthreads();  // the tables stay until every wave of the group is done
         }
     }
     flush_counts(e.counters, tc);
@@ -3071,6 +3097,23 @@ Ensemble::Ensemble(const std::vector<const Model *> &cells, const std::vector<ui
     hip_check(hipMalloc(&lat_[0], total_vox_ * S_), "hipMalloc ensemble lattice");
     hip_check(hipMalloc(&lat_[1], total_vox_ * S_), "hipMalloc ensemble lattice");
     hip_check(hipMalloc(&reg_, total_vox_), "hipMalloc ensemble regions");
+    // padded labels per cell (see Device): two voxels of 0xFF around the lattice, rows padded to wave tiles
+    regp_off_.assign(n, 0);
+    uint64_t regp_words = 0;
+    {
+        EnsCell *d = desc.data();
+        for (uint32_t c = 0; c < n; ++c) {
+            d[c].PY = kWY * ((d[c].R1 + kWY - 1) / kWY) + 4u;
+            d[c].PZ4 = (kWZ * ((d[c].R2 + kWZ - 1) / kWZ) + 4u) / 4u;
+            regp_off_[c] = regp_words;
+            if (regp_words >= (uint64_t(1) << 32))
+                throw Error(CS_ERR_UNSUPPORTED, "padded labels of the ensemble exceed 2^32 words");
+            d[c].regp_off = uint32_t(regp_words);
+            regp_words += uint64_t(d[c].R0 + 4u) * d[c].PY * d[c].PZ4;
+        }
+        std::memcpy(cells_.data(), desc.data(), cells_.size());
+    }
+    hip_check(hipMalloc(&regp_, std::max<uint64_t>(regp_words, 1) * 4u), "hipMalloc ensemble padded regions");
     hip_check(hipMalloc(&tab_, tab_words.size() * sizeof(uint32_t)), "hipMalloc ensemble tables");
     hip_check(hipMalloc(&cells_dev_, cells_.size()), "hipMalloc ensemble cells");
     hip_check(hipMalloc(&itime_, size_t(n) * sizeof(uint32_t)), "hipMalloc itime");
@@ -3106,6 +3149,15 @@ Ensemble::Ensemble(const std::vector<const Model *> &cells, const std::vector<ui
             reg8[v] = uint8_t(m.regions()[v]);
         hip_check(hipMemcpyAsync(reg_ + vox_off_[c], reg8.data(), reg8.size(), hipMemcpyHostToDevice, st),
                   "upload ensemble regions");
+        const EnsCell &dc = desc[c];
+        const uint32_t PZ = dc.PZ4 * 4u;
+        std::vector<uint8_t> regp(size_t(dc.R0 + 4u) * dc.PY * PZ, 0xFFu);
+        for (uint32_t x = 0; x < dc.R0; ++x)
+            for (uint32_t y = 0; y < dc.R1; ++y)
+                std::memcpy(&regp[(size_t(x + 2u) * dc.PY + (y + 2u)) * PZ + 2u],
+                            &reg8[(size_t(x) * dc.R1 + y) * dc.R2], dc.R2);
+        hip_check(hipMemcpyAsync(static_cast<uint32_t *>(regp_) + regp_off_[c], regp.data(), regp.size(),
+                                 hipMemcpyHostToDevice, st), "upload ensemble padded regions");
         hip_check(hipStreamSynchronize(st), "upload ensemble cell");  // reg8 is freed next iteration
     }
     hip_check(hipMemcpyAsync(tab_, tab_words.data(), tab_words.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st),
@@ -3133,7 +3185,7 @@ Ensemble::~Ensemble()
     if (stream_)
         hipStreamSynchronize(static_cast<hipStream_t>(stream_));
     destroy_graph();
-    void *bufs[] = {lat_[0], lat_[1], reg_, tab_, cells_dev_, cme_args_dev_, cme_tab_, cme_counts_, itime_, queue_,
+    void *bufs[] = {lat_[0], lat_[1], reg_, regp_, tab_, cells_dev_, cme_args_dev_, cme_tab_, cme_counts_, itime_, queue_,
                     counters_, audit_, fix_ctl_, fix_list_, fix_F_, fix_W_, fix_L_, fix_O_};
     for (void *b : bufs)
         if (b)
@@ -3148,11 +3200,12 @@ void Ensemble::build_descriptors()
     EnsCell *d = reinterpret_cast<EnsCell *>(cells_.data());
     uint64_t items = 0;
     for (uint32_t c = 0; c < n; ++c) {
-        d[c].tiles_z = (d[c].R2 + kTZ - 1) / kTZ;
-        d[c].tiles_y = (d[c].R1 + kTY - 1) / kTY;
+        d[c].tiles_z = (d[c].R2 + kWZ - 1) / kWZ;
+        d[c].wtiles_y = (d[c].R1 + kWY - 1) / kWY;
+        d[c].btiles_y = (d[c].wtiles_y + kWavesPerBlock - 1) / kWavesPerBlock;
         d[c].chunks = (d[c].R0 + tiling_.xc - 1) / tiling_.xc;
         d[c].first_item = uint32_t(items);
-        items += uint64_t(d[c].tiles_z) * d[c].tiles_y * d[c].chunks;
+        items += uint64_t(d[c].tiles_z) * d[c].btiles_y * d[c].chunks;
     }
     if (items >= (uint64_t(1) << 31))
         throw Error(CS_ERR_UNSUPPORTED, "too many work items");
@@ -3194,6 +3247,7 @@ void Ensemble::enqueue_step()
     e.src = lat_[cur_];
     e.dst = lat_[cur_ ^ 1];
     e.reg = reg_;
+    e.regp = static_cast<const uint32_t *>(regp_);
     e.tab = tab_;
     e.itime = itime_;
     e.queue = queue_;
@@ -3212,17 +3266,17 @@ void Ensemble::enqueue_step()
     switch (S_) {
     case 4:
         hipLaunchKernelGGL(ens_prologue<4>, dim3(pro_blocks), dim3(64), 0, st, e, ca, cme_counts_);
-        hipLaunchKernelGGL(rdme_step_ensemble<4>, dim3(grid), dim3(kBlock), shm, st, e);
+        hipLaunchKernelGGL(rdme_step_ensemble<4>, dim3(grid), dim3(kWaveBlock), shm, st, e);
         hipLaunchKernelGGL(ens_epilogue<4>, dim3(1), dim3(64), 0, st, e);
         break;
     case 8:
         hipLaunchKernelGGL(ens_prologue<8>, dim3(pro_blocks), dim3(64), 0, st, e, ca, cme_counts_);
-        hipLaunchKernelGGL(rdme_step_ensemble<8>, dim3(grid), dim3(kBlock), shm, st, e);
+        hipLaunchKernelGGL(rdme_step_ensemble<8>, dim3(grid), dim3(kWaveBlock), shm, st, e);
         hipLaunchKernelGGL(ens_epilogue<8>, dim3(1), dim3(64), 0, st, e);
         break;
     default:
         hipLaunchKernelGGL(ens_prologue<16>, dim3(pro_blocks), dim3(64), 0, st, e, ca, cme_counts_);
-        hipLaunchKernelGGL(rdme_step_ensemble<16>, dim3(grid), dim3(kBlock), shm, st, e);
+        hipLaunchKernelGGL(rdme_step_ensemble<16>, dim3(grid), dim3(kWaveBlock), shm, st, e);
         hipLaunchKernelGGL(ens_epilogue<16>, dim3(1), dim3(64), 0, st, e);
         break;
     }

@@ -74,6 +74,7 @@ private:
 
     std::vector<uint64_t> lat_off_;   // per cell, bytes
     std::vector<uint64_t> vox_off_;   // per cell, voxels
+    std::vector<uint64_t> regp_off_;  // per cell, words of the padded label buffer
     std::vector<uint32_t> res_;       // per cell, 3 entries
     std::vector<uint32_t> cell_nc_;   // well-mixed species per cell
     std::vector<uint32_t> cme_counts_off_;
@@ -81,6 +82,7 @@ private:
 
     uint8_t *lat_[2] = {nullptr, nullptr};
     uint8_t *reg_ = nullptr;
+    void *regp_ = nullptr;             // padded labels of every cell (the wave kernel's label windows)
     uint32_t *tab_ = nullptr;          // all cells' table blobs
     void *cells_dev_ = nullptr;        // EnsCell[n]
     void *cme_args_dev_ = nullptr;     // CmeArgs[n_cme]
