@@ -239,11 +239,12 @@ __device__ __forceinline__ uint32_t hop(const StepArgs &a, const Tab &tb, const 
 {
     const float r = uniform_float(pcg_hash(h.seed0 + slot));
     const uint32_t n1 = a.ns + 1u;
-    const float *row = tb.p2 + n1 * a.nreg * h.src_reg + sp;
+    // table indices are below 2^24: 24-bit multiplies run at full rate, 32-bit ones at a quarter
+    const float *row = tb.p2 + __umul24(n1 * a.nreg, h.src_reg) + sp;
     float val[6];
 #pragma unroll
     for (int d = 0; d < 6; ++d)
-        val[d] = row[n1 * h.rg[d]];
+        val[d] = row[__umul24(n1, h.rg[d])];
     float acc = 0.0f;
     uint32_t dir = 6u;  // 6: no stop yet; 7: stopped on a missing neighbour (stays)
 #pragma unroll
@@ -325,7 +326,8 @@ template <int S>
 __device__ __forceinline__ bool react_may_fire(const StepArgs &a, const Tab &tb, uint32_t cnt, uint32_t x,
                                                uint32_t y, uint32_t z)
 {
-    const float r1 = uniform_float(pcg_hash(a.itime + x * y + z));
+    // x * y with x, y < 2^24: the low 32 bits of the 24-bit product are those of the 32-bit one
+    const float r1 = uniform_float(pcg_hash(a.itime + __umul24(x, y) + z));
     return !(r1 > tb.bound[cnt] * a.tau * 1.001f + 1e-6f);
 }
 
@@ -1478,6 +1480,52 @@ __device__ __forceinline__ void gather_arrivals_rolled(const StepArgs &a, const 
     }
 }
 
+// The byte-vector helpers on a voxel's first four slots held in one word (positions 0..3).
+__device__ __forceinline__ uint32_t b4_below(uint32_t d) { return d >= 4u ? ~0u : ((1u << (8u * d)) - 1u); }
+__device__ __forceinline__ uint32_t b4_shl(uint32_t v, uint32_t k) { return k >= 4u ? 0u : v << (8u * k); }
+__device__ __forceinline__ uint32_t b4_shr(uint32_t v, uint32_t k) { return k >= 4u ? 0u : v >> (8u * k); }
+__device__ __forceinline__ uint32_t b4_byte(uint32_t v, uint32_t j) { return (v >> (8u * (j & 3u))) & 0xFFu; }
+__device__ __forceinline__ uint32_t b4_set(uint32_t v, uint32_t d, uint32_t val)
+{
+    const uint32_t sh = 8u * (d & 3u);
+    return (v & ~(0xFFu << sh)) | (val << sh);
+}
+__device__ __forceinline__ uint32_t b4_delete(uint32_t v, uint32_t d)
+{
+    const uint32_t m = b4_below(d);
+    return (v & m) | ((v >> 8) & ~m);
+}
+// `val` inserted at position d (< 4); the bytes from d up move one place (the top one is lost).
+__device__ __forceinline__ uint32_t b4_insert(uint32_t v, uint32_t d, uint32_t val)
+{
+    const uint32_t m = b4_below(d);
+    return d >= 4u ? v : b4_set((v & m) | ((v << 8) & ~m), d, val);
+}
+
+// assemble() for a voxel that ends with at most four particles: the same canonical picture on one word.
+__device__ __forceinline__ uint32_t assemble4(uint32_t own, uint32_t occ, uint32_t stay_mask, uint32_t lo,
+                                              uint32_t nlo, uint32_t hi, uint32_t nhi, uint32_t &cnt)
+{
+    uint32_t o = own;
+    uint32_t rem = (occ >= 4u ? 0x1111u : ((1u << (4u * occ)) - 1u) & 0x1111u) & ~stay_mask;
+    uint32_t used = 0;
+    while (rem != 0u && used < nhi) {
+        const uint32_t sl = uint32_t(__ffs(rem) - 1) >> 2;
+        rem &= rem - 1u;
+        o = b4_set(o, sl, b4_byte(hi, used));
+        ++used;
+    }
+    const uint32_t j = occ - uint32_t(__popc(rem));
+    while (rem != 0u) {  // highest first, so the lower positions stay valid
+        const uint32_t bit = 31u - uint32_t(__clz(rem));
+        rem &= ~(1u << bit);
+        o = b4_delete(o, bit >> 2);
+    }
+    o |= b4_shl(lo, j) | b4_shl(b4_shr(hi, used), j + nlo);
+    cnt = j + nlo + (nhi - used);
+    return o;
+}
+
 template <int S>
 struct WaveStaged {
     typename SlotVec<S>::T rec[kWBatches];
@@ -1575,7 +1623,13 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
         L.head[q / kWY][(q % kWY) * 64 + l] = kWEvEnd;
 
     uint32_t n_tight = 0, n_bad = 0, n_near = 0, n_fired = 0, n_rolled = 0;
-    uint32_t occ_prev = 0u;  // interior occupancy bits (one per row) of the plane gathered next
+    uint64_t occ_prev[kWY], occ_cur[kWY], inm[kWY];  // per interior row: occupied lanes of the plane
+                                                      // gathered next, of the plane listed now; in-lattice lanes
+#pragma unroll
+    for (int r = 0; r < kWY; ++r) {
+        occ_prev[r] = occ_cur[r] = 0ull;
+        inm[r] = __ballot((inl >> r) & 1u);
+    }
     Vec prev[kWY];           // interior slots of that plane: the gather reads no global memory, where a
                              // load would also wait for the prefetch issued before it (loads return in order)
 #pragma unroll
@@ -1629,7 +1683,7 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
         wave_sync();
 
         // ---- T0: list the occupied records of plane X
-        uint32_t n = 0u, occ_cur = 0u;
+        uint32_t n = 0u;
         const bool hop_plane = X >= 0 && X < R0 && X <= xe;
         if (hop_plane) {
 #pragma unroll
@@ -1649,7 +1703,7 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
                 }
                 n += uint32_t(__popcll(m));
                 if (k < kWY)
-                    occ_cur |= uint32_t(occ) << k;
+                    occ_cur[k] = m;
             }
         }
         wave_sync();
@@ -1683,7 +1737,7 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
                     h.src_reg = 0u;  // an empty out-of-lattice filler lane
                 const uint32_t xmr = __shfl(xm, rho + 1);
                 const uint32_t xmy = Y == 0u ? xm_zero : xmr;
-                h.seed0 = a.itime + xmy + Z + (Z + Y * a.R2 + uint32_t(X) * a.R12) * uint32_t(S);
+                h.seed0 = a.itime + xmy + Z + (Z + __umul24(Y, a.R2) + uint32_t(X) * a.R12) * uint32_t(S);
                 const uint32_t occ = act ? count_nonzero(sl) : 0u;
                 const bool interior = rho >= 0 && rho < kWY && zeta >= 0 && zeta < kWZ;
                 uint32_t lv = 0u;
@@ -1741,16 +1795,19 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
             const int rq = ring3(Q);
             // a plane whose event list overflowed gathers every interior voxel's arrivals from global memory
             const bool over = L.ev_n[rq] > w.ev_cap;
+            // the voxels to gather, as wave-wide row masks: occupied, or a particle arrives (everything when the
+            // event list overflowed or empty voxels can react)
+            const uint64_t every = (over || w.has_zero_order != 0u) ? ~0ull : 0ull;
             uint32_t ns = 0u, needm = 0u, sidx[kWY];
 #pragma unroll
             for (int r = 0; r < kWY; ++r) {
-                const bool in = (inl >> r) & 1u;
-                const bool occ = (occ_prev >> r) & 1u;
-                const bool arr = ((L.arrive[rq][r] >> l) & 1ull) != 0ull;
-                const bool need = in && (occ || arr || over || w.has_zero_order != 0u);
-                if (in && !need)
-                    n_fired += (a.itime + uint32_t(Q) * uint32_t(y0 + r) + uint32_t(z0 + l)) == 0u;
-                const uint64_t m = __ballot(need);
+                const uint64_t m = inm[r] & (occ_prev[r] | L.arrive[rq][r] | every);
+                // an empty voxel's zero-propensity draw fires at seed 0 (itime + x*y + z == 0)
+                const uint32_t zfire = 0u - a.itime - uint32_t(Q) * uint32_t(y0 + r) - uint32_t(z0);
+                if (l == 0 && zfire < uint32_t(kWZ))
+                    n_fired += uint32_t((inm[r] & ~m) >> zfire) & 1u;
+                const bool need = (m >> l) & 1ull;
+                const bool occ = (occ_prev[r] >> l) & 1ull;
                 const uint32_t rank =
                     __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
                 sidx[r] = ns + rank;
@@ -1770,47 +1827,78 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
                 const uint32_t me = code & 0xFFu;
                 const int r = int(me) / kWZ, z = int(me) % kWZ;
                 const uint32_t Y = uint32_t(y0 + r), Z = uint32_t(z0 + z);
-                const uint32_t V = Z + Y * a.R2 + uint32_t(Q) * a.R12;
+                const uint32_t V = Z + __umul24(Y, a.R2) + uint32_t(Q) * a.R12;
                 const Slots<S> own = from_vec<S>(act ? L.hop_slots[i] : Vec{});
                 const uint32_t occ = count_nonzero(own);
                 const uint32_t lv = act ? lvq[me] : 0u;
                 const uint64_t stay = first_nibbles(occ) & ~spread_nibbles(lv);
-                Slots<S> lo = zero_slots<S>(), hi = zero_slots<S>();
-                uint32_t nlo = 0u, nhi = 0u;
-                if (!over) {
-                    // arrivals in the canonical order: from -x, -y, -z (lower-index sources), then from +z,
-                    // +y, +x, each in source-slot order. The chain holds a voxel's events newest first, so
-                    // each event goes in front of the earlier events of its rank (rank of hop direction d
-                    // in that order = nibble d of 0x231405).
-                    BV arr{0ull, 0ull};
-                    uint64_t cnts = 0ull;  // arrivals per rank, one byte each
-                    uint32_t e = act ? L.head[rq][me] : kWEvEnd;
-                    while (__ballot(e != kWEvEnd) != 0ull) {
-                        if (e != kWEvEnd) {
-                            const uint32_t c = L.ev[rq][e];
-                            const uint32_t k = (0x231405u >> (4u * (c & 7u))) & 0xFu;
-                            const uint32_t pos =
-                                k ? uint32_t(((cnts * 0x0101010101010101ull) >> (8u * (k - 1u))) & 0xFFull) : 0u;
-                            if (pos < 16u)
-                                arr = bv_insert(arr, pos, (c >> 3) & 0xFFu);
-                            cnts += 1ull << (8u * k);
-                            e = c >> 11;
+                // Most voxels end with at most four particles: then the arrivals and the assembled slots fit
+                // one 32-bit word, and the wave takes that path when every lane fits it.
+                uint32_t cnt = 0u;
+                Slots<S> out = zero_slots<S>();
+                bool done = false;
+                if constexpr (S > 4) {
+                    if (!over && __ballot(occ > 4u) == 0ull) {
+                        uint32_t arr = 0u, tot = 0u;
+                        uint64_t cnts = 0ull;
+                        uint32_t e = act ? L.head[rq][me] : kWEvEnd;
+                        while (__ballot(e != kWEvEnd) != 0ull) {
+                            if (e != kWEvEnd) {
+                                const uint32_t c = L.ev[rq][e];
+                                const uint32_t k = (0x231405u >> (4u * (c & 7u))) & 0xFu;
+                                const uint32_t pos =
+                                    k ? uint32_t(((cnts * 0x0101010101010101ull) >> (8u * (k - 1u))) & 0xFFull) : 0u;
+                                arr = b4_insert(arr, pos, (c >> 3) & 0xFFu);
+                                cnts += 1ull << (8u * k);
+                                ++tot;
+                                e = c >> 11;
+                            }
+                        }
+                        if (__ballot(occ + tot > 4u) == 0ull) {
+                            const uint32_t nlo4 = uint32_t((cnts * 0x010101ull) >> 16) & 0xFFu;
+                            out.w[0] = assemble4(own.w[0], occ, occ ? uint32_t(stay) : 0u, arr & b4_below(nlo4),
+                                                 nlo4, b4_shr(arr, nlo4), tot - nlo4, cnt);
+                            done = true;
                         }
                     }
-                    nlo = uint32_t((cnts * 0x010101ull) >> 16) & 0xFFu;
-                    nhi = uint32_t(((cnts >> 24) * 0x010101ull) >> 16) & 0xFFu;
-                    const BV lom = bytes_below(nlo);
-                    lo = from_bv<S>(BV{arr.lo & lom.lo, arr.hi & lom.hi});
-                    hi = from_bv<S>(shr_bytes(arr, nlo));
-                } else if (act) {
-                    gather_arrivals_rolled<S>(a, tb, Q, Y, Z, V, lo, nlo, hi, nhi);
                 }
-                if (act && occ + nlo + nhi > uint32_t(S)) {
-                    ++n_tight;
-                    note_tight(a, V);
+                if (!done) {
+                    Slots<S> lo = zero_slots<S>(), hi = zero_slots<S>();
+                    uint32_t nlo = 0u, nhi = 0u;
+                    if (!over) {
+                        // arrivals in the canonical order: from -x, -y, -z (lower-index sources), then from +z,
+                        // +y, +x, each in source-slot order. The chain holds a voxel's events newest first, so
+                        // each event goes in front of the earlier events of its rank (rank of hop direction d
+                        // in that order = nibble d of 0x231405).
+                        BV arr{0ull, 0ull};
+                        uint64_t cnts = 0ull;  // arrivals per rank, one byte each
+                        uint32_t e = act ? L.head[rq][me] : kWEvEnd;
+                        while (__ballot(e != kWEvEnd) != 0ull) {
+                            if (e != kWEvEnd) {
+                                const uint32_t c = L.ev[rq][e];
+                                const uint32_t k = (0x231405u >> (4u * (c & 7u))) & 0xFu;
+                                const uint32_t pos =
+                                    k ? uint32_t(((cnts * 0x0101010101010101ull) >> (8u * (k - 1u))) & 0xFFull) : 0u;
+                                if (pos < 16u)
+                                    arr = bv_insert(arr, pos, (c >> 3) & 0xFFu);
+                                cnts += 1ull << (8u * k);
+                                e = c >> 11;
+                            }
+                        }
+                        nlo = uint32_t((cnts * 0x010101ull) >> 16) & 0xFFu;
+                        nhi = uint32_t(((cnts >> 24) * 0x010101ull) >> 16) & 0xFFu;
+                        const BV lom = bytes_below(nlo);
+                        lo = from_bv<S>(BV{arr.lo & lom.lo, arr.hi & lom.hi});
+                        hi = from_bv<S>(shr_bytes(arr, nlo));
+                    } else if (act) {
+                        gather_arrivals_rolled<S>(a, tb, Q, Y, Z, V, lo, nlo, hi, nhi);
+                    }
+                    if (act && occ + nlo + nhi > uint32_t(S)) {
+                        ++n_tight;
+                        note_tight(a, V);
+                    }
+                    out = assemble<S>(own, occ, occ ? stay : 0ull, lo, min(nlo, 16u), hi, min(nhi, 16u), cnt);
                 }
-                uint32_t cnt = 0;
-                Slots<S> out = assemble<S>(own, occ, occ ? stay : 0ull, lo, min(nlo, 16u), hi, min(nhi, 16u), cnt);
                 const bool may = act && react_may_fire<S>(a, tb, cnt, uint32_t(Q), Y, Z);
                 // the result goes back to the list entry; the voxel's own lane stores whole rows below, so no
                 // output line is written in pieces at different times (a reaction still to run is stored
@@ -1843,12 +1931,15 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
                     const Vec v = need ? L.hop_slots[sidx[r]] : Vec{};
                     const bool deferred = need && L.slow[sidx[r]] != 0u;
                     if (!deferred)
-                        *reinterpret_cast<Vec *>(a.dst + size_t(uint32_t(z0 + l) + uint32_t(y0 + r) * a.R2 +
-                                                                uint32_t(Q) * a.R12) * uint32_t(S)) = v;
+                        *reinterpret_cast<Vec *>(a.dst + size_t(off[r] + uint32_t(Q) * a.R12) * uint32_t(S)) = v;
                 }
             }
         }
-        occ_prev = occ_cur;
+#pragma unroll
+        for (int r = 0; r < kWY; ++r) {
+            occ_prev[r] = occ_cur[r];
+            occ_cur[r] = 0ull;
+        }
 #pragma unroll
         for (int r = 0; r < kWY; ++r)
             prev[r] = use.rec[r];
@@ -2614,6 +2705,8 @@ Device::Device(const Model &m, int hip_device) : dev_(hip_device)
         simple_ = true;  // tables too large to stage in LDS next to the plane records
     if (nreg_ >= 255u)
         tiled_ = true;   // the wave kernel's padded labels use 0xFF for "outside the lattice"
+    if (res_[1] > 4096u || res_[2] > 4096u)
+        tiled_ = true;   // the wave kernel forms y * R2 with 24-bit multiplies
     {
         // resident blocks of the persistent wave kernel: every CU filled to its occupancy
         int cus = 0, per_cu = 0;
@@ -3056,6 +3149,8 @@ Ensemble::Ensemble(const std::vector<const Model *> &cells, const std::vector<ui
         const TableBlob b = make_table_blob(m);
         if (b.bytes.size() > kTabLdsMax)
             throw Error(CS_ERR_UNSUPPORTED, "cell tables too large to stage in LDS");
+        if (lp.res[1] > 4096u || lp.res[2] > 4096u || m.num_regions() >= 255u)
+            throw Error(CS_ERR_UNSUPPORTED, "ensemble cells need y and z extents up to 4096 and fewer than 255 regions");
         d.tab_off = uint32_t(tab_words.size());
         d.tab_words = uint32_t(b.bytes.size() / 4u);
         tab_words.resize(tab_words.size() + d.tab_words);
