@@ -1369,9 +1369,9 @@ struct WaveArgs {
     const uint32_t *tab;
     const uint32_t *regp;   // padded labels [R0+4][PY][PZ] (0xFF outside the lattice), read as words
     uint32_t PY, PZ4;       // PY = kWY*wtiles_y + 4 rows; PZ4 = (kWZ*tiles_z + 4) / 4 words per row
-    uint32_t ngroups;       // groups of kWavesPerBlock tiles stacked in y: tiles_z * btiles_y * chunks
+    uint32_t nitems;        // work items: kWavesPerBlock tiles stacked in y over an x-range
     uint32_t *queue;        // [8] work counters, one per XCD; zero at the start of every step
-    const uint32_t *order;  // [ngroups] groups by decreasing particle count (the longest tiles start first)
+    const uint2 *items;     // [nitems] tz | by << 16, x_begin | x_end << 16; by decreasing expected time
 };
 
 template <int S>
@@ -1535,13 +1535,11 @@ struct WaveStaged {
 // One wave tile (tz, wy) of the lattice (a, w), marched through x-chunk `chunk`.
 template <int S>
 __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const StepArgs &a, const WaveArgs &w,
-                                           uint32_t tz, uint32_t wy, uint32_t chunk, TileCounts &tc)
+                                           uint32_t tz, uint32_t wy, int xb, int xe, TileCounts &tc)
 {
     using Vec = typename SlotVec<S>::T;
     const int l = int(__lane_id());
     const int y0 = int(wy) * kWY, z0 = int(tz) * kWZ;
-    const int xb = int(chunk * w.xc);
-    const int xe = min(int(a.R0), xb + int(w.xc));
     const int R0 = int(a.R0), R1 = int(a.R1), R2 = int(a.R2);
 
     // records of this lane: k < kWY interior row k; kWY, kWY+1 halo rows -1 and kWY; the last one a z-halo
@@ -1685,9 +1683,14 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
         // ---- T0: list the occupied records of plane X
         uint32_t n = 0u;
         const bool hop_plane = X >= 0 && X < R0 && X <= xe;
+        // on the planes beside the x-range only the interior rows matter: a particle there reaches the
+        // tile with a hop along x or not at all
+        const int batches = (X == xb - 1 || X == xe) ? kWY : kWBatches;
         if (hop_plane) {
 #pragma unroll
             for (int k = 0; k < kWBatches; ++k) {
+                if (k >= batches)
+                    break;
                 const bool occ = ((okm >> k) & 1u) && vec_nonzero<S>(use.rec[k]);
                 const uint64_t m = __ballot(occ);
                 const uint32_t rank =
@@ -1992,7 +1995,7 @@ __global__ __launch_bounds__(kWaveBlock) void rdme_step_wave(StepArgs a, WaveArg
     TileCounts tc;
     for (uint32_t probe = 0; probe < 8u; ++probe) {
         const uint32_t k = (xcd + probe) & 7u;
-        const uint32_t ngk = w.ngroups > k ? (w.ngroups - k + 7u) / 8u : 0u;
+        const uint32_t ngk = w.nitems > k ? (w.nitems - k + 7u) / 8u : 0u;
         for (;;) {
             if (threadIdx.x == 0)
                 s_item = atomicAdd(&w.queue[k], 1u);
@@ -2001,13 +2004,11 @@ __global__ __launch_bounds__(kWaveBlock) void rdme_step_wave(StepArgs a, WaveArg
             __syncthreads();
             if (it >= ngk)
                 break;
-            const uint32_t g = w.order[it * 8u + k];
-            const uint32_t tz = g % w.tiles_z;
-            const uint32_t by = (g / w.tiles_z) % w.btiles_y;
-            const uint32_t chunk = g / (w.tiles_z * w.btiles_y);
+            const uint2 item = w.items[it * 8u + k];
+            const uint32_t tz = item.x & 0xFFFFu, by = item.x >> 16;
             const uint32_t wy = by * kWavesPerBlock + wv;
             if (wy < w.wtiles_y)
-                wave_march<S>(L[wv], tb, a, w, tz, wy, chunk, tc);
+                wave_march<S>(L[wv], tb, a, w, tz, wy, int(item.y & 0xFFFFu), int(item.y >> 16), tc);
         }
     }
     flush_counts(a.counters, tc);
@@ -2285,9 +2286,9 @@ __device__ __forceinline__ WaveArgs cell_wave_args(const EnsArgs &e, const EnsCe
     w.regp = e.regp + c.regp_off;
     w.PY = c.PY;
     w.PZ4 = c.PZ4;
-    w.ngroups = 0u;
+    w.nitems = 0u;
     w.queue = nullptr;
-    w.order = nullptr;
+    w.items = nullptr;
     return w;
 }
 
@@ -2362,7 +2363,7 @@ __global__ __launch_bounds__(kWaveBlock) void rdme_step_ensemble(EnsArgs e)
             const uint32_t chunk = local / (c.tiles_z * c.btiles_y);
             const uint32_t wy = by * kWavesPerBlock + wv;
             if (wy < c.wtiles_y)
-                wave_march<S>(L[wv], tb, a, w, tz, wy, chunk, tc);
+                wave_march<S>(L[wv], tb, a, w, tz, wy, int(chunk * e.xc), min(int(c.R0), int((chunk + 1u) * e.xc)), tc);
             __syncthreads();  // the tables stay until every wave of the group is done
         }
     }
@@ -2663,10 +2664,20 @@ Device::Device(const Model &m, int hip_device) : dev_(hip_device)
         if (v >= 0 && v <= kWEv)
             ev_cap_ = uint32_t(v);
     }
-    if (const char *x = std::getenv("CS_TILE_PLANES")) {
+    if (const char *x = std::getenv("CS_WAVE_ITEMS")) {  // tuning hook: work items per resident block
+        const long v = std::strtol(x, nullptr, 10);
+        if (v >= 1 && v <= 256)
+            items_per_block_ = uint32_t(v);
+    }
+    if (const char *x = std::getenv("CS_WAVE_MIN_PLANES")) {  // tuning hook: shortest x-range of an item
         const long v = std::strtol(x, nullptr, 10);
         if (v >= 1 && v <= 4096)
-            xc_ = uint32_t(v);
+            xc_min_ = uint32_t(v);
+    }
+    if (const char *x = std::getenv("CS_TILE_PLANES")) {  // fixed chunk length (tests, tuning)
+        const long v = std::strtol(x, nullptr, 10);
+        if (v >= 1 && v <= 4096)
+            xc_ = xc_min_ = tiled_xc_ = uint32_t(v);
     }
 
     hip_check(hipMalloc(&lat_[0], N_ * S_), "hipMalloc lattice");
@@ -2705,8 +2716,8 @@ Device::Device(const Model &m, int hip_device) : dev_(hip_device)
         simple_ = true;  // tables too large to stage in LDS next to the plane records
     if (nreg_ >= 255u)
         tiled_ = true;   // the wave kernel's padded labels use 0xFF for "outside the lattice"
-    if (res_[1] > 4096u || res_[2] > 4096u)
-        tiled_ = true;   // the wave kernel forms y * R2 with 24-bit multiplies
+    if (res_[1] > 4096u || res_[2] > 4096u || res_[0] > 65535u)
+        tiled_ = true;   // the wave kernel forms y * R2 with 24-bit multiplies, packs x into 16 bits
     {
         // resident blocks of the persistent wave kernel: every CU filled to its occupancy
         int cus = 0, per_cu = 0;
@@ -2765,7 +2776,7 @@ Device::Device(const Model &m, int hip_device) : dev_(hip_device)
     }
 
     hip_check(hipMemcpyAsync(lat_[0], m.lattice().data(), N_ * S_, hipMemcpyHostToDevice, st), "upload lattice");
-    plan_wave_order(m.lattice().data());
+    plan_wave_items(m.lattice().data());
     hip_check(hipMemcpyAsync(reg_, reg8.data(), N_, hipMemcpyHostToDevice, st), "upload regions");
     hip_check(hipMemcpyAsync(regp_, regp.data(), regp.size(), hipMemcpyHostToDevice, st), "upload padded regions");
     hip_check(hipMemcpyAsync(tab_, blob.bytes.data(), blob_bytes, hipMemcpyHostToDevice, st), "upload tables");
@@ -2783,7 +2794,7 @@ Device::~Device()
     hipFree(lat_[1]);
     hipFree(reg_);
     hipFree(regp_);
-    hipFree(group_order_);
+    hipFree(items_);
     hipFree(tab_);
     hipFree(counters_);
     hipFree(hist_);
@@ -2886,10 +2897,10 @@ void Device::step(uint32_t itime)
         w.regp = reinterpret_cast<const uint32_t *>(regp_);
         w.PY = pad_y_;
         w.PZ4 = pad_z_ / 4u;
-        w.ngroups = w.tiles_z * w.btiles_y * w.chunks;
+        w.nitems = n_items_;
         w.queue = fix_ctl_ + kFixQueue;
-        w.order = group_order_;
-        const uint32_t blocks = std::min(w.ngroups, wave_blocks_);
+        w.items = static_cast<const uint2 *>(items_);
+        const uint32_t blocks = std::min(n_items_, wave_blocks_);
         const size_t shm = size_t(tab_words_) * 4u;
         switch (S_) {
         case 4: hipLaunchKernelGGL(rdme_step_wave<4>, dim3(blocks), dim3(kWaveBlock), shm, st, a, w); break;
@@ -2901,8 +2912,8 @@ void Device::step(uint32_t itime)
         TileArgs t;
         t.tiles_z = (res_[2] + kTZ - 1) / kTZ;
         t.tiles_y = (res_[1] + kTY - 1) / kTY;
-        t.xc = xc_;
-        t.chunks = (res_[0] + xc_ - 1) / xc_;
+        t.xc = tiled_xc_;
+        t.chunks = (res_[0] + tiled_xc_ - 1) / tiled_xc_;
         t.has_zero_order = has_zero_order_ ? 1u : 0u;
         t.tab_words = tab_words_;
         t.off_p2 = off_p2_;
@@ -2974,46 +2985,75 @@ void Device::write_lattice(const uint8_t *in)
     hip_check(hipSetDevice(dev_), "hipSetDevice");
     hipStream_t st = static_cast<hipStream_t>(stream_);
     hip_check(hipMemcpyAsync(lat_[cur_], in, N_ * S_, hipMemcpyHostToDevice, st), "write lattice");
-    plan_wave_order(in);
+    plan_wave_items(in);
     hip_check(hipStreamSynchronize(st), "write lattice");
 }
 
-// Queue order of the wave kernel's tile groups: by decreasing particle count in the given lattice (the
-// spatial density changes slowly, so the order of one state serves the steps after it).
-void Device::plan_wave_order(const uint8_t *lat)
+// Work items of the wave kernel: each column of kWavesPerBlock wave tiles stacked in y is cut along x into
+// ranges of about equal expected time, queued longest first. A tile's time is a fixed cost per plane plus a
+// cost per particle (about 70 particles' worth per tile and plane, measured), and every range also marches
+// the plane on each side of it; so ranges through dense regions are short (a long one would finish last)
+// and ranges through empty space long (fewer side planes). The particle counts come from the given
+// lattice: the spatial density changes slowly, so the plan of one state serves the steps after it.
+void Device::plan_wave_items(const uint8_t *lat)
 {
     const uint32_t tz_n = wtiles_z_, by_n = (wtiles_y_ + kWavesPerBlock - 1) / kWavesPerBlock;
-    const uint32_t chunks = (res_[0] + xc_ - 1) / xc_;
-    const uint32_t ng = tz_n * by_n * chunks;
-    std::vector<uint64_t> cost(ng, 0);
-    const uint32_t gy = uint32_t(kWY * kWavesPerBlock);
-    for (uint32_t x = 0; x < res_[0]; ++x)
+    const uint32_t R0 = res_[0], gy = uint32_t(kWY * kWavesPerBlock);
+    std::vector<uint64_t> cnt(size_t(tz_n) * by_n * R0, 0);  // particles per (column, plane)
+    for (uint32_t x = 0; x < R0; ++x)
         for (uint32_t y = 0; y < res_[1]; ++y) {
-            const uint32_t gbase = tz_n * (y / gy + by_n * (x / xc_));
             const uint8_t *row = lat + (size_t(x) * res_[1] + y) * res_[2] * S_;
+            uint64_t *c = &cnt[(size_t(y / gy) * tz_n) * R0 + x];
             for (uint32_t z = 0; z < res_[2]; ++z) {
                 const uint8_t *v = row + size_t(z) * S_;
-                uint32_t c = 0;
-                for (uint32_t k = 0; k < S_ && v[k]; ++k)
-                    ++c;
-                cost[gbase + z / uint32_t(kWZ)] += c;
+                uint32_t k = 0;
+                while (k < S_ && v[k])
+                    ++k;
+                c[size_t(z / uint32_t(kWZ)) * R0] += k;
             }
         }
-    std::vector<uint32_t> order(ng);
-    for (uint32_t g = 0; g < ng; ++g)
-        order[g] = g;
-    std::stable_sort(order.begin(), order.end(), [&](uint32_t p, uint32_t q) { return cost[p] > cost[q]; });
-    if (!group_order_ || group_order_n_ != ng) {
-        if (group_order_)
-            hipFree(group_order_);
-        group_order_ = nullptr;
-        hip_check(hipMalloc(&group_order_, std::max<size_t>(ng, 1) * sizeof(uint32_t)), "hipMalloc group order");
-        group_order_n_ = ng;
+    const double plane_cost = 70.0 * kWavesPerBlock;
+    double total = 0.0;
+    for (uint64_t v : cnt)
+        total += double(v) + plane_cost;
+    // about `items_per_block_` items per resident block, so the longest ones end with the rest
+    const double target = std::max(total / (double(items_per_block_) * std::max(wave_blocks_, 1u)), 2.0 * plane_cost);
+    const uint32_t min_len = std::min<uint32_t>(xc_min_, R0), max_len = std::max(xc_, min_len);
+    struct Item {
+        double cost;
+        uint32_t col, x0, x1;
+    };
+    std::vector<Item> items;
+    for (uint32_t col = 0; col < tz_n * by_n; ++col) {
+        const uint64_t *c = &cnt[size_t(col) * R0];
+        uint32_t x0 = 0;
+        while (x0 < R0) {
+            double cost = 2.0 * plane_cost;  // the side planes
+            uint32_t x1 = x0;
+            while (x1 < R0 && (x1 - x0 < min_len || (x1 - x0 < max_len && cost + double(c[x1]) + plane_cost <= target))) {
+                cost += double(c[x1]) + plane_cost;
+                ++x1;
+            }
+            items.push_back({cost, col, x0, x1});
+            x0 = x1;
+        }
     }
+    std::stable_sort(items.begin(), items.end(), [](const Item &p, const Item &q) { return p.cost > q.cost; });
+    std::vector<uint32_t> packed(2 * items.size());
+    for (size_t i = 0; i < items.size(); ++i) {
+        const uint32_t tz = items[i].col % tz_n, by = items[i].col / tz_n;
+        packed[2 * i] = tz | (by << 16);
+        packed[2 * i + 1] = items[i].x0 | (items[i].x1 << 16);
+    }
+    if (items_)
+        hipFree(items_);
+    items_ = nullptr;
+    hip_check(hipMalloc(&items_, std::max<size_t>(packed.size(), 2) * sizeof(uint32_t)), "hipMalloc wave items");
+    n_items_ = uint32_t(items.size());
     hipStream_t st = static_cast<hipStream_t>(stream_);
-    hip_check(hipMemcpyAsync(group_order_, order.data(), ng * sizeof(uint32_t), hipMemcpyHostToDevice, st),
-              "upload group order");
-    hip_check(hipStreamSynchronize(st), "upload group order");
+    hip_check(hipMemcpyAsync(items_, packed.data(), packed.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st),
+              "upload wave items");
+    hip_check(hipStreamSynchronize(st), "upload wave items");
 }
 
 void Device::read_cme_counts(int32_t *out)

@@ -45,7 +45,7 @@ public:
     void read_cme_audit(double *max_rel_error, uint64_t *divergences, uint64_t *events);
 
 private:
-    void plan_wave_order(const uint8_t *lat);
+    void plan_wave_items(const uint8_t *lat);
 
     int dev_ = 0;
     void *stream_ = nullptr;       // hipStream_t
@@ -58,12 +58,16 @@ private:
     bool has_zero_order_ = false;  // some reaction has no reactants
     bool simple_ = false;          // one-lane-per-voxel kernel without LDS staging
     bool tiled_ = false;           // block-shared plane-march kernel instead of the wave-march kernel
-    uint32_t xc_ = 8;              // planes marched per tile
+    uint32_t xc_ = 32;             // planes marched per tile: the longest x-range of a wave-kernel work item
     uint32_t wtiles_y_ = 0, wtiles_z_ = 0, pad_y_ = 0, pad_z_ = 0;  // wave tiles; padded label volume
     uint32_t ev_cap_ = 96;         // arrival events a wave-tile plane may hold
     uint32_t wave_blocks_ = 256;   // resident blocks of the persistent wave kernel
-    uint32_t *group_order_ = nullptr;  // wave kernel: tile groups by decreasing particle count
-    uint32_t group_order_n_ = 0;
+    void *items_ = nullptr;        // wave kernel work items (uint2 each), longest first
+    uint32_t n_items_ = 0;
+    uint32_t xc_min_ = 12;         // shortest x-range of a work item (its two side planes cost as much as
+                                   // 1/6 of it; a longer one through a dense column would finish last)
+    uint32_t items_per_block_ = 8; // wave-kernel work items per resident block (planning target)
+    uint32_t tiled_xc_ = 16;       // planes marched per block by the block-tiled kernel
 
     uint8_t *lat_[2] = {nullptr, nullptr};
     uint8_t *reg_ = nullptr;
