@@ -881,6 +881,43 @@ __device__ __forceinline__ Slots<S> from_vec(const typename SlotVec<S>::T &v)
 
 constexpr int kHopSeg = 192;  // hop items a wave can list per plane
 
+// The part of a record the wave kernel stages: all of it up to 8 slots; the first 8 of 16. A voxel holding more
+// (its slot 7 is taken) is rare and reads its whole record from global memory where needed.
+template <int S> struct HeadVec { using T = typename SlotVec<S>::T; };
+template <> struct HeadVec<16> { using T = uint2; };
+template <int S> constexpr int kHeadSlots = S > 8 ? 8 : S;
+
+template <int S>
+__device__ __forceinline__ Slots<S> from_head(const typename HeadVec<S>::T &h)
+{
+    if constexpr (S == 16) {
+        Slots<S> s;
+        s.w[0] = h.x; s.w[1] = h.y; s.w[2] = 0u; s.w[3] = 0u;
+        return s;
+    } else {
+        return from_vec<S>(h);
+    }
+}
+
+template <int S>
+__device__ __forceinline__ typename HeadVec<S>::T to_head(const Slots<S> &s)
+{
+    if constexpr (S == 16)
+        return make_uint2(s.w[0], s.w[1]);
+    else
+        return to_vec<S>(s);
+}
+
+// The head's last slot is taken: the voxel may hold more particles than the head shows.
+template <int S>
+__device__ __forceinline__ bool head_full(const typename HeadVec<S>::T &h)
+{
+    if constexpr (S == 16)
+        return (h.y >> 24) != 0u;
+    else
+        return false;
+}
+
 template <int S>
 struct TileLds {
     typename SlotVec<S>::T slots[kRing][kRec];
@@ -1354,11 +1391,13 @@ constexpr int kWLabWords = kWLY * kWLZ / 4;       // 136
 constexpr int kWBatches = kWY + 3;                // records per lane: interior rows, rows -1 and kWY, z halo
 constexpr int kWHop = kWY * kWZ + 2 * kWZ + 2 * kWY;  // records a plane can list (392)
 constexpr int kWEv = 96;                          // arrival events per destination plane
-constexpr int kWReact = 128;                      // deferred reactions a wave holds (processed 64 at a time)
+constexpr int kWReact = 80;                       // deferred reactions a wave holds (runs of up to 64 once 16 wait)
+constexpr int kWReactRun = 16;
 constexpr uint32_t kWEvEnd = 0xFFu;               // end of an event chain
 constexpr int kWaveBlock = 256;                   // four waves: four tiles stacked in y share the tables
 constexpr int kWavesPerBlock = kWaveBlock / 64;
-static_assert(kWZ == 64 && kWLZ % 4 == 0 && kWLabWords <= 3 * 64, "wave tile geometry");
+constexpr int kWLabLane = (kWLabWords + 63) / 64;  // label words a lane loads per plane
+static_assert(kWZ == 64 && kWLZ % 4 == 0 && kWY * kWZ <= 512 && kWY % 2 == 0, "wave tile geometry");
 
 struct WaveArgs {
     uint32_t tiles_z, wtiles_y, btiles_y, chunks, xc;  // wave tiles in z and y, block rows of tiles, x-chunks
@@ -1376,11 +1415,11 @@ struct WaveArgs {
 
 template <int S>
 struct WaveLds {
-    typename SlotVec<S>::T hop_slots[kWHop];  // listed records of the plane being hopped
-    uint16_t hop_rec[kWHop];                   // their record ids: (row + 1) * kWRZ + (column + 1)
+    typename HeadVec<S>::T hop_slots[kWHop];  // listed records of the plane being hopped (their heads)
+    uint16_t hop_rec[kWHop];                   // their record ids: (row + 1) * kWRZ + (column + 1); bit 15: full head
     uint32_t ev[3][kWEv];                      // arrival events per destination plane (X % 3): direction |
                                                // species << 3 | next event to the same voxel << 11
-    uint32_t head[3][kWY * kWZ];               // latest event into each interior voxel (kWEvEnd: none)
+    uint32_t head[3][kWY * kWZ / 2];           // latest event into each interior voxel (kWEvEnd: none), 16 bits each
     uint32_t ev_n[4];                          // events appended (may pass the cap: the plane overflowed)
     uint64_t arrive[3][kWY];                   // bit z of row r: an event targets voxel (r, z)
     uint32_t leave[2][kWY * kWZ / 2];          // 16-bit leave mask per interior voxel (X % 2)
@@ -1391,6 +1430,14 @@ struct WaveLds {
     uint32_t rb_xc[kWReact];                   // x | particle count << 24
     uint32_t rb_yz[kWReact];                   // and y << 16 | z (a circular list)
 };
+
+// Waits for this wave's outstanding vector-memory operations. A rare path that loads from global memory ends
+// with it, so that no load result is pending where the paths join: the wait the compiler would put there
+// instead runs on every path and, as loads complete in order, waits for the next plane's prefetch too.
+__device__ __forceinline__ void vm_drain()
+{
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0), expcnt and lgkmcnt left alone
+}
 
 __device__ __forceinline__ void wave_sync()
 {
@@ -1417,6 +1464,43 @@ template <int S>
 __device__ __forceinline__ typename SlotVec<S>::T load_vec(const uint8_t *base, uint32_t v)
 {
     return *reinterpret_cast<const typename SlotVec<S>::T *>(base + size_t(v) * uint32_t(S));
+}
+
+// The whole record of voxel v for the lanes in `m` (a rare case: more particles than the staged head). Read with
+// scalar loads one lane at a time: a vector load here would make the wave wait for every vector load issued
+// before it, the next plane's prefetch included (vector loads complete in order). The lattice is not written
+// during the step, so the scalar cache may hold it.
+template <int S>
+__device__ __forceinline__ void full_records(const uint8_t *src, uint32_t v, uint64_t m, Slots<S> &out)
+{
+    typedef __attribute__((address_space(4))) const uint32_t *cptr;
+    while (m != 0ull) {
+        const uint32_t ln = uint32_t(__ffsll((long long)m) - 1);
+        m &= m - 1ull;
+        const uint32_t vv = __builtin_amdgcn_readlane(v, ln);
+        const cptr p = (cptr)(src + size_t(vv) * uint32_t(S));
+        Slots<S> q;
+#pragma unroll
+        for (int k = 0; k < S / 4; ++k)
+            q.w[k] = p[k];
+        if (__lane_id() == ln)
+            out = q;
+    }
+}
+
+template <int S>
+__device__ __forceinline__ typename HeadVec<S>::T load_head(const uint8_t *base, uint32_t v)
+{
+    return *reinterpret_cast<const typename HeadVec<S>::T *>(base + size_t(v) * uint32_t(S));
+}
+
+template <int S>
+__device__ __forceinline__ bool head_nonzero(const typename HeadVec<S>::T &h)
+{
+    if constexpr (S == 16)
+        return (h.x | h.y) != 0u;
+    else
+        return vec_nonzero<S>(h);
 }
 
 // Bit 4s set for every bit s of a 16-bit mask.
@@ -1526,10 +1610,27 @@ __device__ __forceinline__ uint32_t assemble4(uint32_t own, uint32_t occ, uint32
     return o;
 }
 
+// Event-chain heads, 16 bits per voxel in 32-bit words: exchange (returns the previous head) and read.
+__device__ __forceinline__ uint32_t head_exchange(uint32_t *heads, uint32_t v, uint32_t idx)
+{
+    uint32_t *word = heads + (v >> 1);
+    const uint32_t sh = (v & 1u) * 16u;
+    uint32_t old = *word, assumed;
+    do {
+        assumed = old;
+        old = atomicCAS(word, assumed, (assumed & ~(0xFFFFu << sh)) | (idx << sh));
+    } while (old != assumed);
+    return (assumed >> sh) & 0xFFFFu;
+}
+__device__ __forceinline__ uint32_t head_read(const uint32_t *heads, uint32_t v)
+{
+    return (heads[v >> 1] >> ((v & 1u) * 16u)) & 0xFFFFu;
+}
+
 template <int S>
 struct WaveStaged {
-    typename SlotVec<S>::T rec[kWBatches];
-    uint32_t lab[3];
+    typename HeadVec<S>::T rec[kWBatches];
+    uint32_t lab[kWLabLane];
 };
 
 // One wave tile (tz, wy) of the lattice (a, w), marched through x-chunk `chunk`.
@@ -1538,6 +1639,7 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
                                            uint32_t tz, uint32_t wy, int xb, int xe, TileCounts &tc)
 {
     using Vec = typename SlotVec<S>::T;
+    using HVec = typename HeadVec<S>::T;
     const int l = int(__lane_id());
     const int y0 = int(wy) * kWY, z0 = int(tz) * kWZ;
     const int R0 = int(a.R0), R1 = int(a.R1), R2 = int(a.R2);
@@ -1566,10 +1668,10 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
             inl |= uint32_t(ok) << k;
     }
     const uint32_t zrec = uint32_t((zrow + 1) * kWRZ + (zcol + 1));
-    // label words of this lane: word i of the plane's 8 x 17-word grid is row i / 17, word i % 17
-    uint32_t loff[3];
+    // label words of this lane: word i of the plane's (kWY+4) x 17-word grid is row i / 17, word i % 17
+    uint32_t loff[kWLabLane];
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {
+    for (int i = 0; i < kWLabLane; ++i) {
         const int wd = min(l + 64 * i, kWLabWords - 1);
         loff[i] = uint32_t((y0 + wd / (kWLZ / 4)) * int(w.PZ4) + z0 / 4 + wd % (kWLZ / 4));
     }
@@ -1587,38 +1689,38 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
     const uint32_t xm_zero = a.mod_zero;
 
     // labels of planes xb-2 and xb-1; slots of plane xb-1 and labels of plane xb into the first register set
-    auto store_lab = [&](int X, const uint32_t (&r)[3]) {
+    auto store_lab = [&](int X, const uint32_t (&r)[kWLabLane]) {
         uint32_t *dst = L.lab[ring3(X)];
-        dst[l] = r[0];
-        dst[l + 64] = r[1];
-        if (l < kWLabWords - 128)
-            dst[l + 128] = r[2];
+#pragma unroll
+        for (int i = 0; i < kWLabLane; ++i)
+            if (i + 1 < kWLabLane || l + 64 * i < kWLabWords)
+                dst[l + 64 * i] = r[i];
     };
     WaveStaged<S> st0, st1;
     {
-        uint32_t r[3];
+        uint32_t r[kWLabLane];
 #pragma unroll
-        for (int i = 0; i < 3; ++i)
+        for (int i = 0; i < kWLabLane; ++i)
             r[i] = w.regp[lab_base(xb - 2) + loff[i]];
         store_lab(xb - 2, r);
 #pragma unroll
-        for (int i = 0; i < 3; ++i)
+        for (int i = 0; i < kWLabLane; ++i)
             r[i] = w.regp[lab_base(xb - 1) + loff[i]];
         store_lab(xb - 1, r);
     }
 #pragma unroll
     for (int k = 0; k < kWBatches; ++k)
-        st0.rec[k] = load_vec<S>(a.src, slot_base(xb - 1) + off[k]);
+        st0.rec[k] = load_head<S>(a.src, slot_base(xb - 1) + off[k]);
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
+    for (int i = 0; i < kWLabLane; ++i)
         st0.lab[i] = w.regp[lab_base(xb) + loff[i]];
     if (l < 3)
         L.ev_n[l] = 0u;
     if (l < 3 * kWY)
         L.arrive[l / kWY][l % kWY] = 0ull;
 #pragma unroll
-    for (int q = 0; q < 3 * kWY; ++q)
-        L.head[q / kWY][(q % kWY) * 64 + l] = kWEvEnd;
+    for (int q = 0; q < 3 * kWY / 2; ++q)
+        L.head[q / (kWY / 2)][(q % (kWY / 2)) * 64 + l] = kWEvEnd | (kWEvEnd << 16);
 
     uint32_t n_tight = 0, n_bad = 0, n_near = 0, n_fired = 0, n_rolled = 0;
     uint64_t occ_prev[kWY], occ_cur[kWY], inm[kWY];  // per interior row: occupied lanes of the plane
@@ -1628,18 +1730,19 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
         occ_prev[r] = occ_cur[r] = 0ull;
         inm[r] = __ballot((inl >> r) & 1u);
     }
-    Vec prev[kWY];           // interior slots of that plane: the gather reads no global memory, where a
+    HVec prev[kWY];          // interior slots of that plane: the gather reads no global memory, where a
                              // load would also wait for the prefetch issued before it (loads return in order)
 #pragma unroll
     for (int r = 0; r < kWY; ++r)
-        prev[r] = Vec{};
+        prev[r] = HVec{};
     // Reactions whose draw passes the bound test are rare in sparse tiles and common in dense ones; either way
     // only a few lanes of a gather pass have one. They wait in a circular list and run 64 at a time.
     uint32_t rb_head = 0u, rb_n = 0u;
     auto react_flush = [&](uint32_t k) __attribute__((always_inline)) {
         wave_sync();
         const bool act = uint32_t(l) < k;
-        const uint32_t idx = (rb_head + uint32_t(l)) & uint32_t(kWReact - 1);
+        uint32_t idx = rb_head + uint32_t(l);
+        idx -= idx >= uint32_t(kWReact) ? uint32_t(kWReact) : 0u;
         Slots<S> out = from_vec<S>(act ? L.rb_slots[idx] : Vec{});
         const uint32_t V = L.rb_v[idx], xc = L.rb_xc[idx], yz = L.rb_yz[idx];
         if (act) {
@@ -1651,7 +1754,8 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
             n_bad += bad;
             store_slots<S>(a.dst, V, out);
         }
-        rb_head = (rb_head + k) & uint32_t(kWReact - 1);
+        rb_head += k;
+        rb_head -= rb_head >= uint32_t(kWReact) ? uint32_t(kWReact) : 0u;
         rb_n -= k;
         wave_sync();
     };
@@ -1662,9 +1766,9 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
             const uint32_t sb = slot_base(X + 1), lb = lab_base(X + 2);
 #pragma unroll
             for (int k = 0; k < kWBatches; ++k)
-                fill.rec[k] = load_vec<S>(a.src, sb + off[k]);
+                fill.rec[k] = load_head<S>(a.src, sb + off[k]);
 #pragma unroll
-            for (int i = 0; i < 3; ++i)
+            for (int i = 0; i < kWLabLane; ++i)
                 fill.lab[i] = w.regp[lb + loff[i]];
         }
         store_lab(X + 1, use.lab);
@@ -1674,10 +1778,11 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
         if (l < kWY)
             L.arrive[ring3(X + 1)][l] = 0ull;
 #pragma unroll
-        for (int q = 0; q < kWY; ++q)
-            L.head[ring3(X + 1)][q * 64 + l] = kWEvEnd;
-        L.leave[X & 1][l] = 0u;
-        L.leave[X & 1][l + 64] = 0u;
+        for (int q = 0; q < kWY / 2; ++q)
+            L.head[ring3(X + 1)][q * 64 + l] = kWEvEnd | (kWEvEnd << 16);
+#pragma unroll
+        for (int q = 0; q < kWY / 2; ++q)
+            L.leave[X & 1][l + 64 * q] = 0u;
         wave_sync();
 
         // ---- T0: list the occupied records of plane X
@@ -1691,7 +1796,7 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
             for (int k = 0; k < kWBatches; ++k) {
                 if (k >= batches)
                     break;
-                const bool occ = ((okm >> k) & 1u) && vec_nonzero<S>(use.rec[k]);
+                const bool occ = ((okm >> k) & 1u) && head_nonzero<S>(use.rec[k]);
                 const uint64_t m = __ballot(occ);
                 const uint32_t rank =
                     __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
@@ -1702,7 +1807,7 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
                     else if (k == kWY + 1) rec = uint32_t((kWY + 1) * kWRZ + l + 1);
                     else rec = zrec;
                     L.hop_slots[n + rank] = use.rec[k];
-                    L.hop_rec[n + rank] = uint16_t(rec);
+                    L.hop_rec[n + rank] = uint16_t(rec | (head_full<S>(use.rec[k]) ? 0x8000u : 0u));
                 }
                 n += uint32_t(__popcll(m));
                 if (k < kWY)
@@ -1720,10 +1825,18 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
             for (uint32_t base = 0; base < n; base += 64u) {
                 const uint32_t i = base + uint32_t(l);
                 const bool act = i < n;
-                const uint32_t rec = act ? L.hop_rec[i] : uint32_t(kWRZ + 1);
-                const Slots<S> sl = from_vec<S>(act ? L.hop_slots[i] : Vec{});
+                const uint32_t rcode = act ? L.hop_rec[i] : uint32_t(kWRZ + 1);
+                const uint32_t rec = rcode & 0x7FFFu;
+                Slots<S> sl = from_head<S>(act ? L.hop_slots[i] : HVec{});
                 const int rho = int(rec / kWRZ) - 1, zeta = int(rec % kWRZ) - 1;
                 const uint32_t Y = uint32_t(y0 + rho), Z = uint32_t(z0 + zeta);
+                if constexpr (S > kHeadSlots<S>) {
+                    const uint64_t fm = __ballot((rcode & 0x8000u) != 0u);  // full heads: whole records (rare)
+                    if (fm != 0ull) {
+                        full_records<S>(a.src, Z + __umul24(Y, a.R2) + uint32_t(X) * a.R12, fm, sl);
+                        vm_drain();
+                    }
+                }
                 const int gi = (rho + 2) * kWLZ + (zeta + 2);
                 Hood h;
                 h.src_reg = lcur[gi];
@@ -1774,7 +1887,7 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
                         if (mine) {
                             const uint32_t idx = cnt0 + rank;
                             if (idx < w.ev_cap) {
-                                const uint32_t prev = atomicExch(&L.head[rg3][dest], idx);
+                                const uint32_t prev = head_exchange(L.head[rg3], dest, idx);
                                 L.ev[rg3][idx] = d | (sp << 3) | (prev << 11);
                             }
                             atomicOr(reinterpret_cast<unsigned long long *>(&L.arrive[rg3][r2]),
@@ -1816,8 +1929,9 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
                 sidx[r] = ns + rank;
                 needm |= uint32_t(need) << r;
                 if (need) {
-                    L.slow[ns + rank] = uint16_t(uint32_t(r * kWZ + l) | (uint32_t(occ) << 8));
-                    L.hop_slots[ns + rank] = occ ? prev[r] : Vec{};  // the hop list is done with
+                    const bool full = occ && head_full<S>(prev[r]);
+                    L.slow[ns + rank] = uint16_t(uint32_t(r * kWZ + l) | (uint32_t(full) << 12));
+                    L.hop_slots[ns + rank] = occ ? prev[r] : HVec{};  // the hop list is done with
                 }
                 ns += uint32_t(__popcll(m));
             }
@@ -1827,11 +1941,18 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
                 const uint32_t i = base + uint32_t(l);
                 const bool act = i < ns;
                 const uint32_t code = act ? L.slow[i] : 0u;
-                const uint32_t me = code & 0xFFu;
+                const uint32_t me = code & 0xFFFu;
                 const int r = int(me) / kWZ, z = int(me) % kWZ;
                 const uint32_t Y = uint32_t(y0 + r), Z = uint32_t(z0 + z);
                 const uint32_t V = Z + __umul24(Y, a.R2) + uint32_t(Q) * a.R12;
-                const Slots<S> own = from_vec<S>(act ? L.hop_slots[i] : Vec{});
+                Slots<S> own = from_head<S>(act ? L.hop_slots[i] : HVec{});
+                if constexpr (S > kHeadSlots<S>) {
+                    const uint64_t fm = __ballot(act && (code & 0x1000u) != 0u);  // full heads (rare)
+                    if (fm != 0ull) {
+                        full_records<S>(a.src, V, fm, own);
+                        vm_drain();
+                    }
+                }
                 const uint32_t occ = count_nonzero(own);
                 const uint32_t lv = act ? lvq[me] : 0u;
                 const uint64_t stay = first_nibbles(occ) & ~spread_nibbles(lv);
@@ -1844,7 +1965,7 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
                     if (!over && __ballot(occ > 4u) == 0ull) {
                         uint32_t arr = 0u, tot = 0u;
                         uint64_t cnts = 0ull;
-                        uint32_t e = act ? L.head[rq][me] : kWEvEnd;
+                        uint32_t e = act ? head_read(L.head[rq], me) : kWEvEnd;
                         while (__ballot(e != kWEvEnd) != 0ull) {
                             if (e != kWEvEnd) {
                                 const uint32_t c = L.ev[rq][e];
@@ -1875,7 +1996,7 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
                         // in that order = nibble d of 0x231405).
                         BV arr{0ull, 0ull};
                         uint64_t cnts = 0ull;  // arrivals per rank, one byte each
-                        uint32_t e = act ? L.head[rq][me] : kWEvEnd;
+                        uint32_t e = act ? head_read(L.head[rq], me) : kWEvEnd;
                         while (__ballot(e != kWEvEnd) != 0ull) {
                             if (e != kWEvEnd) {
                                 const uint32_t c = L.ev[rq][e];
@@ -1893,8 +2014,10 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
                         const BV lom = bytes_below(nlo);
                         lo = from_bv<S>(BV{arr.lo & lom.lo, arr.hi & lom.hi});
                         hi = from_bv<S>(shr_bytes(arr, nlo));
-                    } else if (act) {
-                        gather_arrivals_rolled<S>(a, tb, Q, Y, Z, V, lo, nlo, hi, nhi);
+                    } else {
+                        if (act)
+                            gather_arrivals_rolled<S>(a, tb, Q, Y, Z, V, lo, nlo, hi, nhi);
+                        vm_drain();  // the global loads above complete here, not at the join below
                     }
                     if (act && occ + nlo + nhi > uint32_t(S)) {
                         ++n_tight;
@@ -1906,24 +2029,29 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
                 // the result goes back to the list entry; the voxel's own lane stores whole rows below, so no
                 // output line is written in pieces at different times (a reaction still to run is stored
                 // by its batch instead, flagged here)
+                // an output with more particles than a head holds is stored here (rare)
+                const bool wide = act && !may && cnt > uint32_t(kHeadSlots<S>);
+                if (wide)
+                    store_slots<S>(a.dst, V, out);
                 if (act) {
-                    L.hop_slots[i] = to_vec<S>(out);
-                    L.slow[i] = uint16_t(may ? 1u : 0u);
+                    L.hop_slots[i] = to_head<S>(out);
+                    L.slow[i] = uint16_t((may || wide) ? 1u : 0u);
                 }
                 const uint64_t mr = __ballot(may);
                 if (mr != 0ull) {
                     const uint32_t rank =
                         __builtin_amdgcn_mbcnt_hi(uint32_t(mr >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mr), 0u));
                     if (may) {
-                        const uint32_t idx = (rb_head + rb_n + rank) & uint32_t(kWReact - 1);
+                        uint32_t idx = rb_head + rb_n + rank;
+                        idx -= idx >= uint32_t(kWReact) ? uint32_t(kWReact) : 0u;
                         L.rb_slots[idx] = to_vec<S>(out);
                         L.rb_v[idx] = V;
                         L.rb_xc[idx] = uint32_t(Q) | (cnt << 24);
                         L.rb_yz[idx] = (Y << 16) | Z;
                     }
                     rb_n += uint32_t(__popcll(mr));
-                    if (rb_n >= 64u)
-                        react_flush(64u);
+                    while (rb_n >= uint32_t(kWReactRun))
+                        react_flush(min(rb_n, 64u));
                 }
             }
             wave_sync();
@@ -1931,7 +2059,7 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
             for (int r = 0; r < kWY; ++r) {
                 if ((inl >> r) & 1u) {
                     const bool need = (needm >> r) & 1u;
-                    const Vec v = need ? L.hop_slots[sidx[r]] : Vec{};
+                    const Vec v = to_vec<S>(from_head<S>(need ? L.hop_slots[sidx[r]] : HVec{}));
                     const bool deferred = need && L.slow[sidx[r]] != 0u;
                     if (!deferred)
                         *reinterpret_cast<Vec *>(a.dst + size_t(off[r] + uint32_t(Q) * a.R12) * uint32_t(S)) = v;
@@ -1974,7 +2102,8 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
 // first and the short ones fill in behind them. The queue is split per XCD (XCD k takes the groups of
 // rank k mod 8, an equal share of the heavy ones); an XCD whose queue is empty helps the others.
 template <int S>
-__global__ __launch_bounds__(kWaveBlock) void rdme_step_wave(StepArgs a, WaveArgs w)
+__global__ __launch_bounds__(kWaveBlock) __attribute__((amdgpu_waves_per_eu(S == 16 ? 3 : 2)))
+void rdme_step_wave(StepArgs a, WaveArgs w)
 {
     __shared__ WaveLds<S> L[kWavesPerBlock];
     extern __shared__ uint32_t tab_lds[];

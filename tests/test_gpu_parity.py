@@ -212,6 +212,27 @@ def test_dense_capacity_replay(oracle_mod, kernel):
         print(f"compared {compared} steps; oracle {orc.counters.as_dict()}; tight {int(c[CS_CNT_TIGHT])}")
 
 
+def test_dense_s16_full_heads(oracle_mod):
+    """S=16 at about 7 particles per voxel: many voxels hold 8 or more, past the 8-slot head the wave kernel
+    stages, so their hops and gathers take the whole record from memory. Step by step against the oracle, up to
+    and including the first slot-S alias (rdme.wgsl:30-40)."""
+    announce("dense S=16 16^3 density 7 (full 8-slot heads), step by step")
+    sim = _dense_model(16, 16, 7.0, seed=17)
+    orc = oracle_mod.Oracle.from_simulation(sim)
+    sim.prepare_for_gpu(0)
+    lat = sim.lattice()
+    assert (np.count_nonzero(lat, axis=-1) >= 8).sum() > 100, "the case must hold voxels past the head"
+    compared = 0
+    for t, it in enumerate(default_itime_schedule(17, 12)):
+        sim.step(t, int(it), 0)
+        orc.step(int(it))
+        _compare(sim, orc, f"dense S=16 step {t}")
+        compared += 1
+        if orc.counters.alias_writes:
+            break
+    print(f"compared {compared} steps; oracle {orc.counters.as_dict()}")
+
+
 def _walled_model(edge, slots, density, seed):
     """Every voxel's slot 0 holds an immobile, unreactive species, so a claim into a full voxel finds the
     next voxel's slot 0 taken and is rejected (never the slot-S alias)."""
