@@ -5,6 +5,7 @@ mirror of the reference's host API.
 """
 from ._lib import CellSimError, lib  # noqa: F401
 from .ensemble import Ensemble  # noqa: F401
+from .stats import StatisticContainer, collect_stats, make_all_stats  # noqa: F401
 from .simulation import (  # noqa: F401
     LatticeParams,
     RegionType,
