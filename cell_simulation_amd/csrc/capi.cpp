@@ -593,6 +593,80 @@ int cs_sim_read_reservoir(const cs_sim *sim, uint32_t *out, size_t n)
     });
 }
 
+int cs_sim_read_concentrations(cs_sim *sim, int32_t *out, size_t n)
+{
+    return guard([&] {
+        need(out, "out");
+        cs::Model &m = model_of(sim);
+        const uint32_t n1 = m.num_species() + 1u, S = m.slots();
+        const uint64_t N = m.n_voxels();
+        if (n != N * n1)
+            throw cs::Error(CS_ERR_ARG, "concentration buffer must hold N*(ns+1) entries");
+        std::vector<uint8_t> lat(N * S);
+        std::vector<uint32_t> vox;
+        std::vector<int32_t> rs;
+        int32_t c0 = 0;
+        if (sim->device) {
+            sim->device->read_lattice(lat.data());
+            sim->device->read_reservoir_state(vox, rs);
+            c0 = sim->device->steps() > 0u ? 1 : 0;  // every step sets C[v][0] = 1 (cme.wgsl:31)
+        } else {
+            std::memcpy(lat.data(), m.lattice().data(), lat.size());
+        }
+        std::memset(out, 0, n * sizeof(int32_t));
+        for (uint64_t v = 0; v < N; ++v) {
+            int32_t *c = out + v * n1;
+            c[0] = c0;
+            for (uint32_t k = 0; k < S; ++k)
+                if (lat[v * S + k])
+                    c[lat[v * S + k]] += 1;
+        }
+        if (sim->device) {
+            for (size_t i = 0; i < vox.size(); ++i) {
+                const uint32_t sp = uint32_t(rs[4 * i]), K = sp & 0xFFu, R = sp >> 8;
+                if (K)
+                    out[uint64_t(vox[i]) * n1 + K] += rs[4 * i + 2];
+                if (R)
+                    out[uint64_t(vox[i]) * n1 + R] += rs[4 * i + 1];
+            }
+        } else {
+            const auto &res = m.reservoir();
+            for (uint64_t v = 0; v < N; ++v)
+                if (res[v])
+                    out[v * n1 + res[v]] += 1;  // lattice.rs:297
+        }
+    });
+}
+
+int cs_sim_read_occupancy(cs_sim *sim, uint32_t *out, size_t n)
+{
+    return guard([&] {
+        need(out, "out");
+        cs::Model &m = model_of(sim);
+        const uint32_t S = m.slots();
+        const uint64_t N = m.n_voxels();
+        if (n != N)
+            throw cs::Error(CS_ERR_ARG, "occupancy buffer must hold N entries");
+        std::vector<uint8_t> lat(N * S);
+        std::vector<uint32_t> vox;
+        std::vector<int32_t> rs;
+        if (sim->device) {
+            sim->device->read_lattice(lat.data());
+            sim->device->read_reservoir_state(vox, rs);
+        } else {
+            std::memcpy(lat.data(), m.lattice().data(), lat.size());
+        }
+        for (uint64_t v = 0; v < N; ++v) {
+            uint32_t c = 0;
+            for (uint32_t k = 0; k < S; ++k)
+                c += lat[v * S + k] != 0u;
+            out[v] = c;
+        }
+        for (size_t i = 0; i < vox.size(); ++i)
+            out[vox[i]] += uint32_t(rs[4 * i + 3]);  // u32 wrap-around, as the reference's atomics
+    });
+}
+
 int cs_sim_read_species_counts(cs_sim *sim, int32_t *out, size_t n)
 {
     return guard([&] {
@@ -618,12 +692,15 @@ int cs_sim_read_region_counts(cs_sim *sim, int32_t *out, size_t n)
         } else {
             std::memset(out, 0, n * sizeof(int32_t));
             const uint32_t S = m.slots();
-            for (uint64_t v = 0; v < m.n_voxels(); ++v)
+            for (uint64_t v = 0; v < m.n_voxels(); ++v) {
                 for (uint32_t k = 0; k < S; ++k) {
                     const uint8_t sp = m.lattice()[v * S + k];
                     if (sp)
                         out[size_t(m.regions()[v]) * n1 + sp] += 1;
                 }
+                if (m.reservoir()[v])
+                    out[size_t(m.regions()[v]) * n1 + m.reservoir()[v]] += 1;  // lattice.rs:297
+            }
         }
     });
 }

@@ -46,6 +46,18 @@ void hip_check(hipError_t e, const char *what)
         throw Error(CS_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+// One reservoir voxel (lattice.rs:292-300, cme.wgsl:76-110). The reference's per-voxel counts C[v][sp]
+// equal the slot contents everywhere except at voxels touched by reservoirs: a reservoir site v of species
+// R holds one unit of C[v][R] that no slot holds, and the reaction at v skips the species K =
+// reservoirs[v*S] (indexed by the lattice index v*S; a read past the end reads 0). There C - slots is kept
+// for two species, and the occupancy word's offset from the slot count.
+struct ResVoxel {
+    uint32_t sp;  // K | R << 8; R = 0 when R == K (its unit is counted in gk)
+    int32_t gr;   // C[v][R] - slots holding R
+    int32_t gk;   // C[v][K] - slots holding K
+    int32_t off;  // occupancy[v] - occupied slots (u32 arithmetic in the reference)
+};
+
 struct StepArgs {
     const uint8_t *src;
     uint8_t *dst;
@@ -71,7 +83,17 @@ struct StepArgs {
     uint32_t itime;
     uint32_t mod_zero;
     float tau;
+    const uint32_t *res_bits;  // [N/32] reservoir voxels (their reaction runs in reservoir_react_kernel), or null
+    uint32_t *res_skew;        // [N/32] reservoir voxels whose occupancy word differs from the slot count
+    const uint32_t *res_v;     // [res_n] reservoir voxel indices, ascending
+    ResVoxel *res_st;          // [res_n]
+    uint32_t res_n;
 };
+
+__device__ __forceinline__ bool voxel_bit(const uint32_t *bits, uint32_t v)
+{
+    return (bits[v >> 5] >> (v & 31u)) & 1u;
+}
 
 // ---- slot words: S one-byte slots packed little-endian into S/4 u32 ----------------------------
 template <int S>
@@ -342,6 +364,8 @@ __device__ __forceinline__ void react(const StepArgs &a, const Tab &tb, Slots<S>
     // above the bound's reject line is above the exact one below as well: nothing fires, nothing is near.
     if (r1 > tb.bound[cnt] * a.tau * 1.001f + 1e-6f)
         return;
+    if (a.res_bits != nullptr && voxel_bit(a.res_bits, z + y * a.R2 + x * a.R12))
+        return;  // a reservoir voxel: its reaction runs in reservoir_react_kernel after the step
     const uint32_t words = wave_words<S>(cnt);  // slot words holding particles in any lane still here
 
     uint64_t cand[kMaxRW];
@@ -572,7 +596,10 @@ __device__ __forceinline__ Slots<S> gather_voxel(const StepArgs &a, const Tab &t
     if (z + 1u < a.R2) nhi = gather_from<S>(a, tb, x, y, z + 1u, v + 1u, 4u, hi, nhi);  // from +z, hopped -z
     if (y + 1u < a.R1) nhi = gather_from<S>(a, tb, x, y + 1u, z, v + a.R2, 2u, hi, nhi);  // from +y, hopped -y
     if (x + 1u < a.R0) nhi = gather_from<S>(a, tb, x + 1u, y, z, v + a.R12, 0u, hi, nhi);  // from +x, hopped -x
-    f.tight = occ + nlo + nhi > uint32_t(S);
+    // a reservoir voxel whose occupancy word is off the slot count can refuse a claim that fits (rdme.wgsl:62)
+    // and walks that many slots (rdme.wgsl:213): replayed like a tight one
+    f.tight = occ + nlo + nhi > uint32_t(S) ||
+              (a.res_skew != nullptr && occ + nlo + nhi != 0u && voxel_bit(a.res_skew, v));
     uint32_t cnt = 0;
     Slots<S> out = assemble<S>(own, occ, stay, lo, nlo, hi, nhi, cnt);
     react<S>(a, tb, out, cnt, x, y, z, f.fired, f.rolled, f.near, f.bad);
@@ -647,6 +674,15 @@ __device__ __forceinline__ int find_sorted(const uint32_t *arr, uint32_t n, uint
     return (lo < n && arr[lo] == v) ? int(lo) : -1;
 }
 
+// Offset of the reference's occupancy word from the slot count of voxel v (non-zero at reservoir voxels only).
+__device__ __forceinline__ int32_t res_offset(const StepArgs &a, uint32_t v)
+{
+    if (a.res_n == 0u)
+        return 0;
+    const int i = find_sorted(a.res_v, a.res_n, v);
+    return i < 0 ? 0 : a.res_st[i].off;
+}
+
 __device__ __forceinline__ void counter_add(unsigned long long *c, int idx, long long d)
 {
     if (d)
@@ -695,13 +731,25 @@ __device__ __noinline__ void fixup_tight(const StepArgs &a, uint32_t nt_all)
         const Slots<S> s = load_slots<S>(a.src, F[k]);
         for (int q = 0; q < S; ++q)
             Lr[size_t(k) * S + q] = uint8_t(get_byte(s, q));
-        Or[k] = count_nonzero(s);
+        Or[k] = count_nonzero(s) + uint32_t(res_offset(a, F[k]));
     }
     // canonical replay of the sources in W
     for (uint32_t wi = 0; wi < nW; ++wi) {
         const uint32_t w = W[wi];
         const Slots<S> sw = load_slots<S>(a.src, w);
-        const uint32_t occ = count_nonzero(sw);
+        uint32_t occ = count_nonzero(sw);
+        if (const int32_t offw = res_offset(a, w)) {
+            // The sweep walks as many slots as the occupancy word says (rdme.wgsl:213), with the reference's
+            // u32 end index: below the occupied count the last particles stay put; an end at or below the
+            // start walks nothing. Past slot S it would walk the next voxel's particles as this voxel's,
+            // which the byte-slot state cannot follow (flagged).
+            const uint32_t base = w * uint32_t(S), end = base + occ + uint32_t(offw);
+            const uint32_t n = end > base ? end - base : 0u;
+            if (n > uint32_t(S) &&
+                (n > 2u * uint32_t(S) || (w + 1u < a.N && count_nonzero(load_slots<S>(a.src, w + 1u)) != 0u)))
+                d_bad += 1;
+            occ = min(occ, n);
+        }
         if (!occ)
             continue;
         const int wF = find_sorted(F, nF, w);
@@ -790,6 +838,120 @@ __global__ __launch_bounds__(64) void tight_replay_kernel(StepArgs a)
         return;
     fixup_tight<S>(a, a.fix_ctl[0]);
     a.fix_ctl[0] = 0u;
+}
+
+// ---- reservoir voxels: the reaction with the reference's per-voxel counts (cme.wgsl:25-120) ----------
+// C[v][sp] = slots holding sp + the ghost units of ResVoxel (species R and K only).
+template <int S>
+__device__ __forceinline__ int32_t res_count(const Slots<S> &s, uint32_t sp, const ResVoxel &e)
+{
+    if (sp == 0u)
+        return 1;  // C[v][0] = 1 (cme.wgsl:31)
+    int32_t c = int32_t(count_species(s, sp));
+    if (sp == (e.sp & 0xFFu))
+        c += e.gk;
+    if (sp == (e.sp >> 8))
+        c += e.gr;
+    return c;
+}
+
+// k_r * f32(C[i0] * C[i1] * C[i2]) with the i32 product of cme.wgsl:41-45 (wrapping, signed).
+template <int S>
+__device__ __forceinline__ float res_propensity(const StepArgs &a, const Slots<S> &s, const ResVoxel &e, uint32_t r)
+{
+    const uint32_t ix = reinterpret_cast<const uint32_t *>(a.ridx)[r];
+    const uint32_t p = uint32_t(res_count<S>(s, ix & 0xFFu, e)) * uint32_t(res_count<S>(s, (ix >> 8) & 0xFFu, e)) *
+                       uint32_t(res_count<S>(s, (ix >> 16) & 0xFFu, e));
+    return a.rates[r] * float(int32_t(p));
+}
+
+// Launched after tight_replay_kernel when the model has reservoir voxels: one lane per reservoir voxel runs
+// cme.wgsl:25-120 on the compacted post-diffusion slots plus the ghost units (the step kernels skipped
+// these voxels), then keeps C - slots and the occupancy offset for the next step:
+//  * fired, kept: every species but K is rewritten as C copies, so its ghost is 0 (R keeps a negative C,
+//    which writes no slot); K is skipped, so its slots vanish and C[K] stays: ghost K = old C[K];
+//    occupancy += sum of the stoichiometry over the species but K (cme.wgsl:78-84, 114);
+//  * rolled back (cme.wgsl:95-108): slots and occupancy stay; C[q] -= stoich[q] for every q up to the
+//    species being written, K included although it was never added: ghost K -= stoich[K] then.
+template <int S>
+__global__ __launch_bounds__(64) void reservoir_react_kernel(StepArgs a)
+{
+    const uint32_t i = blockIdx.x * 64u + threadIdx.x;
+    bool fired = false, rolled = false, near = false, bad = false;
+    if (i < a.res_n) {
+        const uint32_t v = a.res_v[i];
+        ResVoxel e = a.res_st[i];
+        const uint32_t K = e.sp & 0xFFu, R = e.sp >> 8;
+        const uint32_t z = v % a.R2, y = (v / a.R2) % a.R1, x = v / a.R12;
+        const Slots<S> s = load_slots<S>(a.dst, v);
+        float tot = 0.0f;
+        for (uint32_t r = 1; r <= a.nr; ++r)
+            tot = tot + res_propensity<S>(a, s, e, r);
+        const uint32_t P = pcg_hash(a.itime + x * y + z);
+        const float r1 = uniform_float(P);
+        const float xt = tot * a.tau;
+        const float thr = 1.0f - float(exp(double(-xt)));
+        near = xt != 0.0f && fabsf(r1 - thr) <= 5.9604644775390625e-08f;  // exp(-0) = 1 exactly
+        if (r1 <= thr) {
+            fired = true;
+            // first q with r2 <= cum[q] / tot, q < nr (cme.wgsl:50-52, 68-73)
+            const float r2 = uniform_float(P + 1u);
+            uint32_t q = 0;
+            float cum = 0.0f;
+            while (q < a.nr && r2 > cum / tot) {
+                ++q;
+                cum = cum + res_propensity<S>(a, s, e, q);
+            }
+            const int8_t *row = a.stoich + size_t(q) * (a.ns + 1u);
+            Slots<S> out = zero_slots<S>();
+            uint32_t j = 0, sp_stop = 0;
+            int32_t docc = 0, gr = 0;
+            bool neg = false;
+            for (uint32_t sp = 1; sp <= a.ns; ++sp) {
+                if (sp == K)
+                    continue;  // reservoirs[v*S] == sp (cme.wgsl:80-83)
+                const int32_t c = res_count<S>(s, sp, e) + int32_t(row[sp]);
+                docc += int32_t(row[sp]);
+                if (c > 0) {
+                    if (j + uint32_t(c) >= uint32_t(S)) {
+                        sp_stop = sp;
+                        break;
+                    }
+                    for (int32_t t = 0; t < c; ++t)
+                        put_byte_dyn(out, j++, sp);
+                } else if (c < 0) {
+                    if (sp == R)
+                        gr = c;
+                    else
+                        neg = true;  // a negative count of a third species has no place to live
+                }
+            }
+            if (sp_stop != 0u) {
+                rolled = true;
+                if (K != 0u && K <= sp_stop)
+                    e.gk -= int32_t(row[K]);
+            } else {
+                bad = neg;
+                const int32_t occ = int32_t(count_nonzero(s)) + e.off;
+                if (K != 0u)
+                    e.gk = res_count<S>(s, K, e);
+                if (R != 0u)
+                    e.gr = gr;
+                e.off = occ + docc - int32_t(j);
+                store_slots<S>(a.dst, v, out);
+            }
+            a.res_st[i] = e;
+            const uint32_t bit = 1u << (v & 31u);
+            if (e.off != 0)
+                atomicOr(a.res_skew + (v >> 5), bit);
+            else
+                atomicAnd(a.res_skew + (v >> 5), ~bit);
+        }
+    }
+    count_event(a.counters, CS_CNT_EXCEPTION, bad);
+    count_event(a.counters, CS_CNT_NEAR_THRESHOLD, near);
+    count_event(a.counters, CS_CNT_FIRED, fired);
+    count_event(a.counters, CS_CNT_ROLLBACK, rolled);
 }
 
 template <int S>
@@ -1302,7 +1464,8 @@ __device__ __forceinline__ void tile_march(TileLds<S> &L, uint32_t *tab_lds, con
                 nhi = take_arrivals<S>(sy1, sel(4u, cy1, 2u), hi, nhi);   // from +y, hopped -y
                 nhi = take_arrivals<S>(sx1, sel(1u, cx1, 0u), hi, nhi);   // from +x, hopped -x
                 const uint32_t V = gz + gy * a.R2 + uint32_t(x) * a.R12;
-                if (occ + nlo + nhi > uint32_t(S)) {
+                if (occ + nlo + nhi > uint32_t(S) ||
+                    (a.res_skew != nullptr && occ + nlo + nhi != 0u && voxel_bit(a.res_skew, V))) {
                     ++n_tight;
                     const uint32_t k = atomicAdd(&L.n_tight, 1u);
                     if (k < uint32_t(kTightLds))
@@ -1958,9 +2121,9 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
                 const uint64_t stay = first_nibbles(occ) & ~spread_nibbles(lv);
                 // Most voxels end with at most four particles: then the arrivals and the assembled slots fit
                 // one 32-bit word, and the wave takes that path when every lane fits it.
-                uint32_t cnt = 0u;
+                uint32_t cnt = 0u, narr = 0u;
                 Slots<S> out = zero_slots<S>();
-                bool done = false;
+                bool done = false, tight = false;
                 if constexpr (S > 4) {
                     if (!over && __ballot(occ > 4u) == 0ull) {
                         uint32_t arr = 0u, tot = 0u;
@@ -1982,6 +2145,7 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
                             const uint32_t nlo4 = uint32_t((cnts * 0x010101ull) >> 16) & 0xFFu;
                             out.w[0] = assemble4(own.w[0], occ, occ ? uint32_t(stay) : 0u, arr & b4_below(nlo4),
                                                  nlo4, b4_shr(arr, nlo4), tot - nlo4, cnt);
+                            narr = tot;
                             done = true;
                         }
                     }
@@ -2019,12 +2183,18 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
                             gather_arrivals_rolled<S>(a, tb, Q, Y, Z, V, lo, nlo, hi, nhi);
                         vm_drain();  // the global loads above complete here, not at the join below
                     }
-                    if (act && occ + nlo + nhi > uint32_t(S)) {
+                    tight = occ + nlo + nhi > uint32_t(S);
+                    if (act && tight) {
                         ++n_tight;
                         note_tight(a, V);
                     }
+                    narr = nlo + nhi;
                     out = assemble<S>(own, occ, occ ? stay : 0ull, lo, min(nlo, 16u), hi, min(nhi, 16u), cnt);
                 }
+                // a reservoir voxel whose occupancy word is off the slot count can refuse a claim that fits
+                // (rdme.wgsl:62) and walks that many slots (rdme.wgsl:213): replayed like a tight one
+                if (a.res_skew != nullptr && act && !tight && occ + narr != 0u && voxel_bit(a.res_skew, V))
+                    note_tight(a, V);
                 const bool may = act && react_may_fire<S>(a, tb, cnt, uint32_t(Q), Y, Z);
                 // the result goes back to the list entry; the voxel's own lane stores whole rows below, so no
                 // output line is written in pieces at different times (a reaction still to run is stored
@@ -2391,6 +2561,11 @@ __device__ __forceinline__ StepArgs cell_step_args(const EnsArgs &e, const EnsCe
     a.itime = itime;
     a.mod_zero = c.mod_zero;
     a.tau = c.tau;
+    a.res_bits = nullptr;  // ensembles take no reservoir models
+    a.res_skew = nullptr;
+    a.res_v = nullptr;
+    a.res_st = nullptr;
+    a.res_n = 0u;
     return a;
 }
 
@@ -2614,8 +2789,8 @@ static std::vector<float> propensity_bounds(const Model &m, uint32_t S)
 // Checks that the model fits the device encodings (one-byte labels and slots, int8 stoichiometry).
 static void check_device_model(const Model &m)
 {
-    if (m.has_reservoir())
-        throw Error(CS_ERR_UNSUPPORTED, "reservoir particles are not supported by the HIP path");
+    if (m.has_reservoir() && m.num_cme_species() > 0u)
+        throw Error(CS_ERR_UNSUPPORTED, "reservoir particles together with well-mixed species are not supported");
     if (m.num_regions() > 255)
         throw Error(CS_ERR_UNSUPPORTED, "at most 255 regions fit the one-byte label");
     if (m.num_reactions() > 255)
@@ -2904,6 +3079,47 @@ Device::Device(const Model &m, int hip_device) : dev_(hip_device)
                   "upload cme tables");
     }
 
+    // Reservoir voxels: sites of a reservoir species, and voxels v whose reservoirs[v*S] names one
+    // (cme.wgsl:80 indexes the reservoir buffer with the lattice index).
+    std::vector<ResVoxel> rst;
+    std::vector<uint32_t> rbits;
+    if (m.has_reservoir()) {
+        const std::vector<uint32_t> &rs = m.reservoir();
+        rbits.assign((N_ + 31u) / 32u, 0u);
+        for (uint64_t v = 0; v < N_; ++v) {
+            const uint32_t R = rs[v];
+            const uint64_t vk = v * S_;
+            const uint32_t K = vk < N_ ? rs[vk] : 0u;
+            if (R == 0u && K == 0u)
+                continue;
+            if (R > 255u || K > 255u)
+                throw Error(CS_ERR_UNSUPPORTED, "reservoir species index beyond one byte");
+            ResVoxel e;
+            e.sp = K | ((R == K ? 0u : R) << 8);
+            e.gr = (R != 0u && R != K) ? 1 : 0;  // the reservoir unit (lattice.rs:297)
+            e.gk = (R != 0u && R == K) ? 1 : 0;
+            e.off = 0;
+            res_vh_.push_back(uint32_t(v));
+            res_reg_.push_back(reg8[v]);
+            rst.push_back(e);
+            rbits[v >> 5] |= 1u << (v & 31u);
+        }
+        res_n_ = uint32_t(res_vh_.size());
+    }
+    if (res_n_ > 0u) {
+        const size_t nb = rbits.size() * sizeof(uint32_t);
+        hip_check(hipMalloc(&res_bits_, nb), "hipMalloc reservoir bits");
+        hip_check(hipMalloc(&res_skew_, nb), "hipMalloc reservoir skew bits");
+        hip_check(hipMalloc(&res_v_, res_n_ * sizeof(uint32_t)), "hipMalloc reservoir voxels");
+        hip_check(hipMalloc(&res_st_, res_n_ * sizeof(ResVoxel)), "hipMalloc reservoir state");
+        hip_check(hipMemcpyAsync(res_bits_, rbits.data(), nb, hipMemcpyHostToDevice, st), "upload reservoir bits");
+        hip_check(hipMemsetAsync(res_skew_, 0, nb, st), "clear reservoir skew bits");
+        hip_check(hipMemcpyAsync(res_v_, res_vh_.data(), res_n_ * sizeof(uint32_t), hipMemcpyHostToDevice, st),
+                  "upload reservoir voxels");
+        hip_check(hipMemcpyAsync(res_st_, rst.data(), res_n_ * sizeof(ResVoxel), hipMemcpyHostToDevice, st),
+                  "upload reservoir state");
+    }
+
     hip_check(hipMemcpyAsync(lat_[0], m.lattice().data(), N_ * S_, hipMemcpyHostToDevice, st), "upload lattice");
     plan_wave_items(m.lattice().data());
     hip_check(hipMemcpyAsync(reg_, reg8.data(), N_, hipMemcpyHostToDevice, st), "upload regions");
@@ -2936,6 +3152,10 @@ Device::~Device()
     hipFree(cme_counts_);
     hipFree(cme_tab_);
     hipFree(audit_);
+    hipFree(res_bits_);
+    hipFree(res_skew_);
+    hipFree(res_v_);
+    hipFree(res_st_);
     if (stream_)
         hipStreamDestroy(static_cast<hipStream_t>(stream_));
 }
@@ -2973,6 +3193,11 @@ void Device::step(uint32_t itime)
     a.itime = itime;
     a.mod_zero = mod_zero_;
     a.tau = tau_;
+    a.res_bits = res_bits_;
+    a.res_skew = res_skew_;
+    a.res_v = res_v_;
+    a.res_st = static_cast<ResVoxel *>(res_st_);
+    a.res_n = res_n_;
     hipStream_t st = static_cast<hipStream_t>(stream_);
     if (has_cme_) {
         const uint8_t *cb = static_cast<const uint8_t *>(cme_tab_);
@@ -3068,7 +3293,30 @@ void Device::step(uint32_t itime)
     default: hipLaunchKernelGGL(tight_replay_kernel<16>, dim3(1), dim3(64), 0, st, a); break;
     }
     hip_check(hipGetLastError(), "tight replay launch");
+    if (res_n_ > 0u) {
+        const uint32_t blocks = (res_n_ + 63u) / 64u;
+        switch (S_) {
+        case 4: hipLaunchKernelGGL(reservoir_react_kernel<4>, dim3(blocks), dim3(64), 0, st, a); break;
+        case 8: hipLaunchKernelGGL(reservoir_react_kernel<8>, dim3(blocks), dim3(64), 0, st, a); break;
+        default: hipLaunchKernelGGL(reservoir_react_kernel<16>, dim3(blocks), dim3(64), 0, st, a); break;
+        }
+        hip_check(hipGetLastError(), "reservoir reaction launch");
+    }
+    ++steps_;
     cur_ ^= 1;
+}
+
+void Device::read_reservoir_state(std::vector<uint32_t> &voxels, std::vector<int32_t> &state)
+{
+    voxels = res_vh_;
+    state.assign(size_t(res_n_) * 4u, 0);
+    if (res_n_ == 0u)
+        return;
+    hip_check(hipSetDevice(dev_), "hipSetDevice");
+    hipStream_t st = static_cast<hipStream_t>(stream_);
+    hip_check(hipMemcpyAsync(state.data(), res_st_, res_n_ * sizeof(ResVoxel), hipMemcpyDeviceToHost, st),
+              "read reservoir state");
+    hip_check(hipStreamSynchronize(st), "read reservoir state");
 }
 
 float Device::run_timed(const uint32_t *itime, uint32_t nsteps)
@@ -3231,6 +3479,19 @@ void Device::region_counts(int32_t *out)
     hip_check(hipGetLastError(), "region_count_kernel launch");
     hip_check(hipMemcpyAsync(out, hist_, nbins * sizeof(int32_t), hipMemcpyDeviceToHost, st), "read hist");
     hip_check(hipStreamSynchronize(st), "region counts");
+    if (res_n_ > 0u) {  // the reference's counts hold the reservoir units that no slot holds
+        std::vector<uint32_t> vox;
+        std::vector<int32_t> rs;
+        read_reservoir_state(vox, rs);
+        for (uint32_t i = 0; i < res_n_; ++i) {
+            const uint32_t sp = uint32_t(rs[4 * i]), K = sp & 0xFFu, R = sp >> 8;
+            int32_t *row = out + size_t(res_reg_[i]) * (ns_ + 1u);
+            if (K)
+                row[K] += rs[4 * i + 2];
+            if (R)
+                row[R] += rs[4 * i + 1];
+        }
+    }
 }
 
 void Device::read_counters(uint64_t *out)
@@ -3270,6 +3531,8 @@ Ensemble::Ensemble(const std::vector<const Model *> &cells, const std::vector<ui
         if (m->slots() != S_)
             throw Error(CS_ERR_ARG, "all cells of an ensemble have the same slots per voxel");
         check_device_model(*m);
+        if (m->has_reservoir())
+            throw Error(CS_ERR_UNSUPPORTED, "ensemble cells cannot hold reservoir particles");
     }
     int ndev = 0;
     hip_check(hipGetDeviceCount(&ndev), "hipGetDeviceCount");

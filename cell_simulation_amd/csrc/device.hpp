@@ -37,6 +37,11 @@ public:
 
     uint64_t n_voxels() const { return N_; }
     uint32_t slots() const { return S_; }
+    uint64_t steps() const { return steps_; }
+
+    // Reservoir voxels (ascending indices) and, per voxel, four int32: K | R << 8, the units of R and of K
+    // that the reference's per-voxel counts hold beyond the slots, and occupancy - occupied slots.
+    void read_reservoir_state(std::vector<uint32_t> &voxels, std::vector<int32_t> &state);
 
     // Well-mixed (CME) pool: counts, seed of the draws and the f64 precision audit.
     void read_cme_counts(int32_t *out);
@@ -98,6 +103,15 @@ private:
     uint32_t off_creac_ = 0, off_cdelta_ = 0, off_cemit_ = 0, off_cregion_ = 0, off_crates_ = 0;
     uint32_t off_clists_ = 0, off_cloff_ = 0, off_cllen_ = 0;
     unsigned long long *audit_ = nullptr;  // [3]
+
+    uint64_t steps_ = 0;           // steps enqueued
+    uint32_t res_n_ = 0;           // reservoir voxels
+    uint32_t *res_bits_ = nullptr; // [N/32] bit per reservoir voxel
+    uint32_t *res_skew_ = nullptr; // [N/32] bit per reservoir voxel whose occupancy offset is non-zero
+    uint32_t *res_v_ = nullptr;    // [res_n] voxel indices, ascending
+    void *res_st_ = nullptr;       // [res_n] per-voxel ghost units and occupancy offset
+    std::vector<uint32_t> res_vh_; // host copies: voxel indices and their region labels
+    std::vector<uint8_t> res_reg_;
 };
 
 }  // namespace cs

@@ -348,6 +348,22 @@ class Simulation:
         check(lib().cs_sim_read_reservoir(self._h, out.ctypes.data, out.size))
         return out.reshape(*inf["res"])
 
+    def concentrations(self) -> np.ndarray:
+        """Per-voxel species counts int32[N][ns+1], the reference's `concentrations` buffer
+        (src/lattice.rs:14-23): the slot contents plus the reservoir units no slot holds."""
+        inf = self.info()
+        n1 = inf["num_species"] + 1
+        out = np.empty(inf["n_voxels"] * n1, dtype=np.int32)
+        check(lib().cs_sim_read_concentrations(self._h, out.ctypes.data, out.size))
+        return out.reshape(inf["n_voxels"], n1)
+
+    def occupancy(self) -> np.ndarray:
+        """Per-voxel occupancy word uint32[N] (src/lattice.rs:14-23; rdme.wgsl:61-80, cme.wgsl:114)."""
+        inf = self.info()
+        out = np.empty(inf["n_voxels"], dtype=np.uint32)
+        check(lib().cs_sim_read_occupancy(self._h, out.ctypes.data, out.size))
+        return out
+
     def species_counts(self) -> np.ndarray:
         n = self.info()["num_species"] + 1
         out = np.empty(n, dtype=np.int32)

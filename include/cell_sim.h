@@ -188,6 +188,14 @@ int cs_sim_read_tables(const cs_sim *sim, float *diffusion, size_t n_diffusion, 
                        size_t n_rates);
 /* Reservoir species per voxel, u32[N] (src/lattice.rs:292-300). */
 int cs_sim_read_reservoir(const cs_sim *sim, uint32_t *out, size_t n);
+/* Per-voxel species counts, int32[N][ns+1]: the reference's `concentrations` buffer (src/lattice.rs:14-23,
+ * updated by rdme.wgsl:66,71 and cme.wgsl:84,104). They equal the slot contents except at reservoir
+ * voxels; entry 0 of a voxel is 1 once a step has run (cme.wgsl:31), 0 before. */
+int cs_sim_read_concentrations(cs_sim *sim, int32_t *out, size_t n);
+/* Per-voxel occupancy word, u32[N]: the reference's `occupancy` buffer (src/lattice.rs:14-23, rdme.wgsl:61-80,
+ * cme.wgsl:114). It equals the occupied slot count except at reservoir voxels, where the reaction's
+ * skipped species leave it off by a signed amount (kept modulo 2^32, as the reference's u32 atomics). */
+int cs_sim_read_occupancy(cs_sim *sim, uint32_t *out, size_t n);
 
 /* A statistics sample (src/statistics.rs:4-9 SolverStatisticSample). Returns CS_OK and fills the
  * sample, or CS_ERR_NOT_FOUND when the queue is empty. */

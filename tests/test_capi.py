@@ -65,10 +65,31 @@ def test_device_preparation_fails_loudly_without_a_gpu():
 
 
 def test_unsupported_models_are_rejected_before_any_device_work():
+    # reservoir particles together with well-mixed species have no device form: refused before any HIP call
     sim = Simulation_with_reservoir()
+    sim.add_cme_species("pool", 10)
     with pytest.raises(CellSimError) as e:
         sim.prepare_for_gpu(0)
-    assert e.value.code in (_lib.CS_ERR_UNSUPPORTED, _lib.CS_ERR_HIP, _lib.CS_ERR_ARG)
+    assert e.value.code == _lib.CS_ERR_UNSUPPORTED
+
+
+def test_concentrations_and_occupancy_before_device():
+    # before any step: counts = slot contents + one unit per reservoir site (lattice.rs:297), entry 0 is 0;
+    # occupancy = occupied slots
+    sim = Simulation_with_reservoir()
+    inf = sim.info()
+    N, S = inf["n_voxels"], inf["max_particles_site"]
+    lat = sim.lattice().reshape(N, S)
+    res = sim.reservoir().ravel()
+    c = sim.concentrations()
+    assert c.shape == (N, inf["num_species"] + 1) and not c[:, 0].any()
+    exp = np.zeros_like(c)
+    for sp in range(1, c.shape[1]):
+        exp[:, sp] = np.count_nonzero(lat == sp, axis=1) + (res == sp)
+    assert np.array_equal(c, exp) and np.count_nonzero(res) > 0
+    assert np.array_equal(sim.occupancy(), np.count_nonzero(lat, axis=1))
+    assert np.array_equal(sim.species_counts()[1:], exp.sum(axis=0)[1:])
+    assert np.array_equal(sim.region_counts().sum(axis=0)[1:], exp.sum(axis=0)[1:])
 
 
 def Simulation_with_reservoir():

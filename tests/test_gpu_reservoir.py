@@ -1,0 +1,106 @@
+"""Reservoir particles on the HIP path, step by step against the oracle.
+
+A reservoir site (src/lattice.rs:292-300) adds one unit of its species to the voxel's counts and to no
+slot; the reaction (res/shader/cme.wgsl:76-110) skips the species reservoirs[v*S] at voxel v, an index in
+lattice units, so it names the voxel's own entry only at v = 0 and another voxel's entry below N/S (an index
+past the end reads 0). From then on the reference's per-voxel counts and occupancy word differ from the
+slots at a few voxels: units that join the slots, slots that vanish from them while the counts keep them,
+rollbacks that subtract the skipped species, occupancy words off by a signed amount (modulo 2^32, which
+refuses every claim into an emptied voxel with a word of -1). The HIP step runs those voxels' reactions in
+a pass of their own and replays the claims into voxels whose word is off the slot count; these cases check
+the lattice, the per-voxel counts, the occupancy words, the global counts and the event counters after
+every step, with each kernel.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from cell_simulation_amd import LatticeParams, RegionType, Simulation, default_itime_schedule
+from cell_simulation_amd._lib import CS_CNT_EXCEPTION, CS_CNT_FIRED, CS_CNT_NEAR_THRESHOLD, CS_CNT_ROLLBACK
+from tests.conftest import announce
+
+pytestmark = pytest.mark.gpu
+
+
+def _reservoir_model(edge, slots, seed, n_res=400, density=0.3):
+    lam = 32e-9
+    sim = Simulation(LatticeParams([edge * lam] * 3, [edge] * 3, 50e-6, lam, slots))
+    sim.set_seed(seed)
+    sim.add_region(RegionType.Cube("cell", [0, 0, 0], [edge * lam] * 3), 2e-12)
+    sim.prepare_regions()
+    n = int(density * edge ** 3)
+    sim.add_particle_count("Ares", "cell", n_res, True, True)
+    sim.add_particle_count("A", "cell", n, True, False)
+    sim.add_particle_count("B", "cell", n // 2, True, False)
+    sim.add_particle_count("C", "cell", 0, True, False)
+    sim.add_reaction(["Ares"], ["Ares", "B"], 4000.0)
+    sim.add_reaction(["B"], [], 1000.0)
+    sim.add_reaction(["A", "B"], ["C"], 5000.0)
+    sim.add_reaction(["C"], ["A", "B"], 2000.0)
+    sim.add_reaction(["Ares", "B"], ["A"], 3000.0)
+    sim.add_reaction(["A", "A"], ["A", "A", "A"], 500.0)  # fills voxels: rollbacks near S
+    return sim
+
+
+def _check_step(sim, orc, tag):
+    g = sim.lattice()
+    o = orc.lattice_u8()
+    if not np.array_equal(g, o):
+        diff = np.argwhere(np.any(g != o, axis=-1))
+        v = tuple(diff[0])
+        raise AssertionError(f"{tag}: {len(diff)} voxels differ; first {v}: gpu {g[v]} oracle {o[v]}")
+    gc, oc = sim.concentrations(), orc.counts_array()
+    if not np.array_equal(gc, oc):
+        bad = np.argwhere(np.any(gc != oc, axis=1))[:, 0]
+        raise AssertionError(f"{tag}: counts of {len(bad)} voxels differ; first {bad[0]}: gpu {gc[bad[0]]} "
+                             f"oracle {oc[bad[0]]}")
+    go = sim.occupancy()
+    if not np.array_equal(go, orc.occupancy):
+        bad = np.nonzero(go != orc.occupancy)[0]
+        raise AssertionError(f"{tag}: occupancy of {len(bad)} voxels differs; first {bad[0]}: gpu {go[bad[0]]} "
+                             f"oracle {orc.occupancy[bad[0]]}")
+    assert np.array_equal(sim.species_counts()[1:], orc.stat[1:]), (tag, sim.species_counts(), orc.stat)
+
+
+@pytest.mark.parametrize("kernel", ["wave", "tiled", "simple"])
+def test_reservoir_parity(oracle_mod, kernel):
+    announce(f"reservoir model 20^3 S=8, 400 reservoir sites ({kernel}), step by step up to the first alias")
+    os.environ["CS_STEP_KERNEL"] = kernel
+    try:
+        sim = _reservoir_model(20, 8, seed=23)
+        orc = oracle_mod.Oracle.from_simulation(sim)
+        sim.prepare_for_gpu(0)
+    finally:
+        os.environ.pop("CS_STEP_KERNEL", None)
+    S = 8
+    res = sim.reservoir().ravel()
+    N = res.size
+    # the case must hold voxels that skip another voxel's reservoir species (v*S < N) and sites of their own
+    k_vox = [v for v in range(N // S) if res[v * S]]
+    assert len(k_vox) > 10 and np.count_nonzero(res) == 400
+    _check_step(sim, orc, f"{kernel} initial state")
+    # up to the step before the first slot-S alias (rdme.wgsl:30-40), whose shared slot the byte-slot state
+    # cannot hold; A + A -> 3A fills voxels until one happens (step 41 with this seed)
+    compared = 0
+    skewed = 0
+    seen = orc.counters.as_dict()
+    for t, it in enumerate(default_itime_schedule(23, 60)):
+        orc.step(int(it))
+        if orc.counters.alias_writes:
+            break
+        seen = orc.counters.as_dict()
+        sim.step(t, int(it), 0)
+        _check_step(sim, orc, f"{kernel} step {t}")
+        compared += 1
+        slots = np.count_nonzero(orc.lattice_u8().reshape(N, S), axis=1)
+        skewed = max(skewed, int(np.count_nonzero(orc.occupancy.astype(np.int64) != slots)))
+    c = sim.counters()
+    print(f"compared {compared} steps; oracle {seen}; occupancy words off the slot count: up to {skewed}")
+    assert compared >= 35
+    assert c[CS_CNT_EXCEPTION] == 0
+    assert c[CS_CNT_FIRED] == seen["fired"] > 0
+    assert c[CS_CNT_ROLLBACK] == seen["rollbacks"] > 0
+    assert c[CS_CNT_NEAR_THRESHOLD] == seen["near_threshold"]
+    assert seen["failed_claims"] > 0
+    assert skewed > 20, "the case must leave occupancy words off the slot count"

@@ -180,6 +180,72 @@ def test_rollback_when_rewrite_reaches_s(oracle_mod):
     assert list(full.lattice) == [1, 1, 1, 0] and full.counters.rollbacks == 1
 
 
+# ------------------------------------------------------- reservoirs ----
+# A reservoir site holds one unit of its species in the per-voxel counts and in no slot
+# (lattice.rs:292-300). cme.wgsl:80 skips the species reservoirs[v*S] -- the lattice index of the voxel's
+# slot 0, which names the voxel's own entry only for v = 0; an index past the end reads 0 here.
+def _reservoir_column(oracle_mod, slots, reservoir, stoich_rows, reactants, rates, S=4, ns=3):
+    n = len(reservoir)
+    stoich = np.zeros((len(stoich_rows) + 1, ns + 1), np.int32)
+    for i, r in enumerate(stoich_rows):
+        stoich[i + 1] = r
+    rx = np.zeros((len(reactants) + 1, 3), np.int32)
+    for i, r in enumerate(reactants):
+        rx[i + 1, :len(r)] = r
+    k = np.concatenate([[0.0], rates]).astype(np.float32)
+    return oracle_mod.Oracle((1, 1, n), S, 1.0, 1.0, np.zeros((1, 1, ns + 1), np.float32), stoich, rx, k,
+                             np.zeros(n, np.uint32), np.asarray(slots, np.uint32), np.asarray(reservoir, np.uint32))
+
+
+def test_reservoir_at_voxel_zero_is_a_constant_source(oracle_mod):
+    # voxel 0 reads its own entry: A -> A + B never rewrites A, so the unit stays out of the slots while B
+    # piles up until the rewrite would reach S (then it is undone)
+    o = _reservoir_column(oracle_mod, [0, 0, 0, 0], [1], [[0, 0, 1, 0]], [[1]], [1e30])
+    for t in range(3):
+        o.step(100 + t)
+    assert list(o.lattice) == [2, 2, 2, 0] and list(o.occupancy) == [3]
+    assert list(o.counts_array()[0]) == [1, 1, 3, 0]
+    o.step(103)
+    assert list(o.lattice) == [2, 2, 2, 0] and o.counters.rollbacks == 1
+    assert list(o.stat[1:]) == [1, 3, 0]
+
+
+def test_reservoir_unit_joins_the_slots_where_the_index_misses(oracle_mod):
+    # voxel 1 of 2 reads reservoirs[4], past the end: nothing is skipped, so the first firing writes C[A] = 1
+    # copy of A next to the new B, while the occupancy word only adds the stoichiometry (+1)
+    o = _reservoir_column(oracle_mod, [0] * 8, [0, 1], [[0, 0, 1, 0]], [[1]], [1e30])
+    o.step(100)
+    assert list(o.lattice[4:]) == [1, 2, 0, 0] and list(o.occupancy) == [0, 1]
+    o.step(200)
+    assert list(o.lattice[4:]) == [1, 2, 2, 0] and list(o.occupancy) == [0, 2]
+    assert list(o.counts_array()[1]) == [1, 1, 2, 0] and list(o.stat[1:]) == [1, 2, 0]
+
+
+def test_reservoir_index_skips_another_species_at_voxel_one(oracle_mod):
+    # voxel 4 is a reservoir of A, so voxel 1 (index 1*S = 4) skips A: B -> C there drops its real A from
+    # the slots but keeps it in the counts, and the occupancy word keeps counting it
+    slots = [0] * 32
+    slots[4:8] = [1, 2, 0, 0]
+    o = _reservoir_column(oracle_mod, slots, [0, 0, 0, 0, 1, 0, 0, 0], [[0, 0, -1, 1]], [[2]], [1e30])
+    o.step(100)
+    assert list(o.lattice[4:8]) == [3, 0, 0, 0] and o.occupancy[1] == 2
+    assert list(o.counts_array()[1]) == [1, 1, 0, 1] and list(o.counts_array()[4]) == [1, 1, 0, 0]
+    assert list(o.stat[1:]) == [2, 0, 1]
+
+
+def test_rollback_subtracts_the_skipped_species(oracle_mod):
+    # A + B -> 3B at voxel 1 (skips A): B would reach S, so the rewrite is undone, and the undo subtracts
+    # stoich[A] = -1 from C[A] although A was never added: C[A] grows by one per firing
+    slots = [0] * 32
+    slots[4:8] = [1, 2, 2, 0]
+    o = _reservoir_column(oracle_mod, slots, [0, 0, 0, 0, 1, 0, 0, 0], [[0, -1, 2, 0]], [[1, 2]], [1e30])
+    for t in range(3):
+        o.step(100 + t)
+    assert list(o.lattice[4:8]) == [1, 2, 2, 0] and o.occupancy[1] == 3
+    assert o.counters.rollbacks == 3
+    assert list(o.counts_array()[1]) == [1, 4, 2, 0] and list(o.stat[1:]) == [5, 2, 0]
+
+
 def test_zero_propensity_fires_only_at_seed_zero(oracle_mod):
     # No reactions: a0 = 0, threshold 1 - exp(0) = 0, so it "fires" only when UF(PCG(seed)) == 0, i.e. seed 0
     # (itime + x*y + z = itime here). cum = 0/0 = NaN selects the dummy row 0 and the rewrite sorts the
