@@ -102,6 +102,9 @@ SIGNATURES = {
     "cs_sim_add_particle_count": (C.c_int, [_SIM, C.c_char_p, C.c_char_p, C.c_uint32, C.c_int, C.c_int]),
     "cs_sim_add_particle_concentration": (C.c_int, [_SIM, C.c_char_p, C.c_char_p, C.c_float, C.c_int, C.c_int]),
     "cs_sim_fill_region": (C.c_int, [_SIM, C.c_char_p, C.c_char_p, C.c_int]),
+    "cs_sim_particle_random_walk": (C.c_int, [_SIM, C.c_char_p, C.c_char_p, C.c_float, C.c_float, C.c_float,
+                                              C.c_int]),
+    "cs_sim_add_sparse_region": (C.c_int, [_SIM, C.c_char_p, C.c_float, C.c_char_p, C.c_uint32, C.c_float]),
     "cs_sim_particle_index": (C.c_int, [_SIM, C.c_char_p]),
     "cs_sim_particle_name": (C.c_int, [_SIM, C.c_uint32, C.c_char_p, C.c_size_t]),
     "cs_sim_set_diffusion_rate": (C.c_int, [_SIM, C.c_char_p, C.c_float]),

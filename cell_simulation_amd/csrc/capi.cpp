@@ -272,6 +272,24 @@ int cs_sim_fill_region(cs_sim *sim, const char *name, const char *to_region, int
     return guard([&] { builder(sim).fill_region(str(name, "name"), str(to_region, "to_region"), logging != 0); });
 }
 
+int cs_sim_particle_random_walk(cs_sim *sim, const char *name, const char *to_region, float total_length,
+                                float block_length, float radius, int logging)
+{
+    return guard([&] {
+        builder(sim).particle_random_walk(str(name, "name"), str(to_region, "to_region"), total_length, block_length,
+                                          radius, logging != 0);
+    });
+}
+
+int cs_sim_add_sparse_region(cs_sim *sim, const char *name, float radius, const char *to_region,
+                             uint32_t max_volume, float diffusion_rate)
+{
+    return guard([&] {
+        builder(sim).add_sparse_region(str(name, "name"), radius, str(to_region, "to_region"), max_volume,
+                                       diffusion_rate);
+    });
+}
+
 int cs_sim_particle_index(const cs_sim *sim, const char *name)
 {
     if (!sim || !name)

@@ -3,9 +3,11 @@
 #include "model.hpp"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
 
 #include "../../include/cell_sim.h"
 #include "json.hpp"
@@ -50,6 +52,8 @@ Model::Model(const LatticeParams &p) : params_(p), rng_(0)
     // simulation.rs:59-66: region 0 is the background cube covering the box.
     regions_.assign(N, 0u);
     region_types_.push_back({RegionKind::Cube, "background"});
+    for (int i = 0; i < 3; ++i)
+        region_types_.back().b[i] = p.dims[i];
     volumes_.push_back(static_cast<uint32_t>(N));
     // lattice.rs:33: species 0 is "void".
     particle_names_.push_back("void");
@@ -97,6 +101,11 @@ void Model::add_region_cylinder(const std::string &name, const float p0[3], cons
                                 float diffusion_rate)
 {
     region_types_.push_back({RegionKind::Cylinder, name});
+    for (int i = 0; i < 3; ++i) {
+        region_types_.back().a[i] = p0[i];
+        region_types_.back().b[i] = pf[i];
+    }
+    region_types_.back().r = radius;
     const uint32_t new_idx = static_cast<uint32_t>(region_types_.size()) - 1u;
     const float res[3] = {float(params_.res[0]), float(params_.res[1]), float(params_.res[2])};
     float vs[3];
@@ -180,6 +189,10 @@ void Model::add_region_cube(const std::string &name, const float p0[3], const fl
             throw Error(CS_ERR_ARG, "cube extends past the lattice");
     }
     region_types_.push_back({RegionKind::Cube, name});
+    for (int i = 0; i < 3; ++i) {
+        region_types_.back().a[i] = p0[i];
+        region_types_.back().b[i] = pf[i];
+    }
     const uint32_t new_idx = static_cast<uint32_t>(region_types_.size()) - 1u;
     const uint64_t volume = (end[0] - start[0]) * (end[1] - start[1]) * (end[2] - start[2]);
     for (uint64_t x = start[0]; x < end[0]; ++x)
@@ -197,6 +210,9 @@ void Model::add_region_cube(const std::string &name, const float p0[3], const fl
 void Model::add_region_sphere(const std::string &name, const float center[3], float radius, float diffusion_rate)
 {
     region_types_.push_back({RegionKind::Sphere, name});
+    for (int i = 0; i < 3; ++i)
+        region_types_.back().a[i] = center[i];
+    region_types_.back().r = radius;
     const float res[3] = {float(params_.res[0]), float(params_.res[1]), float(params_.res[2])};
     const uint32_t new_idx = static_cast<uint32_t>(region_types_.size()) - 1u;
     float vs[3];
@@ -228,6 +244,9 @@ void Model::add_region_semisphere(const std::string &name, const float center[3]
                                   const float direction[3], float diffusion_rate)
 {
     region_types_.push_back({RegionKind::SemiSphere, name});
+    for (int i = 0; i < 3; ++i)
+        region_types_.back().a[i] = center[i];
+    region_types_.back().r = radius;
     const float res[3] = {float(params_.res[0]), float(params_.res[1]), float(params_.res[2])};
     float vs[3];
     params_.voxel_size(vs);
@@ -363,6 +382,271 @@ const std::vector<uint32_t> &Model::index_list(int region_idx) const
     if (it == index_buffer_.end())
         throw Error(CS_ERR_FULL, "region has no voxels");
     return it->second;
+}
+
+// region.rs:204-248: a random point (real units) of a region's primitive; region kinds without a rule give
+// the origin, as the reference's catch-all arm.
+void Model::generate_point(uint32_t region_idx, float out[3])
+{
+    const RegionDesc &d = region_types_.at(region_idx);
+    const float two_pi = 6.2831853071795864f, pi = 3.1415926535897932f;
+    switch (d.kind) {
+    case RegionKind::Cube:
+        for (int i = 0; i < 3; ++i)
+            out[i] = d.a[i] + (d.b[i] - d.a[i]) * rng_.unit();
+        break;
+    case RegionKind::Sphere:
+    case RegionKind::SemiSphere: {  // the semisphere draws from its whole sphere (region.rs:225-234)
+        const float theta = two_pi * rng_.unit(), phi = pi * rng_.unit(), r = d.r * rng_.unit();
+        out[0] = r * std::cos(theta) * std::sin(phi) + d.a[0];
+        out[1] = r * std::sin(theta) * std::sin(phi) + d.a[1];
+        out[2] = r * std::cos(phi) + d.a[2];
+        break;
+    }
+    case RegionKind::Cylinder: {  // a disc around p0 in x-y, z between the end centers (region.rs:235-244)
+        const float theta = two_pi * rng_.unit(), r = d.r * rng_.unit();
+        out[0] = r * std::cos(theta) + d.a[0];
+        out[1] = r * std::sin(theta) + d.a[1];
+        out[2] = d.a[2] + (d.b[2] - d.a[2]) * rng_.unit();
+        break;
+    }
+    default:
+        out[0] = out[1] = out[2] = 0.f;
+    }
+}
+
+// region.rs:150-194: a voxel of the region whose box of +-radius_voxels lies inside the lattice and holds
+// only that region's voxels; ten draws, then an error (the reference panics).
+void Model::generate_boundary_aware(uint32_t region_idx, const uint64_t rv[3], uint64_t out[3])
+{
+    float vs[3];
+    params_.voxel_size(vs);
+    for (int attempt = 0; attempt < 10; ++attempt) {
+        float pt[3];
+        generate_point(region_idx, pt);
+        uint64_t p[3];
+        bool ok = true;
+        for (int i = 0; i < 3; ++i) {
+            const float f = pt[i] / vs[i];  // usize cast: saturates at 0, truncates
+            p[i] = f > 0.f ? uint64_t(f) : 0u;
+            if (p[i] >= params_.res[i])
+                ok = false;
+        }
+        if (!ok || region_at(site_index(p[0], p[1], p[2])) != region_idx)
+            continue;
+        for (int i = 0; i < 3; ++i)
+            if (p[i] < rv[i] || p[i] + rv[i] >= params_.res[i])
+                ok = false;
+        if (!ok)
+            continue;
+        // every voxel of [p - rv, p + rv) in the region (the reference compares the label sum)
+        for (uint64_t x = p[0] - rv[0]; ok && x < p[0] + rv[0]; ++x)
+            for (uint64_t y = p[1] - rv[1]; ok && y < p[1] + rv[1]; ++y)
+                for (uint64_t z = p[2] - rv[2]; ok && z < p[2] + rv[2]; ++z)
+                    if (region_at(site_index(x, y, z)) != region_idx)
+                        ok = false;
+        if (!ok)
+            continue;
+        for (int i = 0; i < 3; ++i)
+            out[i] = p[i];
+        return;
+    }
+    throw Error(CS_ERR_FULL, "no point of region " + region_types_.at(region_idx).name + " leaves room for the box");
+}
+
+// simulation.rs:483-535
+void Model::add_sparse_region(const std::string &name, float radius, const std::string &to_region,
+                              uint32_t max_volume, float diffusion_rate)
+{
+    const int ti = find_region_index(to_region);
+    if (ti < 0)
+        throw Error(CS_ERR_NOT_FOUND, "region not found: " + to_region);
+    if (!(volumes_[ti] > max_volume))
+        throw Error(CS_ERR_ARG, "sparse region: max_volume must be below the volume of " + to_region);
+    float vs[3];
+    params_.voxel_size(vs);
+    const float vs2[3] = {vs[0] * vs[0], vs[1] * vs[1], vs[2] * vs[2]};
+    const float r2 = radius * radius;
+    uint64_t rv[3];
+    for (int i = 0; i < 3; ++i) {
+        const float f = radius / vs[i];
+        rv[i] = f > 0.f ? uint64_t(f) : 0u;
+    }
+    if (rv[0] == 0 && rv[1] == 0 && rv[2] == 0)
+        throw Error(CS_ERR_ARG, "sparse region: the sphere is smaller than a voxel");
+    const uint32_t new_idx = static_cast<uint32_t>(region_types_.size());
+    uint32_t volume = 0;
+    while (volume < max_volume) {
+        uint64_t p[3];
+        generate_boundary_aware(uint32_t(ti), rv, p);
+        for (uint64_t x = p[0] - rv[0]; x < p[0] + rv[0]; ++x)
+            for (uint64_t y = p[1] - rv[1]; y < p[1] + rv[1]; ++y)
+                for (uint64_t z = p[2] - rv[2]; z < p[2] + rv[2]; ++z) {
+                    const float dx = float(x > p[0] ? x - p[0] : p[0] - x);
+                    const float dy = float(y > p[1] ? y - p[1] : p[1] - y);
+                    const float dz = float(z > p[2] ? z - p[2] : p[2] - z);
+                    float dist = (dx * dx) * vs2[0];
+                    dist = dist + (dy * dy) * vs2[1];
+                    dist = dist + (dz * dz) * vs2[2];
+                    if (dist < r2) {
+                        const uint64_t vi = site_index(x, y, z);
+                        check_regions_collision(vi);
+                        set_region(new_idx, vi);
+                        volume += 1;
+                    }
+                }
+    }
+    RegionDesc d{RegionKind::Sparse, name};
+    d.r = radius;  // region.rs:82-85: a sphere at the origin stands for the whole set
+    region_types_.push_back(d);
+    update_matrices_region(diffusion_rate, 0.f);
+    volumes_.push_back(volume);
+    if (regions_prepared_)
+        prepare_regions();
+}
+
+// lattice.rs:302-312: empties the voxel's last occupied slot.
+bool Model::remove_particle_site(uint64_t v)
+{
+    if (occupancy_[v] == 0)
+        return false;
+    occupancy_[v] -= 1;
+    lattice_[v * slots() + occupancy_[v]] = 0;
+    return true;
+}
+
+// f32 -> u32 as Rust's `as` casts: truncation toward zero, saturating at 0 and 2^32 - 1 (NaN gives 0).
+static uint32_t f32_to_u32_saturating(float f)
+{
+    if (!(f > 0.f))
+        return 0u;
+    if (f >= 4294967296.f)
+        return 0xFFFFFFFFu;
+    return uint32_t(f);
+}
+
+// lattice.rs:172-278. The box is the reference's: from min(ip0 - r, ipf + r) to max(ipf + r, ip0 - r) per
+// axis, clipped to the lattice, walked in whole-voxel steps from that corner. A voxel belongs to the cylinder
+// when it lies between the two end planes and within the radius (the radius test is in real units, scaled by
+// the voxel size). The first voxel of the box outside the start voxel's region ends the block (false).
+bool Model::add_particles_cylinder(const uint64_t site[3], const float dir[3], float length, float radius,
+                                   uint8_t sp, uint64_t end[3], std::vector<uint64_t> &added)
+{
+    const uint32_t start_region = region_at(site_index(site[0], site[1], site[2]));
+    float vs[3];
+    params_.voxel_size(vs);
+    const float res[3] = {float(params_.res[0]), float(params_.res[1]), float(params_.res[2])};
+    float ip0[3], ipf[3], v[3], lo[3], hi[3];
+    for (int i = 0; i < 3; ++i) {
+        const float len = length / vs[i], rad = radius / vs[i];
+        ip0[i] = float(site[i]);
+        ipf[i] = ip0[i] + dir[i] * len;
+        v[i] = ipf[i] - ip0[i];
+        lo[i] = std::fmax(std::fmin(ip0[i] - rad, ipf[i] + rad), 0.f);
+        hi[i] = std::fmin(std::fmax(ipf[i] + rad, ip0[i] - rad), res[i]);
+    }
+    const float v2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    const float r_const = radius * radius * v2;
+    const float vs2[3] = {vs[0] * vs[0], vs[1] * vs[1], vs[2] * vs[2]};
+    for (float x = lo[0]; x < hi[0]; x += 1.f)
+        for (float y = lo[1]; y < hi[1]; y += 1.f)
+            for (float z = lo[2]; z < hi[2]; z += 1.f) {
+                const uint64_t vi = site_index(uint64_t(x), uint64_t(y), uint64_t(z));
+                if (region_at(vi) != start_region)
+                    return false;
+                const float w1[3] = {x - ipf[0], y - ipf[1], z - ipf[2]};
+                if (w1[0] * v[0] + w1[1] * v[1] + w1[2] * v[2] > 0.f)
+                    continue;  // past the far end plane
+                const float w[3] = {x - ip0[0], y - ip0[1], z - ip0[2]};
+                if (w[0] * v[0] + w[1] * v[1] + w[2] * v[2] < 0.f)
+                    continue;  // before the near end plane
+                const float c0 = w[1] * v[2] - w[2] * v[1], c1 = w[2] * v[0] - w[0] * v[2],
+                            c2 = w[0] * v[1] - w[1] * v[0];
+                const float cn = c0 * c0 * vs2[0] + c1 * c1 * vs2[1] + c2 * c2 * vs2[2];
+                if (cn <= r_const) {
+                    if (!add_particle_site(vi, sp))
+                        return false;
+                    added.push_back(vi);
+                }
+            }
+    for (int i = 0; i < 3; ++i)
+        end[i] = f32_to_u32_saturating(ipf[i]);
+    return true;
+}
+
+// simulation.rs:842-864 and lattice.rs:113-170.
+uint32_t Model::particle_random_walk(const std::string &name, const std::string &to_region, float total_length,
+                                     float block_length, float radius, bool logging)
+{
+    const int ri = find_region_index(to_region);
+    if (ri < 0)
+        throw Error(CS_ERR_NOT_FOUND, "region not found: " + to_region);
+    if (particle_names_.size() >= 255)
+        throw Error(CS_ERR_UNSUPPORTED, "at most 254 species fit the one-byte slot encoding");
+    if (!(block_length > 0.f) || !(total_length >= 0.f) || !(radius >= 0.f))
+        throw Error(CS_ERR_ARG, "random walk: lengths and radius must be positive");
+    const uint8_t sp = uint8_t(particle_names_.size());
+    const int back = 25;  // blocks stepped back when a block fails
+    float vs[3];
+    params_.voxel_size(vs);
+    const float safety = block_length * float(back) * 0.4f;
+    uint64_t rv[3];
+    for (int i = 0; i < 3; ++i) {
+        const float f = safety / vs[i];
+        rv[i] = f > 0.f ? uint64_t(f) : 0u;
+    }
+    uint64_t site[3];
+    generate_boundary_aware(uint32_t(ri), rv, site);
+    auto direction = [&](float d[3]) {  // utils.rs:14-21, uniform on the sphere
+        const float theta = 6.2831853071795864f * rng_.unit();
+        const float phi = std::acos(1.f - 2.f * rng_.unit());
+        d[0] = std::sin(phi) * std::cos(theta);
+        d[1] = std::sin(phi) * std::sin(theta);
+        d[2] = std::cos(phi);
+    };
+    float dir[3];
+    direction(dir);
+    std::deque<std::array<uint64_t, 3>> sites{{site[0], site[1], site[2]}};
+    std::deque<std::vector<uint64_t>> blocks;
+    float length = 0.f;
+    // The reference retries without bound; a block that keeps failing (radius wider than the region, say)
+    // ends the walk with an error here instead.
+    uint64_t failures = 0;
+    while (length < total_length) {
+        std::vector<uint64_t> added;
+        uint64_t end[3];
+        if (!add_particles_cylinder(site, dir, block_length, radius, sp, end, added)) {
+            // the failed block's own particles stay, as in the reference; the last `back` blocks go
+            if (++failures > 100000)
+                throw Error(CS_ERR_FULL, "random walk of " + name + " keeps leaving region " + to_region);
+            const size_t steps = std::min<size_t>(sites.size() - 1, size_t(back));
+            for (size_t k = 0; k < steps; ++k) {
+                sites.pop_back();
+                if (!blocks.empty()) {
+                    for (uint64_t vi : blocks.back())
+                        if (!remove_particle_site(vi))
+                            throw Error(CS_ERR_STATE, "random walk: stepping back met an empty voxel");
+                    blocks.pop_back();
+                }
+                length -= block_length;
+            }
+            for (int i = 0; i < 3; ++i)
+                site[i] = sites.front()[i];
+        } else {
+            length += block_length;
+            sites.push_back({end[0], end[1], end[2]});
+            if (sites.size() > size_t(back) + 1u)
+                sites.pop_front();
+            blocks.push_back(std::move(added));
+            if (blocks.size() > size_t(back) + 1u)
+                blocks.pop_front();
+            for (int i = 0; i < 3; ++i)
+                site[i] = end[i];
+        }
+        direction(dir);
+    }
+    new_species(name, logging);
+    return sp;
 }
 
 int Model::find_particle(const std::string &name) const

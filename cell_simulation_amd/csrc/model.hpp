@@ -38,12 +38,17 @@ struct LatticeParams {
     uint64_t n_voxels() const { return uint64_t(res[0]) * res[1] * res[2]; }
 };
 
-enum class RegionKind { Cube, Sphere, SemiSphere, Cylinder };
+enum class RegionKind { Cube, Sphere, SemiSphere, Cylinder, Sparse };
 
-// region.rs:9-20, primitives only: composite types are expanded into primitives when added.
+// region.rs:9-20, primitives only: composite types are expanded into primitives when added. The geometry is
+// what region.rs:204-248 draws random points from: a cube's corners p0 = a, pf = b; a sphere's or
+// semisphere's center a and radius r; a cylinder's end centers a, b and radius r.
 struct RegionDesc {
     RegionKind kind;
     std::string name;
+    float a[3] = {0.f, 0.f, 0.f};
+    float b[3] = {0.f, 0.f, 0.f};
+    float r = 0.f;
 };
 
 // SplitMix64: the build's seeded placement generator (the reference draws from rand::thread_rng,
@@ -58,6 +63,8 @@ struct SplitMix64 {
         z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
         return z ^ (z >> 31);
     }
+    // Uniform f32 in [0, 1) from the top 24 bits.
+    float unit() { return float(next() >> 40) * (1.0f / 16777216.0f); }
     // Uniform integer in [0, n), n >= 1, without modulo bias (Lemire's multiply-and-reject).
     uint64_t below(uint64_t n)
     {
@@ -103,6 +110,10 @@ public:
     void add_region_capsid(const std::string &shell, const std::string &interior, const float center[3],
                            const float dir[3], float internal_radius, float external_radius, float total_length,
                            float diffusion_rate);
+    // simulation.rs:483-535: spheres of `radius` drawn inside `to_region` (each wholly inside it) become the
+    // new region `name` until it holds at least max_volume voxels.
+    void add_sparse_region(const std::string &name, float radius, const std::string &to_region, uint32_t max_volume,
+                           float diffusion_rate);
     void prepare_regions();
     int find_region_index(const std::string &name) const;  // -1 if absent
     uint32_t region_volume(uint32_t idx) const { return volumes_.at(idx); }
@@ -113,6 +124,11 @@ public:
     void add_particle_concentration(const std::string &name, const std::string &to_region, float concentration,
                                     bool logging, bool is_reservoir);
     void fill_region(const std::string &name, const std::string &to_region, bool logging);
+    // simulation.rs:842-864 with lattice.rs:113-278: a new species laid as a chain of cylinders (blocks of
+    // block_length, radius `radius`, random directions) of total length total_length inside to_region,
+    // stepping back 25 blocks whenever a block leaves the region or meets a full voxel. Returns the species.
+    uint32_t particle_random_walk(const std::string &name, const std::string &to_region, float total_length,
+                                  float block_length, float radius, bool logging);
     int find_particle(const std::string &name) const;  // lattice.rs:315-322; -1 if absent
     const std::string &particle_name(uint32_t i) const { return particle_names_.at(i); }
     const std::vector<uint32_t> &logging_particles() const { return logging_; }
@@ -183,6 +199,12 @@ private:
         return z + y * params_.res[2] + x * uint64_t(params_.res[2]) * params_.res[1];
     }
     bool add_particle_site(uint64_t v, uint8_t sp);  // lattice.rs:280-290
+    bool remove_particle_site(uint64_t v);           // lattice.rs:302-312
+    void generate_point(uint32_t region_idx, float out[3]);  // region.rs:204-248
+    void generate_boundary_aware(uint32_t region_idx, const uint64_t radius_voxels[3], uint64_t out[3]);
+    // lattice.rs:172-278; false on a site outside the start's region or a full voxel (added sites stay listed)
+    bool add_particles_cylinder(const uint64_t site[3], const float dir[3], float length, float radius, uint8_t sp,
+                                uint64_t end[3], std::vector<uint64_t> &added);
     bool add_reservoir_site(uint64_t v, uint32_t sp);  // lattice.rs:292-300
     const std::vector<uint32_t> &index_list(int region_idx) const;
 

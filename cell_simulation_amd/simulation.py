@@ -186,6 +186,15 @@ class Simulation:
             raise CellSimError(_lib.CS_ERR_ARG, f"region type not implemented: {reg!r}")
         check(rc)
 
+    def add_sparse_region(self, name: str, base_region, to_region: str, max_volume: int,
+                          diffusion_rate: float) -> None:
+        """simulation.rs:483-535. `base_region` is a RegionType.Sphere (only its radius is used) or a radius."""
+        radius = base_region if isinstance(base_region, (int, float)) else getattr(base_region, "radius", None)
+        if radius is None:
+            raise CellSimError(_lib.CS_ERR_ARG, "only spheres can be added as sparse regions")
+        check(lib().cs_sim_add_sparse_region(self._h, name.encode(), float(radius), to_region.encode(),
+                                             int(max_volume), float(diffusion_rate)))
+
     def prepare_regions(self) -> None:
         check(lib().cs_sim_prepare_regions(self._h))
 
@@ -204,6 +213,12 @@ class Simulation:
 
     def fill_region(self, name: str, to_region: str, logging: bool) -> None:
         check(lib().cs_sim_fill_region(self._h, name.encode(), to_region.encode(), int(bool(logging))))
+
+    def particle_random_walk(self, name: str, to_region: str, total_length: float, block_length: float,
+                             radius: float, logging: bool) -> None:
+        """simulation.rs:842-864: a new species laid as a chain of cylinders inside `to_region`."""
+        check(lib().cs_sim_particle_random_walk(self._h, name.encode(), to_region.encode(), float(total_length),
+                                                float(block_length), float(radius), int(bool(logging))))
 
     def particle_index(self, name: str) -> int:
         return lib().cs_sim_particle_index(self._h, name.encode())

@@ -80,6 +80,12 @@ int cs_sim_add_region_cylindrical_shell(cs_sim *sim, const char *shell_name, con
 int cs_sim_add_region_capsid(cs_sim *sim, const char *shell_name, const char *interior_name,
                              const float center[3], const float dir[3], float internal_radius,
                              float external_radius, float total_length, float diffusion_rate); /* :449-476 */
+/* Simulation::add_sparse_region (src/simulation.rs:483-535): spheres of `radius` (the base Sphere region;
+ * its center is not used) drawn wholly inside to_region (src/region.rs:150-194) become region `name` until it
+ * holds at least max_volume voxels. max_volume must be below to_region's volume. Placement draws from the
+ * simulation's seeded generator. */
+int cs_sim_add_sparse_region(cs_sim *sim, const char *name, float radius, const char *to_region,
+                             uint32_t max_volume, float diffusion_rate);
 /* Simulation::prepare_regions (src/simulation.rs:424; src/region.rs:129-148). */
 int cs_sim_prepare_regions(cs_sim *sim);
 /* find_region_index (src/simulation.rs:745-756); returns the index or -1. */
@@ -91,6 +97,12 @@ int cs_sim_add_particle_count(cs_sim *sim, const char *name, const char *to_regi
 int cs_sim_add_particle_concentration(cs_sim *sim, const char *name, const char *to_region,
                                       float concentration, int logging, int is_reservoir); /* :816-823 */
 int cs_sim_fill_region(cs_sim *sim, const char *name, const char *to_region, int logging); /* :825-840 */
+/* Simulation::particle_random_walk (src/simulation.rs:842-864; src/lattice.rs:113-278): a new species laid
+ * as a chain of cylinders (length block_length, radius `radius`, random directions) of total_length inside
+ * to_region, stepping back 25 blocks when a block leaves the region or meets a full voxel. Lengths in the
+ * lattice's units. A walk that fails 100000 blocks returns CS_ERR_FULL (the reference retries forever). */
+int cs_sim_particle_random_walk(cs_sim *sim, const char *name, const char *to_region, float total_length,
+                                float block_length, float radius, int logging);
 /* Lattice::find_particle (src/lattice.rs:315-322); returns the species index (>= 1) or -1. */
 int cs_sim_particle_index(const cs_sim *sim, const char *name);
 

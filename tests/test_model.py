@@ -170,3 +170,92 @@ def test_json_errors():
     with pytest.raises(CellSimError) as e:
         Simulation.from_json(json.dumps(m3))
     assert e.value.code == _lib.CS_ERR_PARSE
+
+
+# ---------------------------------------------- sparse regions, random walks ----
+# The reference draws both from rand::thread_rng (unseedable), so their placement is checked by invariants:
+# the geometry each routine guarantees, the bookkeeping of volumes, labels and tables, and the sizes.
+def _cell(edge=40, S=8, seed=5):
+    lam = 32e-9
+    sim = Simulation(LatticeParams([edge * lam] * 3, [edge] * 3, 50e-6, lam, S))
+    sim.set_seed(seed)
+    sim.add_region(RegionType.Sphere("cell", [edge * lam / 2] * 3, 0.45 * edge * lam), 1e-12)
+    sim.prepare_regions()
+    return sim, lam
+
+
+def _ball_size(radius, dims, res):
+    # simulation.rs:507-520 in f32: offsets d of [-rv, rv) per axis (rv = radius / voxel size, truncated)
+    # with sum(d_i^2 * voxel_size_i^2) < radius^2
+    vs = np.float32(dims) / np.float32(res)
+    r = np.float32(radius)
+    rv = int(r / vs)
+    r2, vs2 = r * r, vs * vs
+    n = 0
+    for i in range(-rv, rv):
+        for j in range(-rv, rv):
+            for k in range(-rv, rv):
+                d = np.float32(i * i) * vs2
+                d = d + np.float32(j * j) * vs2
+                d = d + np.float32(k * k) * vs2
+                n += bool(d < r2)
+    return n
+
+
+def test_sparse_region_balls_inside_target():
+    sim, lam = _cell()
+    before = sim.regions().copy()
+    cell = sim.region_index("cell")
+    vol_cell = int(np.count_nonzero(before == cell))
+    sim.add_sparse_region("granules", RegionType.Sphere("g", [0, 0, 0], 3 * lam), "cell", 400, 2e-13)
+    after = sim.regions()
+    g = sim.region_index("granules")
+    assert g == sim.info()["num_regions"] - 1
+    new = after == g
+    ball = _ball_size(3 * lam, 40 * lam, 40)
+    n = int(np.count_nonzero(new))
+    assert n % ball == 0 and 400 <= n < 400 + ball          # whole balls until max_volume is reached
+    assert np.all(before[new] == cell)                       # every ball lies inside the target region
+    assert int(np.count_nonzero(after == cell)) == vol_cell - n
+    assert np.array_equal(after[~new], before[~new])
+    D = sim.tables()["diffusion"]
+    assert np.all(D[g, g] == np.float32(2e-13)) and np.all(D[g, :g] == 0) and np.all(D[:g, g] == 0)
+    # particles can be placed into it once regions are prepared again (it re-lists them itself)
+    sim.add_particle_count("E", "granules", 50, True, False)
+    lat = sim.lattice()
+    assert np.all(after[np.any(lat == sim.particle_index("E"), axis=-1)] == g)
+
+
+def test_sparse_region_rejects_oversized_volume():
+    sim, lam = _cell(edge=16)
+    with pytest.raises(CellSimError):
+        sim.add_sparse_region("g", 2 * lam, "cell", 10 ** 6, 0.0)
+
+
+def test_random_walk_places_a_chain_inside_the_region():
+    # the start keeps 25 * 0.4 blocks of room on every side (lattice.rs:121-123): a box of +-15 voxels here,
+    # whose corners must stay inside the sphere of radius 28.8
+    sim, lam = _cell(edge=64, S=16)
+    regions = sim.regions()
+    cell = sim.region_index("cell")
+    block, radius, total = 1.5 * lam, lam, 40 * lam
+    sim.particle_random_walk("W", "cell", total, block, radius, True)
+    w = sim.particle_index("W")
+    assert w == sim.info()["num_species"]
+    lat = sim.lattice()
+    has_w = np.any(lat == w, axis=-1)
+    assert np.all(regions[has_w] == cell)                    # no block leaves the region
+    count = int(np.count_nonzero(lat == w))
+    # about pi r^2 L voxels along the kept blocks (voxel rasterisation of a thin tube: generous bounds)
+    tube = math.pi * 1.0 ** 2 * 40
+    assert 0.5 * tube < count < 2.0 * tube, (count, tube)
+    # the slots stay compacted and the counts agree with the lattice
+    occ = np.count_nonzero(lat, axis=-1)
+    assert np.all(np.sum(lat[..., :] != 0, axis=-1) == occ)
+    assert sim.species_counts()[w] == count
+
+
+def test_random_walk_fails_cleanly_in_a_region_too_small():
+    sim, lam = _cell(edge=12)
+    with pytest.raises(CellSimError):
+        sim.particle_random_walk("W", "cell", 100 * lam, 4 * lam, lam, False)
