@@ -685,6 +685,19 @@ int cs_sim_read_occupancy(cs_sim *sim, uint32_t *out, size_t n)
     });
 }
 
+int cs_sim_render_view(cs_sim *sim, uint32_t view, uint32_t slice_axis, uint32_t slice, float *out, size_t n)
+{
+    return guard([&] {
+        need(out, "out");
+        cs::Model &m = model_of(sim);
+        if (n != m.n_voxels())
+            throw cs::Error(CS_ERR_ARG, "view buffer must hold N floats");
+        if (!sim->device)
+            throw cs::Error(CS_ERR_STATE, "the view is computed on the device: call cs_sim_prepare_device first");
+        sim->device->render_view(view, slice_axis, slice, out);
+    });
+}
+
 int cs_sim_read_species_counts(cs_sim *sim, int32_t *out, size_t n)
 {
     return guard([&] {

@@ -954,6 +954,51 @@ __global__ __launch_bounds__(64) void reservoir_react_kernel(StepArgs a)
     count_event(a.counters, CS_CNT_ROLLBACK, rolled);
 }
 
+// ---- view (final_stage.wgsl:35-67): one value per voxel for a renderer -----------------------------
+// Voxel order (v = z + y*R2 + x*R2*R1). Hidden (0) when res[axis] - coord[axis] < slice. View 0: the species
+// found most often among slots 0..occupancy-1 in lattice-index order (slot S and on are the next voxel's,
+// past the lattice 0; ties keep the first in slot order), over 255; view 1: region label / 255; view 2:
+// reservoir species / 255. The occupancy word is the slot count plus the reservoir offset; a word above 2S
+// (only one wrapped below zero, whose loop in the reference walks the whole buffer) gives 0.
+template <int S>
+__global__ __launch_bounds__(kBlock) void view_kernel(StepArgs a, const uint8_t *resv, uint32_t view,
+                                                      uint32_t axis, uint32_t slice, float *out)
+{
+    const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
+    if (v >= a.N)
+        return;
+    const uint32_t z = v % a.R2, y = (v / a.R2) % a.R1, x = v / a.R12;
+    const uint32_t c = axis == 0u ? x : (axis == 1u ? y : z);
+    const uint32_t R = axis == 0u ? a.R0 : (axis == 1u ? a.R1 : a.R2);
+    float val = 0.0f;
+    if (!(R - c < slice)) {
+        if (view == 1u) {
+            val = float(a.reg[v]) / 255.0f;
+        } else if (view == 2u) {
+            val = float(resv != nullptr ? resv[v] : 0u) / 255.0f;
+        } else {
+            const Slots<S> s = load_slots<S>(a.src, v);
+            const uint32_t occ = count_nonzero(s) + uint32_t(res_offset(a, v));
+            uint32_t best = 0u, maxc = 0u;
+            if (occ <= 2u * uint32_t(S)) {
+                const Slots<S> nx = (occ > uint32_t(S) && v + 1u < a.N) ? load_slots<S>(a.src, v + 1u) : zero_slots<S>();
+                for (uint32_t i = 0; i < occ; ++i) {
+                    const uint32_t si = i < uint32_t(S) ? get_byte_dyn(s, i) : get_byte_dyn(nx, i - uint32_t(S));
+                    uint32_t cnt = 0u;
+                    for (uint32_t j = 0; j < occ; ++j)
+                        cnt += (j < uint32_t(S) ? get_byte_dyn(s, j) : get_byte_dyn(nx, j - uint32_t(S))) == si;
+                    if (cnt > maxc) {
+                        maxc = cnt;
+                        best = si;
+                    }
+                }
+            }
+            val = float(best) / 255.0f;
+        }
+    }
+    out[v] = val;
+}
+
 template <int S>
 __global__ __launch_bounds__(kBlock) void rdme_step_kernel(StepArgs a)
 {
@@ -3119,6 +3164,14 @@ Device::Device(const Model &m, int hip_device) : dev_(hip_device)
         hip_check(hipMemcpyAsync(res_st_, rst.data(), res_n_ * sizeof(ResVoxel), hipMemcpyHostToDevice, st),
                   "upload reservoir state");
     }
+    std::vector<uint8_t> rvol;
+    if (m.has_reservoir()) {  // the reservoir view's labels (final_stage.wgsl:63-66)
+        rvol.resize(N_);
+        for (uint64_t v = 0; v < N_; ++v)
+            rvol[v] = uint8_t(m.reservoir()[v]);
+        hip_check(hipMalloc(&res_vol_, N_), "hipMalloc reservoir volume");
+        hip_check(hipMemcpyAsync(res_vol_, rvol.data(), N_, hipMemcpyHostToDevice, st), "upload reservoir volume");
+    }
 
     hip_check(hipMemcpyAsync(lat_[0], m.lattice().data(), N_ * S_, hipMemcpyHostToDevice, st), "upload lattice");
     plan_wave_items(m.lattice().data());
@@ -3156,6 +3209,8 @@ Device::~Device()
     hipFree(res_skew_);
     hipFree(res_v_);
     hipFree(res_st_);
+    hipFree(res_vol_);
+    hipFree(view_);
     if (stream_)
         hipStreamDestroy(static_cast<hipStream_t>(stream_));
 }
@@ -3304,6 +3359,36 @@ void Device::step(uint32_t itime)
     }
     ++steps_;
     cur_ ^= 1;
+}
+
+void Device::render_view(uint32_t view, uint32_t axis, uint32_t slice, float *out)
+{
+    if (view > 2u || axis > 2u)
+        throw Error(CS_ERR_ARG, "view must be 0, 1 or 2 and the slice axis 0, 1 or 2");
+    hip_check(hipSetDevice(dev_), "hipSetDevice");
+    hipStream_t st = static_cast<hipStream_t>(stream_);
+    if (!view_)
+        hip_check(hipMalloc(&view_, N_ * sizeof(float)), "hipMalloc view");
+    StepArgs a{};
+    a.src = lat_[cur_];
+    a.reg = reg_;
+    a.R0 = res_[0];
+    a.R1 = res_[1];
+    a.R2 = res_[2];
+    a.R12 = res_[1] * res_[2];
+    a.N = uint32_t(N_);
+    a.res_v = res_v_;
+    a.res_st = static_cast<ResVoxel *>(res_st_);
+    a.res_n = res_n_;
+    const uint32_t blocks = uint32_t((N_ + kBlock - 1) / kBlock);
+    switch (S_) {
+    case 4: hipLaunchKernelGGL(view_kernel<4>, dim3(blocks), dim3(kBlock), 0, st, a, res_vol_, view, axis, slice, view_); break;
+    case 8: hipLaunchKernelGGL(view_kernel<8>, dim3(blocks), dim3(kBlock), 0, st, a, res_vol_, view, axis, slice, view_); break;
+    default: hipLaunchKernelGGL(view_kernel<16>, dim3(blocks), dim3(kBlock), 0, st, a, res_vol_, view, axis, slice, view_); break;
+    }
+    hip_check(hipGetLastError(), "view launch");
+    hip_check(hipMemcpyAsync(out, view_, N_ * sizeof(float), hipMemcpyDeviceToHost, st), "read view");
+    hip_check(hipStreamSynchronize(st), "read view");
 }
 
 void Device::read_reservoir_state(std::vector<uint32_t> &voxels, std::vector<int32_t> &state)

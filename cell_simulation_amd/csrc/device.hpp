@@ -42,6 +42,8 @@ public:
     // Reservoir voxels (ascending indices) and, per voxel, four int32: K | R << 8, the units of R and of K
     // that the reference's per-voxel counts hold beyond the slots, and occupancy - occupied slots.
     void read_reservoir_state(std::vector<uint32_t> &voxels, std::vector<int32_t> &state);
+    // The renderer's view of the current state (final_stage.wgsl:35-67), f32[N] in voxel order.
+    void render_view(uint32_t view, uint32_t axis, uint32_t slice, float *out);
 
     // Well-mixed (CME) pool: counts, seed of the draws and the f64 precision audit.
     void read_cme_counts(int32_t *out);
@@ -112,6 +114,8 @@ private:
     void *res_st_ = nullptr;       // [res_n] per-voxel ghost units and occupancy offset
     std::vector<uint32_t> res_vh_; // host copies: voxel indices and their region labels
     std::vector<uint8_t> res_reg_;
+    uint8_t *res_vol_ = nullptr;   // [N] reservoir species (reservoir models only), for the view
+    float *view_ = nullptr;        // [N] view output, allocated on first use
 };
 
 }  // namespace cs

@@ -379,6 +379,14 @@ class Simulation:
         check(lib().cs_sim_read_occupancy(self._h, out.ctypes.data, out.size))
         return out
 
+    def render_view(self, view: int = 0, slice_axis: int = 0, slice_: int = 0) -> np.ndarray:
+        """final_stage.wgsl:35-67 on the device: float32[R0, R1, R2] (0 = majority species, 1 = region,
+        2 = reservoir, each / 255; voxels with res[axis] - coord[axis] < slice are 0)."""
+        inf = self.info()
+        out = np.empty(inf["n_voxels"], dtype=np.float32)
+        check(lib().cs_sim_render_view(self._h, int(view), int(slice_axis), int(slice_), out.ctypes.data, out.size))
+        return out.reshape(*inf["res"])
+
     def species_counts(self) -> np.ndarray:
         n = self.info()["num_species"] + 1
         out = np.empty(n, dtype=np.int32)

@@ -208,6 +208,12 @@ int cs_sim_read_concentrations(cs_sim *sim, int32_t *out, size_t n);
  * cme.wgsl:114). It equals the occupied slot count except at reservoir voxels, where the reaction's
  * skipped species leave it off by a signed amount (kept modulo 2^32, as the reference's u32 atomics). */
 int cs_sim_read_occupancy(cs_sim *sim, uint32_t *out, size_t n);
+/* The renderer's view of the current state (res/shader/final_stage.wgsl:35-67, written there into an r32float
+ * 3D texture every step), computed on request: f32[N] in voxel order (v = z + y*R2 + x*R2*R1). view 0 = the
+ * majority species of the voxel / 255, 1 = region label / 255, 2 = reservoir species / 255
+ * (Uniforms.rendering_view); voxels with res[slice_axis] - coord[slice_axis] < slice are 0 (Uniforms.slice,
+ * slice_axis). Needs the device (CS_ERR_STATE before cs_sim_prepare_device). */
+int cs_sim_render_view(cs_sim *sim, uint32_t view, uint32_t slice_axis, uint32_t slice, float *out, size_t n);
 
 /* A statistics sample (src/statistics.rs:4-9 SolverStatisticSample). Returns CS_OK and fills the
  * sample, or CS_ERR_NOT_FOUND when the queue is empty. */

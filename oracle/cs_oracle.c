@@ -488,3 +488,43 @@ int cso_step_hybrid(const cso_model *m, const cso_cme_model *c, cso_state *st, i
         cso_cme_phase(m, c, st, counts, itime, ccnt);
     return cso_step(m, st, itime, cnt);
 }
+
+/* final_stage.wgsl:35-67 */
+void cso_view(const cso_model *m, const cso_state *st, uint32_t view, uint32_t axis, uint32_t slice, float *out)
+{
+    const uint64_t total = nvox(m) * m->S;
+    for (uint32_t x = 0; x < m->res[0]; ++x)
+        for (uint32_t y = 0; y < m->res[1]; ++y)
+            for (uint32_t z = 0; z < m->res[2]; ++z) {
+                const uint32_t v = vidx(m, x, y, z);
+                const uint32_t c[3] = {x, y, z};
+                if (m->res[axis] - c[axis] < slice) {                        /* :35-38 */
+                    out[v] = 0.0f;
+                    continue;
+                }
+                if (view == 1u) {                                            /* :59-62 */
+                    out[v] = (float)m->regions[v] / 255.0f;
+                    continue;
+                }
+                if (view == 2u) {                                            /* :63-66 */
+                    out[v] = (float)(m->reservoir ? m->reservoir[v] : 0u) / 255.0f;
+                    continue;
+                }
+                const uint32_t occ = st->occupancy[v];                       /* :41-57 */
+                uint32_t best = 0u, maxcount = 0u;
+                if (occ <= 2u * m->S) {
+                    const uint64_t base = (uint64_t)v * m->S;
+                    for (uint32_t i = 0; i < occ; ++i) {
+                        const uint32_t si = base + i < total ? st->lattice[base + i] : 0u;
+                        uint32_t count = 0u;
+                        for (uint32_t j = 0; j < occ; ++j)
+                            count += (base + j < total ? st->lattice[base + j] : 0u) == si;
+                        if (count > maxcount) {
+                            maxcount = count;
+                            best = si;
+                        }
+                    }
+                }
+                out[v] = (float)best / 255.0f;
+            }
+}

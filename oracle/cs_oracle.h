@@ -83,6 +83,14 @@ void cso_probability_vector(const cso_model *m, uint32_t x, uint32_t y, uint32_t
 /* Multi-step driver: steps t = 0..nsteps-1 use itime[t]. */
 int cso_run(const cso_model *m, cso_state *st, const uint32_t *itime, uint32_t nsteps, cso_counters *cnt);
 
+/* final_stage.wgsl:35-67: the r32float view of the current state, one value per voxel in voxel order
+ * (out[v], v = z + y*R2 + x*R2*R1). Voxels with res[axis] - coord[axis] < slice are 0. view 0: the
+ * species found most often among slots 0..occupancy-1 (lattice index order, so slots past S are the next
+ * voxel's; ties keep the first in slot order; an index past the lattice reads 0), divided by 255; view 1:
+ * region / 255; view 2: reservoir species / 255. An occupancy word above 2*S (only a word wrapped below
+ * zero) gives 0: the reference's loop would walk the whole buffer. */
+void cso_view(const cso_model *m, const cso_state *st, uint32_t view, uint32_t axis, uint32_t slice, float *out);
+
 /* ---- well-mixed (CME) phase: the build's own extension, NO reference counterpart -------------------
  * Once per step, before the lattice step, the well-mixed counts advance by an exact SSA (Gillespie's
  * direct method) over tau. The arithmetic below is the definition the HIP path follows bit for bit:

@@ -89,6 +89,8 @@ def lib() -> C.CDLL:
         d.cso_phase_c.argtypes = [C.POINTER(Model), C.POINTER(State), C.c_uint32, C.POINTER(Counters)]
         d.cso_phase_f.restype = None
         d.cso_phase_f.argtypes = [C.POINTER(Model), C.POINTER(State)]
+        d.cso_view.restype = None
+        d.cso_view.argtypes = [C.POINTER(Model), C.POINTER(State), C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p]
         d.cso_cme_h.restype = C.c_double
         d.cso_cme_h.argtypes = [C.c_void_p, C.c_void_p]
         d.cso_cme_bits.restype = C.c_uint64
@@ -232,6 +234,13 @@ class Oracle:
 
     def lattice_u8(self) -> np.ndarray:
         return self.lattice.astype(np.uint8).reshape(*self.res, self.S)
+
+    def view(self, view: int, axis: int = 0, slice_: int = 0) -> np.ndarray:
+        """final_stage.wgsl:35-67: float32[N] in voxel order."""
+        out = np.empty(self.N, dtype=np.float32)
+        lib().cso_view(C.byref(self.model), C.byref(self.state), int(view), int(axis), int(slice_),
+                       out.ctypes.data)
+        return out
 
     def counts_array(self) -> np.ndarray:
         return self.counts.reshape(self.N, self.ns + 1)

@@ -332,16 +332,15 @@ def test_large_cell_256_first_steps(oracle_mod):
 def test_sparse_region_and_random_walk_model(oracle_mod):
     """A model built with add_sparse_region and particle_random_walk (simulation.rs:483-535, 842-864) steps
     bit-exactly: the labels of the sparse balls and the chain of walk particles are ordinary model state."""
-    announce("sparse region + random walk 48^3 S=16, 120 steps, seed 8")
+    announce("sparse region + random walk 64^3 S=16, 120 steps, seed 8")
     lam = 32e-9
-    edge = 48
+    edge = 64
     sim = Simulation(LatticeParams([edge * lam] * 3, [edge] * 3, 50e-6, lam, 16))
     sim.set_seed(8)
     sim.add_region(RegionType.Sphere("cell", [edge * lam / 2] * 3, 0.45 * edge * lam), 1e-12)
     sim.prepare_regions()
-    sim.add_sparse_region("granules", RegionType.Sphere("g", [0, 0, 0], 2 * lam), "cell", 300, 1e-13)
-    sim.prepare_regions()
     sim.particle_random_walk("W", "cell", 30 * lam, 1.2 * lam, lam, True)
+    sim.add_sparse_region("granules", RegionType.Sphere("g", [0, 0, 0], 2 * lam), "cell", 300, 1e-13)
     sim.add_particle_count("E", "granules", 200, True, False)
     sim.add_particle_count("F", "cell", 1500, True, False)
     sim.set_transition_rate_particle("F", "cell", "granules", 5e-13)

@@ -104,3 +104,21 @@ def test_reservoir_parity(oracle_mod, kernel):
     assert c[CS_CNT_NEAR_THRESHOLD] == seen["near_threshold"]
     assert seen["failed_claims"] > 0
     assert skewed > 20, "the case must leave occupancy words off the slot count"
+
+
+def test_view_matches_oracle(oracle_mod):
+    """final_stage.wgsl:35-67 on the device against the oracle, on a reservoir model after 30 steps (so
+    some occupancy words are off the slot count): every view, with and without slicing."""
+    announce("view kernel: reservoir model 20^3 S=8 after 30 steps, views 0-2, slices")
+    sim = _reservoir_model(20, 8, seed=23)
+    orc = oracle_mod.Oracle.from_simulation(sim)
+    sim.prepare_for_gpu(0)
+    its = default_itime_schedule(23, 30)
+    sim.run(0, its)
+    orc.run(its)
+    _check_step(sim, orc, "after 30 steps")
+    for view, axis, sl in ((0, 0, 0), (1, 0, 0), (2, 0, 0), (0, 2, 7), (1, 1, 3), (0, 0, 25)):
+        got = sim.render_view(view, axis, sl).ravel()
+        exp = orc.view(view, axis, sl)
+        assert np.array_equal(got, exp), (view, axis, sl, int(np.count_nonzero(got != exp)))
+    assert np.count_nonzero(orc.view(0)) > 1000 and np.count_nonzero(orc.view(2)) == 400

@@ -333,3 +333,31 @@ def test_invariants_dense_random(oracle_mod):
     assert in1_after[0] >= in1_before[0]
     nz = L != 0
     assert not np.any(~nz[:, :-1] & nz[:, 1:])  # compacted
+
+
+# ------------------------------------------------------------- view ----
+def test_view_majority_ties_slicing_and_label_views(oracle_mod):
+    # final_stage.wgsl:35-67 on a 1x1x4 column, S=4: majority species (ties keep the first in slot order,
+    # empty voxels give 0) / 255; res[axis] - coord[axis] < slice hides a voxel; views 1 and 2 show labels
+    slots = [2, 3, 3, 0,  3, 2, 0, 0,  0, 0, 0, 0,  1, 1, 2, 2]
+    regions = np.array([0, 1, 1, 0], np.uint32)
+    o = oracle_mod.Oracle((1, 1, 4), 4, 1.0, 1.0, np.zeros((2, 2, 4), np.float32), np.zeros((1, 4), np.int32),
+                          np.zeros((1, 3), np.int32), np.zeros(1, np.float32), regions, np.asarray(slots, np.uint32),
+                          np.array([0, 0, 3, 1], np.uint32))
+    f = np.float32
+    assert list(o.view(0)) == [f(3) / f(255), f(3) / f(255), 0.0, f(1) / f(255)]
+    assert list(o.view(0, axis=2, slice_=2)) == [f(3) / f(255), f(3) / f(255), 0.0, 0.0]  # z = 3: 4 - 3 < 2
+    assert list(o.view(0, axis=2, slice_=5)) == [0.0] * 4
+    assert list(o.view(1)) == [0.0, f(1) / f(255), f(1) / f(255), 0.0]
+    assert list(o.view(2)) == [0.0, 0.0, f(3) / f(255), f(1) / f(255)]
+
+
+def test_view_follows_the_occupancy_word(oracle_mod):
+    # voxel 1 of the reservoir case above holds [C] with an occupancy word of 2: the majority runs over
+    # slots 0..1 = [C, 0], a tie that keeps C
+    slots = [0] * 32
+    slots[4:8] = [1, 2, 0, 0]
+    o = _reservoir_column(oracle_mod, slots, [0, 0, 0, 0, 1, 0, 0, 0], [[0, 0, -1, 1]], [[2]], [1e30])
+    o.step(100)
+    assert o.occupancy[1] == 2 and list(o.lattice[4:8]) == [3, 0, 0, 0]
+    assert o.view(0)[1] == np.float32(3) / np.float32(255)
