@@ -65,6 +65,13 @@ def ensemble_leg(args) -> dict:
     tiling = ens.tiling
     ens.close()
 
+    if args.plain_steps <= 0:  # the batched leg alone (profiling runs)
+        for sim in cells:
+            sim.close()
+        return {"voxels_per_step": vox, "unit": "replicate-voxel-steps/s", "batched_value": batched_value,
+                "batched_ms_per_step": ens_ms / args.ensemble_steps, "batched_steps": args.ensemble_steps,
+                "graph_block": args.graph_block, "tiling": tiling, "plain_value": None, "speedup": None,
+                "cme_events": int(c[5]), "exceptions": int(c[1])}
     log(f"plain: preparing {len(cells)} simulations")
     for sim in cells:
         sim.prepare_for_gpu(0)
@@ -93,9 +100,28 @@ def ensemble_leg(args) -> dict:
         "plain_value": plain_value, "plain_ms_per_step": plain_s * 1e3 / args.plain_steps,
         "plain_steps": args.plain_steps, "plain_path": "one cs_sim per cell, its launches per step (own stream)",
         "speedup": batched_value / plain_value,
-        "batched_hbm_frac": batched_value * b_alg / (HBM_PEAK_GBS * 1e9), "bytes_per_voxel_step": b_alg,
+        "bytes_per_voxel_step": b_alg,
+        # the batched kernel skips the (tile, plane) pieces of a cell that are empty and were empty, so it moves
+        # fewer bytes than b_alg per voxel-step: this dense-equivalent rate is not an HBM figure
+        "batched_dense_equivalent_gbs": batched_value * b_alg / 1e9,
+        "batched_hbm": _ensemble_hbm(args, ens_ms / args.ensemble_steps),
         "cme_events": int(c[5]), "exceptions": int(c[1]),
     }
+
+
+def _ensemble_hbm(args, ms_per_step: float):
+    """HBM bytes the batched step moved (rocprofv3 FETCH_SIZE / WRITE_SIZE per step, scaled by the copy
+    calibration, from --ensemble-traffic-json) over this run's time per step, against the 8 TB/s peak."""
+    path = args.ensemble_traffic_json
+    if not path or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        tj = json.load(f)
+    if tj.get("cells") != args.ensemble_cells:
+        return None
+    gbs = tj["bytes_per_step"] / (ms_per_step * 1e-3) / 1e9
+    return {"achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+            "traffic_per_step": tj["bytes_per_step"], "source": os.path.relpath(path, ROOT)}
 
 
 def precision_leg(steps: int) -> dict:
@@ -140,6 +166,10 @@ def main() -> int:
                     help="replicate cells of the batched-vs-plain comparison (0 = skip it)")
     ap.add_argument("--ensemble-steps", type=int, default=200, help="timed steps of the batched ensemble")
     ap.add_argument("--plain-steps", type=int, default=10, help="timed steps of the per-cell plain launches")
+    ap.add_argument("--ensemble-only", action="store_true",
+                    help="run only the batched-ensemble leg and print its record (for profiling it)")
+    ap.add_argument("--ensemble-traffic-json", default=os.path.join(ROOT, "profiles", "ensemble_512_traffic.json"),
+                    help="HBM bytes per batched step measured with rocprofv3 --pmc (FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--graph-block", type=int, default=100, help="steps per captured hipGraph")
     ap.add_argument("--precision", type=int, default=0, metavar="STEPS",
                     help="also run the 512^3 precision-stress workload for STEPS steps with the f64 audit of the "
@@ -157,6 +187,10 @@ def main() -> int:
         return 2
 
     from cell_simulation_amd import configs, default_itime_schedule
+
+    if args.ensemble_only:
+        print(json.dumps({"batched": ensemble_leg(args)}), flush=True)
+        return 0
 
     log(f"building large_cell edge={args.edge}")
     sim = configs.large_cell(seed=1234, edge=args.edge)

@@ -58,6 +58,26 @@ struct ResVoxel {
     int32_t off;  // occupancy[v] - occupied slots (u32 arithmetic in the reference)
 };
 
+// Occupancy summary of one lattice buffer, kept for the wave kernel: byte ((y / kSumTY) * tiles_z + z / kSumTZ)
+// * planes + x is non-zero whenever that wave tile's voxels of plane x may hold particles (a non-zero byte may be
+// stale; a zero one never is). Every writer of particles into a buffer sets the byte; the wave kernel also
+// clears the bytes of the planes it leaves empty. Plain byte stores: within a step a byte has one writer (the
+// work item owning the plane; the passes before and after the step kernel run as kernels of their own), so no
+// two writers share a byte, while bits packed in a word would need atomics coherent across the XCDs' L2s.
+constexpr uint32_t kSumTY = 4, kSumTZ = 64;  // the wave tile (kWY x kWZ)
+struct SumRef {
+    uint8_t *bytes;  // null: no summary kept for this buffer
+    uint32_t planes, tiles_z, R1, R2;  // planes = R0
+};
+
+__device__ __forceinline__ void sum_mark(const SumRef &s, uint32_t v)
+{
+    if (s.bytes == nullptr)
+        return;
+    const uint32_t z = v % s.R2, t = v / s.R2, y = t % s.R1, x = t / s.R1;
+    s.bytes[((y / kSumTY) * s.tiles_z + z / kSumTZ) * s.planes + x] = 1u;
+}
+
 struct StepArgs {
     const uint8_t *src;
     uint8_t *dst;
@@ -88,6 +108,7 @@ struct StepArgs {
     const uint32_t *res_v;     // [res_n] reservoir voxel indices, ascending
     ResVoxel *res_st;          // [res_n]
     uint32_t res_n;
+    SumRef sum_dst;            // summary of dst, marked by the passes after the step kernel
 };
 
 __device__ __forceinline__ bool voxel_bit(const uint32_t *bits, uint32_t v)
@@ -811,6 +832,8 @@ __device__ __noinline__ void fixup_tight(const StepArgs &a, uint32_t nt_all)
         d_near += near;
         d_bad += bad;
         store_slots<S>(a.dst, v, out);
+        if (count_nonzero(out) != 0u)
+            sum_mark(a.sum_dst, v);
     }
     counter_add(a.counters, CS_CNT_FIRED, d_fired);
     counter_add(a.counters, CS_CNT_ROLLBACK, d_rolled);
@@ -939,6 +962,8 @@ __global__ __launch_bounds__(64) void reservoir_react_kernel(StepArgs a)
                     e.gr = gr;
                 e.off = occ + docc - int32_t(j);
                 store_slots<S>(a.dst, v, out);
+                if (j != 0u)
+                    sum_mark(a.sum_dst, v);
             }
             a.res_st[i] = e;
             const uint32_t bit = 1u << (v & 31u);
@@ -1619,7 +1644,9 @@ struct WaveArgs {
     uint32_t nitems;        // work items: kWavesPerBlock tiles stacked in y over an x-range
     uint32_t *queue;        // [8] work counters, one per XCD; zero at the start of every step
     const uint2 *items;     // [nitems] tz | by << 16, x_begin | x_end << 16; by decreasing expected time
+    const uint8_t *sum_src;   // occupancy summary of src (SumRef layout, geometry of StepArgs::sum_dst), or null
 };
+static_assert(kWY == int(kSumTY) && kWZ == int(kSumTZ), "the occupancy summary counts wave tiles");
 
 template <int S>
 struct WaveLds {
@@ -1848,6 +1875,11 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
 {
     using Vec = typename SlotVec<S>::T;
     using HVec = typename HeadVec<S>::T;
+    // the tile and range are the same on every lane: scalar registers
+    tz = __builtin_amdgcn_readfirstlane(tz);
+    wy = __builtin_amdgcn_readfirstlane(wy);
+    xb = __builtin_amdgcn_readfirstlane(xb);
+    xe = __builtin_amdgcn_readfirstlane(xe);
     const int l = int(__lane_id());
     const int y0 = int(wy) * kWY, z0 = int(tz) * kWZ;
     const int R0 = int(a.R0), R1 = int(a.R1), R2 = int(a.R2);
@@ -1895,6 +1927,56 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
     if (xrow_ok)
         xm = xb - 1 >= 0 ? uint32_t(xb - 1) % uint32_t(xrow) : uint32_t(xrow - 1);
     const uint32_t xm_zero = a.mod_zero;
+
+    // Occupancy summaries: a plane whose records (this tile's voxels and the four face-neighbour tiles') hold
+    // no particle is neither loaded nor listed; a plane gathered empty where the destination was empty
+    // already is not stored; a march with neither is skipped. Bit i of actm: plane xb - 1 + i; of oldm: xb + i.
+    const bool summ = w.sum_src != nullptr && a.sum_dst.bytes != nullptr && xe - xb <= 60;
+    uint64_t actm = ~0ull, oldm = ~0ull;
+    uint8_t *own_sum = nullptr;
+    if (summ) {
+        const uint32_t P = a.sum_dst.planes;
+        const int xa = xb - 1 + l;
+        bool any = false;
+        if (xa >= 0 && xa <= xe && xa < int(P)) {
+#pragma unroll
+            for (int q = 0; q < 5; ++q) {
+                const int ty = int(wy) + (q == 1 ? -1 : (q == 2 ? 1 : 0));
+                const int tq = int(tz) + (q == 3 ? -1 : (q == 4 ? 1 : 0));
+                if (ty >= 0 && ty < int(w.wtiles_y) && tq >= 0 && tq < int(w.tiles_z))
+                    any = any || w.sum_src[(uint32_t(ty) * w.tiles_z + uint32_t(tq)) * P + uint32_t(xa)] != 0u;
+            }
+        }
+        actm = __ballot(any);
+        own_sum = a.sum_dst.bytes + (wy * w.tiles_z + tz) * P;
+        oldm = __ballot(xb + l < xe && own_sum[xb + l] != 0u);
+    }
+    auto pact = [&](int X) -> bool {
+        const int k = X - (xb - 1);
+        return k >= 0 && k < 64 && ((actm >> k) & 1ull) != 0ull;
+    };
+    auto pold = [&](int Q) -> bool {
+        const int k = Q - xb;
+        return k >= 0 && k < 64 && ((oldm >> k) & 1ull) != 0ull;
+    };
+    if (summ && w.has_zero_order == 0u && (actm | oldm) == 0ull) {
+        // nothing to do; an empty voxel's zero-propensity draw still fires at seed 0 (itime + x*y + z == 0)
+        uint64_t im_r[kWY];
+#pragma unroll
+        for (int r = 0; r < kWY; ++r)
+            im_r[r] = __ballot((inl >> r) & 1u);
+        if (l == 0) {
+            uint32_t nf = 0u;
+            for (int r = 0; r < kWY; ++r)
+                for (int Q = xb; Q < xe; ++Q) {
+                    const uint32_t zfire = 0u - a.itime - uint32_t(Q) * uint32_t(y0 + r) - uint32_t(z0);
+                    if (zfire < uint32_t(kWZ))
+                        nf += uint32_t(im_r[r] >> zfire) & 1u;
+                }
+            tc.fired += nf;
+        }
+        return;
+    }
 
     // labels of planes xb-2 and xb-1; slots of plane xb-1 and labels of plane xb into the first register set
     auto store_lab = [&](int X, const uint32_t (&r)[kWLabLane]) {
@@ -1972,9 +2054,15 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
         // ---- prefetch plane X+1's slots and plane X+2's labels (clamped; unused past the chunk)
         {
             const uint32_t sb = slot_base(X + 1), lb = lab_base(X + 2);
+            if (!summ || pact(X + 1)) {
 #pragma unroll
-            for (int k = 0; k < kWBatches; ++k)
-                fill.rec[k] = load_head<S>(a.src, sb + off[k]);
+                for (int k = 0; k < kWBatches; ++k)
+                    fill.rec[k] = load_head<S>(a.src, sb + off[k]);
+            } else {
+#pragma unroll
+                for (int k = 0; k < kWBatches; ++k)
+                    fill.rec[k] = HVec{};  // no particle in any record of that plane
+            }
 #pragma unroll
             for (int i = 0; i < kWLabLane; ++i)
                 fill.lab[i] = w.regp[lb + loff[i]];
@@ -1995,7 +2083,7 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
 
         // ---- T0: list the occupied records of plane X
         uint32_t n = 0u;
-        const bool hop_plane = X >= 0 && X < R0 && X <= xe;
+        const bool hop_plane = X >= 0 && X < R0 && X <= xe && (!summ || pact(X));
         // on the planes beside the x-range only the interior rows matter: a particle there reaches the
         // tile with a hop along x or not at all
         const int batches = (X == xb - 1 || X == xe) ? kWY : kWBatches;
@@ -2270,15 +2358,26 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
                 }
             }
             wave_sync();
+            // an empty output over an empty destination needs no store
+            const bool old_q = summ && pold(Q);
+            bool nz = false;
+            if (!summ || ns != 0u || old_q) {
 #pragma unroll
-            for (int r = 0; r < kWY; ++r) {
-                if ((inl >> r) & 1u) {
-                    const bool need = (needm >> r) & 1u;
-                    const Vec v = to_vec<S>(from_head<S>(need ? L.hop_slots[sidx[r]] : HVec{}));
-                    const bool deferred = need && L.slow[sidx[r]] != 0u;
-                    if (!deferred)
-                        *reinterpret_cast<Vec *>(a.dst + size_t(off[r] + uint32_t(Q) * a.R12) * uint32_t(S)) = v;
+                for (int r = 0; r < kWY; ++r) {
+                    if ((inl >> r) & 1u) {
+                        const bool need = (needm >> r) & 1u;
+                        const Vec v = to_vec<S>(from_head<S>(need ? L.hop_slots[sidx[r]] : HVec{}));
+                        const bool deferred = need && L.slow[sidx[r]] != 0u;
+                        if (!deferred)
+                            *reinterpret_cast<Vec *>(a.dst + size_t(off[r] + uint32_t(Q) * a.R12) * uint32_t(S)) = v;
+                        nz = nz || deferred || vec_nonzero<S>(v);
+                    }
                 }
+            }
+            if (summ) {
+                const bool new_q = __ballot(nz) != 0ull;
+                if (new_q != old_q && l == 0)
+                    own_sum[Q] = uint8_t(new_q);
             }
         }
 #pragma unroll
@@ -2344,7 +2443,7 @@ void rdme_step_wave(StepArgs a, WaveArgs w)
             if (threadIdx.x == 0)
                 s_item = atomicAdd(&w.queue[k], 1u);
             __syncthreads();
-            const uint32_t it = s_item;
+            const uint32_t it = __builtin_amdgcn_readfirstlane(s_item);
             __syncthreads();
             if (it >= ngk)
                 break;
@@ -2394,7 +2493,8 @@ __device__ __forceinline__ void kahan_add(float &s, float &c, float x)
 }
 
 template <int S>
-__device__ void cme_cell(const CmeArgs &a, int32_t *counts, uint8_t *lattice, uint32_t seed, uint32_t itime)
+__device__ void cme_cell(const CmeArgs &a, int32_t *counts, uint8_t *lattice, const SumRef &sm, uint32_t seed,
+                         uint32_t itime)
 {
     const double tau = double(a.tau);
     double t = 0.0, max_rel = 0.0;
@@ -2472,6 +2572,7 @@ __device__ void cme_cell(const CmeArgs &a, int32_t *counts, uint8_t *lattice, ui
                 if (occ < uint32_t(S)) {
                     put_byte_dyn(sl, occ, sp);
                     store_slots<S>(lattice, v, sl);
+                    sum_mark(sm, v);
                     placed = true;
                 }
             }
@@ -2496,11 +2597,11 @@ __device__ void cme_cell(const CmeArgs &a, int32_t *counts, uint8_t *lattice, ui
 
 // One simulation's CME phase: a single lane (the pool is one well-mixed compartment).
 template <int S>
-__global__ __launch_bounds__(64) void cme_kernel(CmeArgs a, int32_t *counts, uint8_t *lattice, uint32_t seed,
-                                                 uint32_t itime)
+__global__ __launch_bounds__(64) void cme_kernel(CmeArgs a, int32_t *counts, uint8_t *lattice, SumRef sm,
+                                                 uint32_t seed, uint32_t itime)
 {
     if (threadIdx.x == 0)
-        cme_cell<S>(a, counts, lattice, seed, itime);
+        cme_cell<S>(a, counts, lattice, sm, seed, itime);
 }
 
 // Per-region species histogram of the lattice (statistics; the reference keeps global counts only,
@@ -2547,6 +2648,8 @@ struct EnsCell {
     uint32_t itime_seed;       // itime(step) = low 32 bits of SplitMix64((itime_seed << 32) + step)
     int32_t cme;               // the cell's well-mixed pool (index of its CmeArgs), -1 = none
     uint32_t cme_counts_off, cme_seed;
+    uint32_t sum_off, sum_planes;  // the cell's occupancy summary: byte offset in each summary buffer, planes per
+                                   // tile (R0); 0: none (cells with zero-order reactions)
 };
 
 struct EnsArgs {
@@ -2562,7 +2665,13 @@ struct EnsArgs {
     uint32_t *fix_ctl, *fix_list, *fix_F, *fix_W, *fix_O;
     uint8_t *fix_L;
     uint32_t max_tight;
+    uint8_t *sum_src, *sum_dst;   // occupancy summaries of src and dst (every cell's, at EnsCell::sum_off)
 };
+
+__device__ __forceinline__ SumRef cell_sum(uint8_t *bytes, const EnsCell &c)
+{
+    return SumRef{c.sum_planes != 0u ? bytes + c.sum_off : nullptr, c.sum_planes, c.tiles_z, c.R1, c.R2};
+}
 
 __device__ __forceinline__ uint32_t splitmix_low32(uint64_t x)
 {
@@ -2611,6 +2720,7 @@ __device__ __forceinline__ StepArgs cell_step_args(const EnsArgs &e, const EnsCe
     a.res_v = nullptr;
     a.res_st = nullptr;
     a.res_n = 0u;
+    a.sum_dst = cell_sum(e.sum_dst, c);
     return a;
 }
 
@@ -2638,6 +2748,7 @@ __device__ __forceinline__ WaveArgs cell_wave_args(const EnsArgs &e, const EnsCe
     w.nitems = 0u;
     w.queue = nullptr;
     w.items = nullptr;
+    w.sum_src = c.sum_planes != 0u ? e.sum_src + c.sum_off : nullptr;
     return w;
 }
 
@@ -2652,7 +2763,8 @@ __global__ __launch_bounds__(64) void ens_prologue(EnsArgs e, const CmeArgs *cme
     const uint32_t it = splitmix_low32((uint64_t(cell.itime_seed) << 32) + e.queue[1]);
     e.itime[c] = it;
     if (cell.cme >= 0)
-        cme_cell<S>(cme[cell.cme], cme_counts + cell.cme_counts_off, e.src + cell.lat_off, cell.cme_seed, it);
+        cme_cell<S>(cme[cell.cme], cme_counts + cell.cme_counts_off, e.src + cell.lat_off, cell_sum(e.sum_src, cell),
+                    cell.cme_seed, it);
 }
 
 // The lattice step of every cell: persistent blocks take (cell, group, x-chunk) items from the queue
@@ -2829,6 +2941,30 @@ static std::vector<float> propensity_bounds(const Model &m, uint32_t S)
         out[c] = f;
     }
     return out;
+}
+
+// Host form of the occupancy summary (SumRef) of one compacted lattice: `out` (zeroed, tiles x R0 bytes) gets a 1
+// for every wave tile and plane holding a particle.
+static void summary_bytes(const uint8_t *lat, const uint32_t res[3], uint32_t S, uint32_t tiles_z, uint8_t *out)
+{
+    for (uint32_t x = 0; x < res[0]; ++x)
+        for (uint32_t y = 0; y < res[1]; ++y) {
+            const uint8_t *row = lat + (size_t(x) * res[1] + y) * res[2] * S;
+            uint8_t *base = out + size_t(y / kSumTY) * tiles_z * res[0] + x;
+            for (uint32_t z = 0; z < res[2]; ++z)
+                if (row[size_t(z) * S] != 0u)
+                    base[size_t(z / kSumTZ) * res[0]] = 1u;
+        }
+}
+
+// Occupancy summaries are kept when at least a fifth of the (wave tile, plane) entries start empty: below that
+// the tests cost more than the planes they skip.
+static bool summary_pays(const std::vector<uint8_t> &bytes)
+{
+    size_t empty = 0;
+    for (uint8_t b : bytes)
+        empty += b == 0u;
+    return !bytes.empty() && empty * 5u >= bytes.size();
 }
 
 // Checks that the model fits the device encodings (one-byte labels and slots, int8 stoichiometry).
@@ -3174,6 +3310,27 @@ Device::Device(const Model &m, int hip_device) : dev_(hip_device)
     }
 
     hip_check(hipMemcpyAsync(lat_[0], m.lattice().data(), N_ * S_, hipMemcpyHostToDevice, st), "upload lattice");
+    hip_check(hipMemsetAsync(lat_[1], 0, N_ * S_, st), "clear lattice");
+    // occupancy summaries of both buffers for the wave kernel (none with zero-order reactions, which can
+    // fire in any voxel)
+    summ_ = !simple_ && !tiled_ && !has_zero_order_;
+    std::vector<uint8_t> sum0;
+    if (summ_) {
+        const size_t nb = size_t(wtiles_y_) * wtiles_z_ * res_[0];
+        sum0.assign(nb, 0u);
+        summary_bytes(m.lattice().data(), res_, S_, wtiles_z_, sum0.data());
+        // they pay where tile planes are empty; a lattice with particles nearly everywhere keeps none
+        summ_ = summary_pays(sum0);
+        if (const char *x = std::getenv("CS_OCC_SUMMARY"))  // "on" / "off": keep them or not regardless
+            summ_ = std::string(x) == "on" ? true : (std::string(x) == "off" ? false : summ_);
+    }
+    if (summ_) {
+        const size_t nb = sum0.size();
+        for (int b = 0; b < 2; ++b)
+            hip_check(hipMalloc(&sum_[b], nb), "hipMalloc occupancy summary");
+        hip_check(hipMemcpyAsync(sum_[0], sum0.data(), nb, hipMemcpyHostToDevice, st), "upload occupancy summary");
+        hip_check(hipMemsetAsync(sum_[1], 0, nb, st), "clear occupancy summary");
+    }
     plan_wave_items(m.lattice().data());
     hip_check(hipMemcpyAsync(reg_, reg8.data(), N_, hipMemcpyHostToDevice, st), "upload regions");
     hip_check(hipMemcpyAsync(regp_, regp.data(), regp.size(), hipMemcpyHostToDevice, st), "upload padded regions");
@@ -3211,6 +3368,8 @@ Device::~Device()
     hipFree(res_st_);
     hipFree(res_vol_);
     hipFree(view_);
+    hipFree(sum_[0]);
+    hipFree(sum_[1]);
     if (stream_)
         hipStreamDestroy(static_cast<hipStream_t>(stream_));
 }
@@ -3253,6 +3412,8 @@ void Device::step(uint32_t itime)
     a.res_v = res_v_;
     a.res_st = static_cast<ResVoxel *>(res_st_);
     a.res_n = res_n_;
+    a.sum_dst = SumRef{summ_ ? sum_[cur_ ^ 1] : nullptr, res_[0], wtiles_z_, res_[1], res_[2]};
+    const SumRef sum_src{summ_ ? sum_[cur_] : nullptr, res_[0], wtiles_z_, res_[1], res_[2]};
     hipStream_t st = static_cast<hipStream_t>(stream_);
     if (has_cme_) {
         const uint8_t *cb = static_cast<const uint8_t *>(cme_tab_);
@@ -3272,9 +3433,9 @@ void Device::step(uint32_t itime)
         c.audit_on = audit_on_ ? 1u : 0u;
         c.tau = tau_;
         switch (S_) {
-        case 4: hipLaunchKernelGGL(cme_kernel<4>, dim3(1), dim3(64), 0, st, c, cme_counts_, lat_[cur_], cme_seed_, itime); break;
-        case 8: hipLaunchKernelGGL(cme_kernel<8>, dim3(1), dim3(64), 0, st, c, cme_counts_, lat_[cur_], cme_seed_, itime); break;
-        default: hipLaunchKernelGGL(cme_kernel<16>, dim3(1), dim3(64), 0, st, c, cme_counts_, lat_[cur_], cme_seed_, itime); break;
+        case 4: hipLaunchKernelGGL(cme_kernel<4>, dim3(1), dim3(64), 0, st, c, cme_counts_, lat_[cur_], sum_src, cme_seed_, itime); break;
+        case 8: hipLaunchKernelGGL(cme_kernel<8>, dim3(1), dim3(64), 0, st, c, cme_counts_, lat_[cur_], sum_src, cme_seed_, itime); break;
+        default: hipLaunchKernelGGL(cme_kernel<16>, dim3(1), dim3(64), 0, st, c, cme_counts_, lat_[cur_], sum_src, cme_seed_, itime); break;
         }
         hip_check(hipGetLastError(), "cme launch");
     }
@@ -3309,6 +3470,7 @@ void Device::step(uint32_t itime)
         w.nitems = n_items_;
         w.queue = fix_ctl_ + kFixQueue;
         w.items = static_cast<const uint2 *>(items_);
+        w.sum_src = sum_src.bytes;
         const uint32_t blocks = std::min(n_items_, wave_blocks_);
         const size_t shm = size_t(tab_words_) * 4u;
         switch (S_) {
@@ -3447,6 +3609,13 @@ void Device::write_lattice(const uint8_t *in)
     hip_check(hipSetDevice(dev_), "hipSetDevice");
     hipStream_t st = static_cast<hipStream_t>(stream_);
     hip_check(hipMemcpyAsync(lat_[cur_], in, N_ * S_, hipMemcpyHostToDevice, st), "write lattice");
+    std::vector<uint8_t> bytes;
+    if (summ_) {
+        bytes.assign(size_t(wtiles_y_) * wtiles_z_ * res_[0], 0u);
+        summary_bytes(in, res_, S_, wtiles_z_, bytes.data());
+        hip_check(hipMemcpyAsync(sum_[cur_], bytes.data(), bytes.size(), hipMemcpyHostToDevice, st),
+                  "upload occupancy summary");
+    }
     plan_wave_items(in);
     hip_check(hipStreamSynchronize(st), "write lattice");
 }
@@ -3701,6 +3870,13 @@ Ensemble::Ensemble(const std::vector<const Model *> &cells, const std::vector<ui
         }
         if (tab_words.size() % 2u)
             tab_words.push_back(0u);  // keep every blob 8-byte aligned
+        // the cell's occupancy summaries (none with zero-order reactions, which can fire in any voxel)
+        const uint64_t nb = uint64_t((d.R1 + kWY - 1) / kWY) * ((d.R2 + kWZ - 1) / kWZ) * d.R0;
+        if (sum_total_ + nb >= (uint64_t(1) << 32))
+            throw Error(CS_ERR_UNSUPPORTED, "occupancy summaries of the ensemble exceed 2^32 bytes");
+        d.sum_off = uint32_t(sum_total_);
+        d.sum_planes = b.has_zero_order ? 0u : d.R0;
+        sum_total_ += nb;
     }
     total_vox_ = vox;
     cells_.resize(sizeof(EnsCell) * n);
@@ -3752,6 +3928,25 @@ Ensemble::Ensemble(const std::vector<const Model *> &cells, const std::vector<ui
                                     counters_, audit_, false));
         ++k;
     }
+    const size_t sum_bytes = std::max<uint64_t>(sum_total_, 1);
+    for (int b = 0; b < 2; ++b)
+        hip_check(hipMalloc(&sum_[b], sum_bytes), "hipMalloc occupancy summary");
+    hip_check(hipMemsetAsync(lat_[1], 0, total_vox_ * S_, st), "clear ensemble lattice");
+    hip_check(hipMemsetAsync(sum_[1], 0, sum_bytes, st), "clear occupancy summary");
+    std::vector<uint8_t> sum0(sum_bytes, 0u);
+    for (uint32_t c = 0; c < n; ++c)
+        if (desc[c].sum_planes != 0u) {
+            const uint32_t tzn = (desc[c].R2 + kWZ - 1) / kWZ;
+            const size_t nb = size_t((desc[c].R1 + kWY - 1) / kWY) * tzn * desc[c].R0;
+            std::vector<uint8_t> cs(nb, 0u);
+            summary_bytes(cells[c]->lattice().data(), &res_[3 * size_t(c)], S_, tzn, cs.data());
+            if (summary_pays(cs))
+                std::memcpy(sum0.data() + desc[c].sum_off, cs.data(), nb);
+            else
+                desc[c].sum_planes = 0u;  // the cell keeps no summary (see summary_pays)
+        }
+    std::memcpy(cells_.data(), desc.data(), cells_.size());
+    hip_check(hipMemcpyAsync(sum_[0], sum0.data(), sum_bytes, hipMemcpyHostToDevice, st), "upload occupancy summary");
     for (uint32_t c = 0; c < n; ++c) {
         const Model &m = *cells[c];
         hip_check(hipMemcpyAsync(lat_[0] + lat_off_[c], m.lattice().data(), m.n_voxels() * S_, hipMemcpyHostToDevice, st),
@@ -3798,7 +3993,7 @@ Ensemble::~Ensemble()
         hipStreamSynchronize(static_cast<hipStream_t>(stream_));
     destroy_graph();
     void *bufs[] = {lat_[0], lat_[1], reg_, regp_, tab_, cells_dev_, cme_args_dev_, cme_tab_, cme_counts_, itime_, queue_,
-                    counters_, audit_, fix_ctl_, fix_list_, fix_F_, fix_W_, fix_L_, fix_O_};
+                    counters_, audit_, fix_ctl_, fix_list_, fix_F_, fix_W_, fix_L_, fix_O_, sum_[0], sum_[1]};
     for (void *b : bufs)
         if (b)
             (void)hipFree(b);
@@ -3858,6 +4053,8 @@ void Ensemble::enqueue_step()
     e.batch = tiling_.batch;
     e.src = lat_[cur_];
     e.dst = lat_[cur_ ^ 1];
+    e.sum_src = sum_[cur_];
+    e.sum_dst = sum_[cur_ ^ 1];
     e.reg = reg_;
     e.regp = static_cast<const uint32_t *>(regp_);
     e.tab = tab_;
@@ -3982,18 +4179,26 @@ float Ensemble::autotune(uint32_t trial_steps)
     // snapshot: current lattice, well-mixed counts, step counter, event counters
     const size_t lat_bytes = total_vox_ * S_;
     const size_t nc_total = std::max<size_t>(1, cme_counts_off_.empty() ? 1 : cme_counts_off_.back() + cell_nc_.back());
+    const size_t sum_bytes = std::max<uint64_t>(sum_total_, 1);
     uint8_t *lat_snap = nullptr;
     int32_t *cme_snap = nullptr;
     uint32_t *q_snap = nullptr;
     unsigned long long *cnt_snap = nullptr;
-    hip_check(hipMalloc(&lat_snap, lat_bytes), "hipMalloc snapshot");
+    uint8_t *sum_snap = nullptr;
+    hip_check(hipMalloc(&lat_snap, 2 * lat_bytes), "hipMalloc snapshot");
+    hip_check(hipMalloc(&sum_snap, 2 * sum_bytes), "hipMalloc snapshot");
     hip_check(hipMalloc(&cme_snap, nc_total * sizeof(int32_t)), "hipMalloc snapshot");
     hip_check(hipMalloc(&q_snap, 2 * sizeof(uint32_t)), "hipMalloc snapshot");
     hip_check(hipMalloc(&cnt_snap, kNumCounters * sizeof(unsigned long long)), "hipMalloc snapshot");
     const int cur0 = cur_;
     const uint64_t steps0 = steps_;
+    // both buffers: the one a step writes is also the one whose emptiness its summary vouches for
     auto save = [&] {
-        hip_check(hipMemcpyAsync(lat_snap, lat_[cur0], lat_bytes, hipMemcpyDeviceToDevice, st), "snapshot");
+        for (int b = 0; b < 2; ++b) {
+            hip_check(hipMemcpyAsync(lat_snap + b * lat_bytes, lat_[b], lat_bytes, hipMemcpyDeviceToDevice, st), "snapshot");
+            hip_check(hipMemcpyAsync(sum_snap + b * sum_bytes, sum_[b], sum_bytes,
+                                     hipMemcpyDeviceToDevice, st), "snapshot");
+        }
         hip_check(hipMemcpyAsync(cme_snap, cme_counts_, nc_total * sizeof(int32_t), hipMemcpyDeviceToDevice, st), "snapshot");
         hip_check(hipMemcpyAsync(q_snap, queue_, 2 * sizeof(uint32_t), hipMemcpyDeviceToDevice, st), "snapshot");
         hip_check(hipMemcpyAsync(cnt_snap, counters_, kNumCounters * sizeof(unsigned long long), hipMemcpyDeviceToDevice, st),
@@ -4003,7 +4208,11 @@ float Ensemble::autotune(uint32_t trial_steps)
         synchronize();
         cur_ = cur0;
         steps_ = steps0;
-        hip_check(hipMemcpyAsync(lat_[cur0], lat_snap, lat_bytes, hipMemcpyDeviceToDevice, st), "restore");
+        for (int b = 0; b < 2; ++b) {
+            hip_check(hipMemcpyAsync(lat_[b], lat_snap + b * lat_bytes, lat_bytes, hipMemcpyDeviceToDevice, st), "restore");
+            hip_check(hipMemcpyAsync(sum_[b], sum_snap + b * sum_bytes, sum_bytes,
+                                     hipMemcpyDeviceToDevice, st), "restore");
+        }
         hip_check(hipMemcpyAsync(cme_counts_, cme_snap, nc_total * sizeof(int32_t), hipMemcpyDeviceToDevice, st), "restore");
         hip_check(hipMemcpyAsync(queue_, q_snap, 2 * sizeof(uint32_t), hipMemcpyDeviceToDevice, st), "restore");
         hip_check(hipMemcpyAsync(counters_, cnt_snap, kNumCounters * sizeof(unsigned long long), hipMemcpyDeviceToDevice, st),
@@ -4031,6 +4240,7 @@ float Ensemble::autotune(uint32_t trial_steps)
             }
     set_tiling(best);
     (void)hipFree(lat_snap);
+    (void)hipFree(sum_snap);
     (void)hipFree(cme_snap);
     (void)hipFree(q_snap);
     (void)hipFree(cnt_snap);

@@ -115,6 +115,8 @@ private:
     std::vector<uint32_t> res_vh_; // host copies: voxel indices and their region labels
     std::vector<uint8_t> res_reg_;
     uint8_t *res_vol_ = nullptr;   // [N] reservoir species (reservoir models only), for the view
+    bool summ_ = false;            // occupancy summaries kept (wave kernel, no zero-order reactions)
+    uint8_t *sum_[2] = {nullptr, nullptr};  // summary of lat_[0], lat_[1]: a byte per wave tile and plane
     float *view_ = nullptr;        // [N] view output, allocated on first use
 };
 

@@ -174,7 +174,7 @@ def test_region_counts_match_lattice(oracle_mod):
     assert np.array_equal(sim.species_counts()[1:], exp.sum(axis=0)[1:])
 
 
-@pytest.mark.parametrize("kernel", ["wave", "wave-overflow", "tiled", "simple"])
+@pytest.mark.parametrize("kernel", ["wave", "wave-overflow", "wave-summary", "tiled", "simple"])
 def test_dense_capacity_replay(oracle_mod, kernel):
     """S=4 at high density: claims into full voxels are rejected or alias into the next voxel's slot 0
     (rdme.wgsl:30-40, 62). The tight neighbourhoods are replayed in canonical order, so the lattice stays
@@ -187,6 +187,8 @@ def test_dense_capacity_replay(oracle_mod, kernel):
         os.environ["CS_STEP_KERNEL"] = kernel.split("-")[0]
         if kernel == "wave-overflow":
             os.environ["CS_WAVE_EVENT_CAP"] = "2"  # most planes overflow and take the global form
+        if kernel == "wave-summary":
+            os.environ["CS_OCC_SUMMARY"] = "on"  # the replay's outputs must mark the occupancy summary
         try:
             sim = _walled_model(20, 4, dens, seed=31) if walled else _dense_model(20, 4, dens, seed=31)
             orc = oracle_mod.Oracle.from_simulation(sim)
@@ -194,6 +196,7 @@ def test_dense_capacity_replay(oracle_mod, kernel):
         finally:
             os.environ.pop("CS_STEP_KERNEL", None)
             os.environ.pop("CS_WAVE_EVENT_CAP", None)
+            os.environ.pop("CS_OCC_SUMMARY", None)
         compared = 0
         for t, it in enumerate(default_itime_schedule(31, nsteps)):
             sim.step(t, int(it), 0)
@@ -276,11 +279,13 @@ def _run_with_env(env, build, seed, nsteps):
     return sim.lattice(), sim.counters()
 
 
-@pytest.mark.parametrize("kernel", ["wave", "tiled"])
+@pytest.mark.parametrize("kernel", ["wave", "wave-summary", "tiled"])
 @pytest.mark.parametrize("planes", ["1", "3", "16", "1000"])
 def test_tiled_kernel_matches_simple_kernel(planes, kernel):
     """The wave-march and block-tiled kernels give the same lattices as the one-lane-per-voxel kernel for
-    any chunk length along x, on a lattice whose y and z extents are not multiples of the tiles (4 x 64)."""
+    any chunk length along x, on a lattice whose y and z extents are not multiples of the tiles (4 x 64).
+    "wave-summary" keeps the occupancy summaries (planes skipped when their tiles are empty) although this
+    lattice would not."""
     announce(f"{kernel} (planes={planes}) vs simple, 37x30x70 S=8, 40 steps")
 
     def build():
@@ -299,7 +304,9 @@ def test_tiled_kernel_matches_simple_kernel(planes, kernel):
         sim.add_reaction(["C"], ["A", "B"], 9.0)
         return sim
 
-    a, ca = _run_with_env({"CS_STEP_KERNEL": kernel, "CS_TILE_PLANES": planes}, build, 21, 40)
+    env = {"CS_STEP_KERNEL": kernel.split("-")[0], "CS_TILE_PLANES": planes}
+    env["CS_OCC_SUMMARY"] = "on" if kernel == "wave-summary" else "off"
+    a, ca = _run_with_env(env, build, 21, 40)
     b, cb = _run_with_env({"CS_STEP_KERNEL": "simple"}, build, 21, 40)
     assert np.array_equal(a, b)
     assert np.array_equal(ca[:5], cb[:5])

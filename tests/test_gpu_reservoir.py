@@ -63,16 +63,19 @@ def _check_step(sim, orc, tag):
     assert np.array_equal(sim.species_counts()[1:], orc.stat[1:]), (tag, sim.species_counts(), orc.stat)
 
 
-@pytest.mark.parametrize("kernel", ["wave", "tiled", "simple"])
+@pytest.mark.parametrize("kernel", ["wave", "wave-summary", "tiled", "simple"])
 def test_reservoir_parity(oracle_mod, kernel):
     announce(f"reservoir model 20^3 S=8, 400 reservoir sites ({kernel}), step by step up to the first alias")
-    os.environ["CS_STEP_KERNEL"] = kernel
+    os.environ["CS_STEP_KERNEL"] = kernel.split("-")[0]
+    if kernel == "wave-summary":
+        os.environ["CS_OCC_SUMMARY"] = "on"  # the reservoir pass must mark the occupancy summary
     try:
         sim = _reservoir_model(20, 8, seed=23)
         orc = oracle_mod.Oracle.from_simulation(sim)
         sim.prepare_for_gpu(0)
     finally:
         os.environ.pop("CS_STEP_KERNEL", None)
+        os.environ.pop("CS_OCC_SUMMARY", None)
     S = 8
     res = sim.reservoir().ravel()
     N = res.size
