@@ -1868,6 +1868,56 @@ struct WaveStaged {
     uint32_t lab[kWLabLane];
 };
 
+// The occupancy summaries of one march (SumRef): bit i of act is plane xb - 1 + i, set when a record of that plane
+// (this tile's voxels or the four face-neighbour tiles') may hold a particle; bit i of old is plane xb + i, set
+// when the destination may hold one there. One byte load per lane and tile, then ballots.
+struct MarchSummary {
+    uint64_t act, old;
+};
+
+__device__ __forceinline__ MarchSummary march_summary(const StepArgs &a, const WaveArgs &w, uint32_t tz, uint32_t wy,
+                                                      int xb, int xe)
+{
+    const int l = int(__lane_id());
+    const uint32_t P = a.sum_dst.planes;
+    const int xa = xb - 1 + l;
+    bool any = false;
+    if (xa >= 0 && xa <= xe && xa < int(P)) {
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+            const int ty = int(wy) + (q == 1 ? -1 : (q == 2 ? 1 : 0));
+            const int tq = int(tz) + (q == 3 ? -1 : (q == 4 ? 1 : 0));
+            if (ty >= 0 && ty < int(w.wtiles_y) && tq >= 0 && tq < int(w.tiles_z))
+                any = any || w.sum_src[(uint32_t(ty) * w.tiles_z + uint32_t(tq)) * P + uint32_t(xa)] != 0u;
+        }
+    }
+    const uint8_t *own = a.sum_dst.bytes + (wy * w.tiles_z + tz) * P;
+    return MarchSummary{__ballot(any), __ballot(xb + l < xe && own[xb + l] != 0u)};
+}
+
+// A march with nothing to load and nothing to clear does nothing, except that an empty voxel's zero-propensity
+// draw still fires at seed 0 (itime + x*y + z == 0): counted here (lane 0) for the skipped planes.
+__device__ __forceinline__ void count_idle_fires(const StepArgs &a, uint32_t tz, uint32_t wy, int xb, int xe,
+                                                 TileCounts &tc)
+{
+    const int l = int(__lane_id());
+    const int y0 = int(wy) * kWY, z0 = int(tz) * kWZ;
+    uint64_t im[kWY];
+#pragma unroll
+    for (int r = 0; r < kWY; ++r)
+        im[r] = __ballot(y0 + r < int(a.R1) && z0 + l < int(a.R2));
+    if (l == 0) {
+        uint32_t nf = 0u;
+        for (int r = 0; r < kWY; ++r)
+            for (int Q = xb; Q < xe; ++Q) {
+                const uint32_t zfire = 0u - a.itime - uint32_t(Q) * uint32_t(y0 + r) - uint32_t(z0);
+                if (zfire < uint32_t(kWZ))
+                    nf += uint32_t(im[r] >> zfire) & 1u;
+            }
+        tc.fired += nf;
+    }
+}
+
 // One wave tile (tz, wy) of the lattice (a, w), marched through x-chunk `chunk`.
 template <int S>
 __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const StepArgs &a, const WaveArgs &w,
@@ -1935,21 +1985,10 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
     uint64_t actm = ~0ull, oldm = ~0ull;
     uint8_t *own_sum = nullptr;
     if (summ) {
-        const uint32_t P = a.sum_dst.planes;
-        const int xa = xb - 1 + l;
-        bool any = false;
-        if (xa >= 0 && xa <= xe && xa < int(P)) {
-#pragma unroll
-            for (int q = 0; q < 5; ++q) {
-                const int ty = int(wy) + (q == 1 ? -1 : (q == 2 ? 1 : 0));
-                const int tq = int(tz) + (q == 3 ? -1 : (q == 4 ? 1 : 0));
-                if (ty >= 0 && ty < int(w.wtiles_y) && tq >= 0 && tq < int(w.tiles_z))
-                    any = any || w.sum_src[(uint32_t(ty) * w.tiles_z + uint32_t(tq)) * P + uint32_t(xa)] != 0u;
-            }
-        }
-        actm = __ballot(any);
-        own_sum = a.sum_dst.bytes + (wy * w.tiles_z + tz) * P;
-        oldm = __ballot(xb + l < xe && own_sum[xb + l] != 0u);
+        const MarchSummary ms = march_summary(a, w, tz, wy, xb, xe);
+        actm = ms.act;
+        oldm = ms.old;
+        own_sum = a.sum_dst.bytes + (wy * w.tiles_z + tz) * a.sum_dst.planes;
     }
     auto pact = [&](int X) -> bool {
         const int k = X - (xb - 1);
@@ -1960,21 +1999,7 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
         return k >= 0 && k < 64 && ((oldm >> k) & 1ull) != 0ull;
     };
     if (summ && w.has_zero_order == 0u && (actm | oldm) == 0ull) {
-        // nothing to do; an empty voxel's zero-propensity draw still fires at seed 0 (itime + x*y + z == 0)
-        uint64_t im_r[kWY];
-#pragma unroll
-        for (int r = 0; r < kWY; ++r)
-            im_r[r] = __ballot((inl >> r) & 1u);
-        if (l == 0) {
-            uint32_t nf = 0u;
-            for (int r = 0; r < kWY; ++r)
-                for (int Q = xb; Q < xe; ++Q) {
-                    const uint32_t zfire = 0u - a.itime - uint32_t(Q) * uint32_t(y0 + r) - uint32_t(z0);
-                    if (zfire < uint32_t(kWZ))
-                        nf += uint32_t(im_r[r] >> zfire) & 1u;
-                }
-            tc.fired += nf;
-        }
+        count_idle_fires(a, tz, wy, xb, xe, tc);
         return;
     }
 
@@ -2666,6 +2691,7 @@ struct EnsArgs {
     uint8_t *fix_L;
     uint32_t max_tight;
     uint8_t *sum_src, *sum_dst;   // occupancy summaries of src and dst (every cell's, at EnsCell::sum_off)
+    const uint32_t *item_cell;    // [total_items] the cell of each work item
 };
 
 __device__ __forceinline__ SumRef cell_sum(uint8_t *bytes, const EnsCell &c)
@@ -2798,17 +2824,25 @@ __global__ __launch_bounds__(kWaveBlock) void rdme_step_ensemble(EnsArgs e)
         rr += gridDim.x;
         const uint32_t last = min(first + per, e.total_items);
         for (uint32_t item = first; item < last; ++item) {
-            uint32_t lo = 0, hi = e.n_cells;  // the cell whose items contain `item`
-            while (hi - lo > 1u) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (e.cells[mid].first_item <= item)
-                    lo = mid;
-                else
-                    hi = mid;
-            }
+            const uint32_t lo = __builtin_amdgcn_readfirstlane(e.item_cell[item]);
             const EnsCell &c = e.cells[lo];
             const StepArgs a = cell_step_args(e, c, e.itime[lo]);
             const WaveArgs w = cell_wave_args(e, c);
+            const uint32_t local = item - c.first_item;
+            const uint32_t tz = local % c.tiles_z, by = (local / c.tiles_z) % c.btiles_y;
+            const uint32_t chunk = local / (c.tiles_z * c.btiles_y);
+            const uint32_t wy = by * kWavesPerBlock + wv;
+            const int xb = int(chunk * e.xc), xe = min(int(c.R0), int((chunk + 1u) * e.xc));
+            // a wave whose march has nothing to do stops here; a group of such waves skips the table staging
+            bool busy = wy < c.wtiles_y;
+            if (busy && w.sum_src != nullptr && c.has_zero_order == 0u && xe - xb <= 60) {
+                const MarchSummary ms = march_summary(a, w, tz, wy, xb, xe);
+                busy = (ms.act | ms.old) != 0ull;
+                if (!busy)
+                    count_idle_fires(a, tz, wy, xb, xe, tc);
+            }
+            if (!__syncthreads_or(busy ? 1 : 0))
+                continue;
             for (uint32_t i = threadIdx.x; i < w.tab_words; i += kWaveBlock)
                 tab_lds[i] = w.tab[i];
             __syncthreads();
@@ -2819,12 +2853,8 @@ __global__ __launch_bounds__(kWaveBlock) void rdme_step_ensemble(EnsArgs e)
                          reinterpret_cast<const int8_t *>(tbase + w.off_stoich),
                          reinterpret_cast<const uint64_t *>(tbase + w.off_rmask),
                          reinterpret_cast<const float *>(tbase + w.off_bound)};
-            const uint32_t local = item - c.first_item;
-            const uint32_t tz = local % c.tiles_z, by = (local / c.tiles_z) % c.btiles_y;
-            const uint32_t chunk = local / (c.tiles_z * c.btiles_y);
-            const uint32_t wy = by * kWavesPerBlock + wv;
-            if (wy < c.wtiles_y)
-                wave_march<S>(L[wv], tb, a, w, tz, wy, int(chunk * e.xc), min(int(c.R0), int((chunk + 1u) * e.xc)), tc);
+            if (busy)
+                wave_march<S>(L[wv], tb, a, w, tz, wy, xb, xe, tc);
             __syncthreads();  // the tables stay until every wave of the group is done
         }
     }
@@ -3993,7 +4023,8 @@ Ensemble::~Ensemble()
         hipStreamSynchronize(static_cast<hipStream_t>(stream_));
     destroy_graph();
     void *bufs[] = {lat_[0], lat_[1], reg_, regp_, tab_, cells_dev_, cme_args_dev_, cme_tab_, cme_counts_, itime_, queue_,
-                    counters_, audit_, fix_ctl_, fix_list_, fix_F_, fix_W_, fix_L_, fix_O_, sum_[0], sum_[1]};
+                    counters_, audit_, fix_ctl_, fix_list_, fix_F_, fix_W_, fix_L_, fix_O_, sum_[0], sum_[1],
+                    item_cell_};
     for (void *b : bufs)
         if (b)
             (void)hipFree(b);
@@ -4017,9 +4048,19 @@ void Ensemble::build_descriptors()
     if (items >= (uint64_t(1) << 31))
         throw Error(CS_ERR_UNSUPPORTED, "too many work items");
     total_items_ = uint32_t(items);
+    std::vector<uint32_t> item_cell(std::max<uint64_t>(items, 1), 0u);
+    for (uint32_t c = 0; c < n; ++c)
+        std::fill(item_cell.begin() + d[c].first_item,
+                  item_cell.begin() + d[c].first_item + size_t(d[c].tiles_z) * d[c].btiles_y * d[c].chunks, c);
     hipStream_t st = static_cast<hipStream_t>(stream_);
+    if (item_cell_)
+        hip_check(hipFree(item_cell_), "hipFree item cells");
+    item_cell_ = nullptr;
+    hip_check(hipMalloc(&item_cell_, item_cell.size() * sizeof(uint32_t)), "hipMalloc item cells");
+    hip_check(hipMemcpyAsync(item_cell_, item_cell.data(), item_cell.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st),
+              "upload item cells");
     hip_check(hipMemcpyAsync(cells_dev_, cells_.data(), cells_.size(), hipMemcpyHostToDevice, st), "upload cells");
-    hip_check(hipStreamSynchronize(st), "upload cells");  // cells_ may change again before the copy ran
+    hip_check(hipStreamSynchronize(st), "upload cells");  // cells_ and item_cell may change again before the copy ran
 }
 
 void Ensemble::set_tiling(const EnsembleTiling &t)
@@ -4055,6 +4096,7 @@ void Ensemble::enqueue_step()
     e.dst = lat_[cur_ ^ 1];
     e.sum_src = sum_[cur_];
     e.sum_dst = sum_[cur_ ^ 1];
+    e.item_cell = item_cell_;
     e.reg = reg_;
     e.regp = static_cast<const uint32_t *>(regp_);
     e.tab = tab_;
