@@ -2192,32 +2192,19 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
                     const int z2 = zeta + (d == 4u ? -1 : (d == 5u ? 1 : 0));
                     const int X2 = X + dx;
                     const bool ev = moves && r2 >= 0 && r2 < kWY && z2 >= 0 && z2 < kWZ && X2 >= xb && X2 < xe;
-                    if (__ballot(ev) == 0ull)
-                        continue;
-                    const uint32_t dest = uint32_t(r2 * kWZ + z2);
-#pragma unroll
-                    for (int t = -1; t <= 1; ++t) {
-                        const bool mine = ev && dx == t;
-                        const uint64_t m = __ballot(mine);
-                        if (m == 0ull)
-                            continue;
-                        const int rg3 = ring3(X + t);
-                        const uint32_t cnt0 = L.ev_n[rg3];
-                        const uint32_t rank = __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32),
-                                                                        __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
-                        if (mine) {
-                            const uint32_t idx = cnt0 + rank;
-                            if (idx < w.ev_cap) {
-                                const uint32_t prev = head_exchange(L.head[rg3], dest, idx);
-                                L.ev[rg3][idx] = d | (sp << 3) | (prev << 11);
-                            }
-                            atomicOr(reinterpret_cast<unsigned long long *>(&L.arrive[rg3][r2]),
-                                     (unsigned long long)(1ull << z2));
+                    // One event per arriving particle, appended to its destination plane's list with an LDS
+                    // atomic (list order is free: the per-voxel chains carry the order, and within one slot index
+                    // j no two events reach one voxel from one direction, so the chains see the source-slot order).
+                    if (ev) {
+                        const uint32_t dest = uint32_t(r2 * kWZ + z2);
+                        const int rg3 = ring3(X + dx);
+                        const uint32_t idx = atomicAdd(&L.ev_n[rg3], 1u);
+                        if (idx < w.ev_cap) {
+                            const uint32_t prev = head_exchange(L.head[rg3], dest, idx);
+                            L.ev[rg3][idx] = d | (sp << 3) | (prev << 11);
                         }
-                        wave_sync();
-                        if (l == 0)
-                            L.ev_n[rg3] = cnt0 + uint32_t(__popcll(m));
-                        wave_sync();
+                        atomicOr(reinterpret_cast<unsigned long long *>(&L.arrive[rg3][r2]),
+                                 (unsigned long long)(1ull << z2));
                     }
                 }
                 if (act && interior && lv != 0u)
