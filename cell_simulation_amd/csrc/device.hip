@@ -2024,8 +2024,8 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
         store_lab(xb - 1, r);
     }
 #pragma unroll
-    for (int k = 0; k < kWBatches; ++k)
-        st0.rec[k] = load_head<S>(a.src, slot_base(xb - 1) + off[k]);
+    for (int k = 0; k < kWBatches; ++k)  // the side plane xb - 1: interior rows only
+        st0.rec[k] = k < kWY ? load_head<S>(a.src, slot_base(xb - 1) + off[k]) : HVec{};
 #pragma unroll
     for (int i = 0; i < kWLabLane; ++i)
         st0.lab[i] = w.regp[lab_base(xb) + loff[i]];
@@ -2078,19 +2078,24 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
     auto plane = [&](int X, WaveStaged<S> &use, WaveStaged<S> &fill) __attribute__((always_inline)) {
         // ---- prefetch plane X+1's slots and plane X+2's labels (clamped; unused past the chunk)
         {
+            // planes past xe are never listed nor gathered; on the side plane xe only the interior rows are
+            // listed (a particle there reaches the tile with a hop along x or not at all)
             const uint32_t sb = slot_base(X + 1), lb = lab_base(X + 2);
-            if (!summ || pact(X + 1)) {
+            if (X + 1 <= xe && (!summ || pact(X + 1))) {
+                const int nb = X + 1 == xe ? kWY : kWBatches;
 #pragma unroll
                 for (int k = 0; k < kWBatches; ++k)
-                    fill.rec[k] = load_head<S>(a.src, sb + off[k]);
+                    fill.rec[k] = k < nb ? load_head<S>(a.src, sb + off[k]) : HVec{};
             } else {
 #pragma unroll
                 for (int k = 0; k < kWBatches; ++k)
                     fill.rec[k] = HVec{};  // no particle in any record of that plane
             }
+            if (X + 1 <= xe) {  // labels up to plane xe + 1 (the hops of plane xe look at them)
 #pragma unroll
-            for (int i = 0; i < kWLabLane; ++i)
-                fill.lab[i] = w.regp[lb + loff[i]];
+                for (int i = 0; i < kWLabLane; ++i)
+                    fill.lab[i] = w.regp[lb + loff[i]];
+            }
         }
         store_lab(X + 1, use.lab);
         // plane X+1's event list starts empty (it was consumed as plane X-2's); plane X's leave masks too
