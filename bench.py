@@ -159,7 +159,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--edge", type=int, default=256)
-    ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps for the CPU baseline (0 = skip)")
+    ap.add_argument("--cpu-steps", type=int, default=11,
+                    help="oracle steps for the CPU baseline: one warm-up, the rest timed (0 = skip)")
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "large_cell_256_traffic.json"),
                     help="per-launch HBM bytes measured with rocprofv3 --pmc (FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--ensemble-cells", type=int, default=512,
