@@ -2433,7 +2433,7 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
 // first and the short ones fill in behind them. The queue is split per XCD (XCD k takes the groups of
 // rank k mod 8, an equal share of the heavy ones); an XCD whose queue is empty helps the others.
 template <int S>
-__global__ __launch_bounds__(kWaveBlock) __attribute__((amdgpu_waves_per_eu(S == 16 ? 3 : 2)))
+__global__ __launch_bounds__(kWaveBlock) __attribute__((amdgpu_waves_per_eu(3)))
 void rdme_step_wave(StepArgs a, WaveArgs w)
 {
     __shared__ WaveLds<S> L[kWavesPerBlock];
