@@ -288,16 +288,21 @@ __device__ __forceinline__ uint32_t hop(const StepArgs &a, const Tab &tb, const 
 #pragma unroll
     for (int d = 0; d < 6; ++d)
         val[d] = row[__umul24(n1, h.rg[d])];
-    float acc = 0.0f;
-    uint32_t dir = 6u;  // 6: no stop yet; 7: stopped on a missing neighbour (stays)
+    float acc[6];
+    float s = 0.0f;
 #pragma unroll
     for (int d = 0; d < 6; ++d) {
         if (h.inside & (1u << d))
-            acc = acc + val[d];
-        if (dir == 6u && !(r > acc))
-            dir = ((h.inside >> d) & 1u) ? uint32_t(d) : 7u;
+            s = s + val[d];
+        acc[d] = s;
     }
-    return dir == 7u ? 6u : dir;
+    // the first d with r <= acc[d] decides: the hop if that neighbour exists, else the particle stays; selects
+    // from the last direction down, so no branch
+    uint32_t dir = 6u;
+#pragma unroll
+    for (int d = 5; d >= 0; --d)
+        dir = !(r > acc[d]) ? (((h.inside >> d) & 1u) ? uint32_t(d) : 6u) : dir;
+    return dir;
 }
 
 // Append the particles of neighbour u that hop in direction `want` (toward the voxel being gathered),
@@ -2038,13 +2043,10 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
         L.head[q / (kWY / 2)][(q % (kWY / 2)) * 64 + l] = kWEvEnd | (kWEvEnd << 16);
 
     uint32_t n_tight = 0, n_bad = 0, n_near = 0, n_fired = 0, n_rolled = 0;
-    uint64_t occ_prev[kWY], occ_cur[kWY], inm[kWY];  // per interior row: occupied lanes of the plane
-                                                      // gathered next, of the plane listed now; in-lattice lanes
-#pragma unroll
-    for (int r = 0; r < kWY; ++r) {
-        occ_prev[r] = occ_cur[r] = 0ull;
-        inm[r] = __ballot((inl >> r) & 1u);
-    }
+    // bit r of this lane: its voxel of interior row r holds particles, in the plane gathered next and in the
+    // plane listed now (per-lane bits rather than wave masks: these live through the march, and scalar
+    // registers are the scarcer ones here)
+    uint32_t occ_prev = 0u, occ_cur = 0u;
     HVec prev[kWY];          // interior slots of that plane: the gather reads no global memory, where a
                              // load would also wait for the prefetch issued before it (loads return in order)
 #pragma unroll
@@ -2137,7 +2139,7 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
                 }
                 n += uint32_t(__popcll(m));
                 if (k < kWY)
-                    occ_cur[k] = m;
+                    occ_cur |= uint32_t(occ) << k;
             }
         }
         wave_sync();
@@ -2148,6 +2150,8 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
             const uint8_t *lcur = reinterpret_cast<const uint8_t *>(L.lab[ring3(X)]);
             const uint8_t *lnext = reinterpret_cast<const uint8_t *>(L.lab[ring3(X + 1)]);
             const uint32_t n1 = a.ns + 1u;
+            // event lists of planes X-1, X, X+1: nibbles 0, 1, 2
+            const uint32_t rings = uint32_t(ring3(X - 1)) | uint32_t(ring3(X)) << 4 | uint32_t(ring3(X + 1)) << 8;
             for (uint32_t base = 0; base < n; base += 64u) {
                 const uint32_t i = base + uint32_t(l);
                 const bool act = i < n;
@@ -2192,17 +2196,19 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
                     (void)n1;
                     const bool moves = d < 6u;
                     lv |= uint32_t(moves) << j;
-                    const int dx = d == 0u ? -1 : (d == 1u ? 1 : 0);
-                    const int r2 = rho + (d == 2u ? -1 : (d == 3u ? 1 : 0));
-                    const int z2 = zeta + (d == 4u ? -1 : (d == 5u ? 1 : 0));
-                    const int X2 = X + dx;
-                    const bool ev = moves && r2 >= 0 && r2 < kWY && z2 >= 0 && z2 < kWZ && X2 >= xb && X2 < xe;
+                    // the step along x, y, z of direction d, plus one: nibble d of each constant (d = 6: none)
+                    const uint32_t dsh = 4u * d;
+                    const uint32_t dx1 = (0x1111120u >> dsh) & 0xFu;
+                    const int r2 = rho + int((0x1112011u >> dsh) & 0xFu) - 1;
+                    const int z2 = zeta + int((0x1201111u >> dsh) & 0xFu) - 1;
+                    const bool ev = moves && uint32_t(r2) < uint32_t(kWY) && uint32_t(z2) < uint32_t(kWZ) &&
+                                    uint32_t(X + int(dx1) - 1 - xb) < uint32_t(xe - xb);
                     // One event per arriving particle, appended to its destination plane's list with an LDS
                     // atomic (list order is free: the per-voxel chains carry the order, and within one slot index
                     // j no two events reach one voxel from one direction, so the chains see the source-slot order).
                     if (ev) {
                         const uint32_t dest = uint32_t(r2 * kWZ + z2);
-                        const int rg3 = ring3(X + dx);
+                        const int rg3 = int((rings >> (4u * dx1)) & 0xFu);
                         const uint32_t idx = atomicAdd(&L.ev_n[rg3], 1u);
                         if (idx < w.ev_cap) {
                             const uint32_t prev = head_exchange(L.head[rg3], dest, idx);
@@ -2230,13 +2236,14 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
             uint32_t ns = 0u, needm = 0u, sidx[kWY];
 #pragma unroll
             for (int r = 0; r < kWY; ++r) {
-                const uint64_t m = inm[r] & (occ_prev[r] | L.arrive[rq][r] | every);
+                const uint64_t inm = __ballot((inl >> r) & 1u);
+                const uint64_t m = inm & (__ballot((occ_prev >> r) & 1u) | L.arrive[rq][r] | every);
                 // an empty voxel's zero-propensity draw fires at seed 0 (itime + x*y + z == 0)
                 const uint32_t zfire = 0u - a.itime - uint32_t(Q) * uint32_t(y0 + r) - uint32_t(z0);
                 if (l == 0 && zfire < uint32_t(kWZ))
-                    n_fired += uint32_t((inm[r] & ~m) >> zfire) & 1u;
+                    n_fired += uint32_t((inm & ~m) >> zfire) & 1u;
                 const bool need = (m >> l) & 1ull;
-                const bool occ = (occ_prev[r] >> l) & 1ull;
+                const bool occ = (occ_prev >> r) & 1u;
                 const uint32_t rank =
                     __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
                 sidx[r] = ns + rank;
@@ -2268,7 +2275,6 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
                 }
                 const uint32_t occ = count_nonzero(own);
                 const uint32_t lv = act ? lvq[me] : 0u;
-                const uint64_t stay = first_nibbles(occ) & ~spread_nibbles(lv);
                 // Most voxels end with at most four particles: then the arrivals and the assembled slots fit
                 // one 32-bit word, and the wave takes that path when every lane fits it.
                 uint32_t cnt = 0u, narr = 0u;
@@ -2277,23 +2283,30 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
                 if constexpr (S > 4) {
                     if (!over && __ballot(occ > 4u) == 0ull) {
                         uint32_t arr = 0u, tot = 0u;
-                        uint64_t cnts = 0ull;
+                        // arrivals per rank, one nibble each: exact while the voxel gets at most 15 per rank, and
+                        // a lane with more has more than four arrivals, so it leaves this path anyway
+                        uint32_t cn = 0u;
                         uint32_t e = act ? head_read(L.head[rq], me) : kWEvEnd;
                         while (__ballot(e != kWEvEnd) != 0ull) {
                             if (e != kWEvEnd) {
                                 const uint32_t c = L.ev[rq][e];
                                 const uint32_t k = (0x231405u >> (4u * (c & 7u))) & 0xFu;
-                                const uint32_t pos =
-                                    k ? uint32_t(((cnts * 0x0101010101010101ull) >> (8u * (k - 1u))) & 0xFFull) : 0u;
+                                // arrivals of lower ranks so far: the nibbles below k summed (24-bit product)
+                                const uint32_t pos = k ? (__umul24(cn, 0x111111u) >> (4u * k - 4u)) & 0xFu : 0u;
                                 arr = b4_insert(arr, pos, (c >> 3) & 0xFFu);
-                                cnts += 1ull << (8u * k);
+                                cn += 1u << (4u * k);
                                 ++tot;
                                 e = c >> 11;
                             }
                         }
                         if (__ballot(occ + tot > 4u) == 0ull) {
-                            const uint32_t nlo4 = uint32_t((cnts * 0x010101ull) >> 16) & 0xFFu;
-                            out.w[0] = assemble4(own.w[0], occ, occ ? uint32_t(stay) : 0u, arr & b4_below(nlo4),
+                            const uint32_t nlo4 = (__umul24(cn, 0x111111u) >> 8) & 0xFu;  // ranks 0..2: from below
+                            // own slots that stay: bit 4s for each slot s < occ whose particle did not leave
+                            uint32_t sp4 = lv & 0xFu;
+                            sp4 = (sp4 | (sp4 << 6)) & 0x0303u;
+                            sp4 = (sp4 | (sp4 << 3)) & 0x1111u;
+                            const uint32_t stay4 = ((1u << (4u * occ)) - 1u) & 0x1111u & ~sp4;
+                            out.w[0] = assemble4(own.w[0], occ, stay4, arr & b4_below(nlo4),
                                                  nlo4, b4_shr(arr, nlo4), tot - nlo4, cnt);
                             narr = tot;
                             done = true;
@@ -2339,6 +2352,7 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
                         note_tight(a, V);
                     }
                     narr = nlo + nhi;
+                    const uint64_t stay = first_nibbles(occ) & ~spread_nibbles(lv);
                     out = assemble<S>(own, occ, occ ? stay : 0ull, lo, min(nlo, 16u), hi, min(nhi, 16u), cnt);
                 }
                 // a reservoir voxel whose occupancy word is off the slot count can refuse a claim that fits
@@ -2397,11 +2411,8 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
                     own_sum[Q] = uint8_t(new_q);
             }
         }
-#pragma unroll
-        for (int r = 0; r < kWY; ++r) {
-            occ_prev[r] = occ_cur[r];
-            occ_cur[r] = 0ull;
-        }
+        occ_prev = occ_cur;
+        occ_cur = 0u;
 #pragma unroll
         for (int r = 0; r < kWY; ++r)
             prev[r] = use.rec[r];
