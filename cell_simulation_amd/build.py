@@ -27,6 +27,9 @@ HIP_FLAGS = COMMON + [
     "-fhip-fp32-correctly-rounded-divide-sqrt",
     "-fno-gpu-flush-denormals-to-zero",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # the machine scheduler's memory-clause strategy: the wave-march kernel's loads and LDS traffic grouped into
+    # clauses measured 3.5% faster on the 256^3 S = 16 step than the default strategy (no change at S = 8)
+    "-mllvm", "-amdgpu-sched-strategy=max-memory-clause",
 ]
 CXX_FLAGS = COMMON
 
