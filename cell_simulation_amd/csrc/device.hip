@@ -1648,7 +1648,7 @@ struct WaveArgs {
     uint32_t PY, PZ4;       // PY = kWY*wtiles_y + 4 rows; PZ4 = (kWZ*tiles_z + 4) / 4 words per row
     uint32_t nitems;        // work items: kWavesPerBlock tiles stacked in y over an x-range
     uint32_t *queue;        // [8] work counters, one per XCD; zero at the start of every step
-    const uint2 *items;     // [nitems] tz | by << 16, x_begin | x_end << 16; by decreasing expected time
+    const uint2 *items;     // [nitems] tz | by << 16, x_begin | x_end << 16; entry e in queue e % 8 (by 0xFFFF: empty)
     const uint8_t *sum_src;   // occupancy summary of src (SumRef layout, geometry of StepArgs::sum_dst), or null
 };
 static_assert(kWY == int(kSumTY) && kWZ == int(kSumTZ), "the occupancy summary counts wave tiles");
@@ -2441,8 +2441,9 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
 // march side by side on one CU, so the halo rows one wave loads are the interior rows of its neighbours,
 // found in L1/L2. A group's time grows with its particles, and a dense region holds several times the
 // particles of the average group, so groups are queued by decreasing particle count: the longest start
-// first and the short ones fill in behind them. The queue is split per XCD (XCD k takes the groups of
-// rank k mod 8, an equal share of the heavy ones); an XCD whose queue is empty helps the others.
+// first and the short ones fill in behind them. The queue is split per XCD: XCD k takes the groups of the k-th
+// band of block rows, whose neighbours' halo lines are then in its own L2; an XCD whose queue is empty helps
+// the others.
 template <int S>
 __global__ __launch_bounds__(kWaveBlock) __attribute__((amdgpu_waves_per_eu(3)))
 void rdme_step_wave(StepArgs a, WaveArgs w)
@@ -3703,17 +3704,46 @@ void Device::plan_wave_items(const uint8_t *lat)
         }
     }
     std::stable_sort(items.begin(), items.end(), [](const Item &p, const Item &q) { return p.cost > q.cost; });
-    std::vector<uint32_t> packed(2 * items.size());
-    for (size_t i = 0; i < items.size(); ++i) {
-        const uint32_t tz = items[i].col % tz_n, by = items[i].col / tz_n;
-        packed[2 * i] = tz | (by << 16);
-        packed[2 * i + 1] = items[i].x0 | (items[i].x1 << 16);
+    // Queue k (XCD k) takes entries k, k + 8, ...: the columns of the k-th band of block rows (all their z tiles and
+    // x ranges, cost order within the band), so the z and x neighbours of an item, whose halo lines it reads, run on
+    // the same XCD and share its L2 (256^3: 10% fewer bytes read and 4.5% less time than dealing the cost-ordered
+    // items round the XCDs, which CS_WAVE_PLACEMENT=rank restores; bands along x measured slower than either).
+    // Shorter queues are padded with empty entries; an XCD whose queue is done helps the others.
+    std::vector<std::vector<const Item *>> q(8);
+    const char *placement = std::getenv("CS_WAVE_PLACEMENT");
+    if (placement != nullptr && std::string(placement) == "rank") {
+        for (size_t i = 0; i < items.size(); ++i)
+            q[i % 8].push_back(&items[i]);
+    } else {
+        for (const Item &it : items)
+            q[std::min<uint32_t>(7u, (it.col / tz_n) * 8u / std::max(by_n, 1u))].push_back(&it);
     }
+    size_t qlen = 0;
+    for (const auto &l : q)
+        qlen = std::max(qlen, l.size());
+    std::vector<uint32_t> packed(2 * 8 * qlen);
+    for (size_t i = 0; i < qlen; ++i)
+        for (uint32_t k = 0; k < 8u; ++k) {
+            const size_t e = i * 8u + k;
+            if (i < q[k].size()) {
+                const Item &it = *q[k][i];
+                packed[2 * e] = (it.col % tz_n) | ((it.col / tz_n) << 16);
+                packed[2 * e + 1] = it.x0 | (it.x1 << 16);
+            } else {
+                packed[2 * e] = 0xFFFFu << 16;  // an empty entry: its block row lies past the lattice
+                packed[2 * e + 1] = 0u;
+            }
+        }
+    // trailing empty entries are never taken
+    size_t n_packed = 8 * qlen;
+    while (n_packed > 0 && packed[2 * (n_packed - 1)] == (0xFFFFu << 16))
+        --n_packed;
+    packed.resize(2 * n_packed);
     if (items_)
         hipFree(items_);
     items_ = nullptr;
     hip_check(hipMalloc(&items_, std::max<size_t>(packed.size(), 2) * sizeof(uint32_t)), "hipMalloc wave items");
-    n_items_ = uint32_t(items.size());
+    n_items_ = uint32_t(n_packed);
     hipStream_t st = static_cast<hipStream_t>(stream_);
     hip_check(hipMemcpyAsync(items_, packed.data(), packed.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st),
               "upload wave items");
