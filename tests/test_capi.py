@@ -28,6 +28,21 @@ def test_every_declared_symbol_is_exported():
     assert sorted(_lib.SIGNATURES) == names
 
 
+def test_rust_binding_lists_every_declaration():
+    """The extern block a Rust crate would add declares every entry point of the header, each
+    once, with as many parameters as the C declaration."""
+    doc = (HEADER.parent.parent / "INTEGRATION.md").read_text()
+    rust = {m.group(1): m.group(2) for m in re.finditer(r"pub fn (cs_\w+)\(([^)]*)\)", doc)}
+    assert len(re.findall(r"pub fn cs_\w+\(", doc)) == len(rust)
+    assert sorted(rust) == declared_functions()
+    text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
+    for name, args in rust.items():
+        c_args = re.search(name + r"\s*\(([^;]*?)\)\s*;", text).group(1).strip()
+        n_c = 0 if c_args in ("", "void") else c_args.count(",") + 1
+        n_r = 0 if not args.strip() else args.count(",") + 1
+        assert n_c == n_r, (name, n_c, n_r)
+
+
 def test_header_constants_match_binding():
     text = HEADER.read_text()
     for name in ("CS_OK", "CS_ERR_ARG", "CS_ERR_NOT_FOUND", "CS_ERR_STATE", "CS_ERR_HIP", "CS_ERR_UNSUPPORTED",
