@@ -165,16 +165,17 @@ def main() -> int:
                     help="per-launch HBM bytes measured with rocprofv3 --pmc (FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--ensemble-cells", type=int, default=512,
                     help="replicate cells of the batched-vs-plain comparison (0 = skip it)")
-    ap.add_argument("--ensemble-steps", type=int, default=200, help="timed steps of the batched ensemble")
+    ap.add_argument("--ensemble-steps", type=int, default=5000,
+                    help="timed steps of the batched ensemble (the workload's run length: 5000 steps in 100-step graphs)")
     ap.add_argument("--plain-steps", type=int, default=10, help="timed steps of the per-cell plain launches")
     ap.add_argument("--ensemble-only", action="store_true",
                     help="run only the batched-ensemble leg and print its record (for profiling it)")
     ap.add_argument("--ensemble-traffic-json", default=os.path.join(ROOT, "profiles", "ensemble_512_traffic.json"),
                     help="HBM bytes per batched step measured with rocprofv3 --pmc (FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--graph-block", type=int, default=100, help="steps per captured hipGraph")
-    ap.add_argument("--precision", type=int, default=0, metavar="STEPS",
-                    help="also run the 512^3 precision-stress workload for STEPS steps with the f64 audit of the "
-                         "well-mixed propensities and report it (off by default: building 512^3 takes a while)")
+    ap.add_argument("--precision", type=int, default=1000, metavar="STEPS",
+                    help="also run the 512^3 precision-stress workload for STEPS steps (its run length: 1000) with "
+                         "the f64 audit of the well-mixed propensities and report it (0 = skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="instead of the step: copy a MIB-MiB device buffer --steps times (a kernel of known "
                          "bytes for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
