@@ -17,6 +17,7 @@
 // A voxel whose own particles plus all arrivals exceed S ("tight") is counted in CS_CNT_TIGHT; there
 // the capacity hazards of rdme.wgsl:57-87 (rejected claims, the slot-S alias) change the outcome.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <cmath>
@@ -3507,10 +3508,12 @@ void Device::step(uint32_t itime)
         w.sum_src = sum_src.bytes;
         const uint32_t blocks = std::min(n_items_, wave_blocks_);
         const size_t shm = size_t(tab_words_) * 4u;
+        // with a timing pair set, the dispatch itself stamps the events at the kernel's start and end
+        hipEvent_t e0 = static_cast<hipEvent_t>(kernel_ev_[0]), e1 = static_cast<hipEvent_t>(kernel_ev_[1]);
         switch (S_) {
-        case 4: hipLaunchKernelGGL(rdme_step_wave<4>, dim3(blocks), dim3(kWaveBlock), shm, st, a, w); break;
-        case 8: hipLaunchKernelGGL(rdme_step_wave<8>, dim3(blocks), dim3(kWaveBlock), shm, st, a, w); break;
-        case 16: hipLaunchKernelGGL(rdme_step_wave<16>, dim3(blocks), dim3(kWaveBlock), shm, st, a, w); break;
+        case 4: hipExtLaunchKernelGGL(rdme_step_wave<4>, dim3(blocks), dim3(kWaveBlock), shm, st, e0, e1, 0u, a, w); break;
+        case 8: hipExtLaunchKernelGGL(rdme_step_wave<8>, dim3(blocks), dim3(kWaveBlock), shm, st, e0, e1, 0u, a, w); break;
+        case 16: hipExtLaunchKernelGGL(rdme_step_wave<16>, dim3(blocks), dim3(kWaveBlock), shm, st, e0, e1, 0u, a, w); break;
         default: throw Error(CS_ERR_UNSUPPORTED, "slots per voxel must be 4, 8 or 16");
         }
     } else {
@@ -3607,10 +3610,25 @@ float Device::run_timed(const uint32_t *itime, uint32_t nsteps)
     std::vector<hipEvent_t> ev(2 * size_t(nsteps));
     for (auto &e : ev)
         hip_check(hipEventCreate(&e), "hipEventCreate");
+    // The wave kernel's own duration: its dispatch stamps the pair. The other kernels (tiled, one lane per voxel)
+    // are timed by events recorded around the step, which then also hold the tight replay and any CME phase.
+    const bool wave = !simple_ && !tiled_;
     for (uint32_t t = 0; t < nsteps; ++t) {
-        hip_check(hipEventRecord(ev[2 * t], st), "hipEventRecord");
-        step(itime[t]);
-        hip_check(hipEventRecord(ev[2 * t + 1], st), "hipEventRecord");
+        if (wave) {
+            kernel_ev_[0] = ev[2 * t];
+            kernel_ev_[1] = ev[2 * t + 1];
+            try {
+                step(itime[t]);
+            } catch (...) {
+                kernel_ev_[0] = kernel_ev_[1] = nullptr;
+                throw;
+            }
+            kernel_ev_[0] = kernel_ev_[1] = nullptr;
+        } else {
+            hip_check(hipEventRecord(ev[2 * t], st), "hipEventRecord");
+            step(itime[t]);
+            hip_check(hipEventRecord(ev[2 * t + 1], st), "hipEventRecord");
+        }
     }
     hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
     float total = 0.f;

@@ -56,6 +56,7 @@ private:
 
     int dev_ = 0;
     void *stream_ = nullptr;       // hipStream_t
+    void *kernel_ev_[2] = {nullptr, nullptr};  // hipEvent_t pair the next wave-kernel dispatch stamps (run_timed)
     uint32_t res_[3] = {0, 0, 0};
     uint64_t N_ = 0;
     uint32_t S_ = 0, ns_ = 0, nr_ = 0, nreg_ = 0, nrw_ = 0;

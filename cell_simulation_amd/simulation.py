@@ -313,7 +313,8 @@ class Simulation:
 
     def run_timed(self, itimes: Sequence[int]) -> float:
         """Steps without statistics sampling; returns the summed step-kernel time in ms (HIP events on the
-        simulation's stream)."""
+        simulation's stream: stamped by the wave kernel's own dispatch, or recorded around each step for the
+        other kernels)."""
         arr = np.ascontiguousarray(np.asarray(itimes, dtype=np.uint32))
         ms = C.c_float(0.0)
         check(lib().cs_sim_run_timed(self._h, arr.ctypes.data_as(C.POINTER(C.c_uint32)), len(arr), C.byref(ms)))
