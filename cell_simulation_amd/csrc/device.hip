@@ -1908,20 +1908,21 @@ __device__ __forceinline__ void count_idle_fires(const StepArgs &a, uint32_t tz,
 {
     const int l = int(__lane_id());
     const int y0 = int(wy) * kWY, z0 = int(tz) * kWZ;
-    uint64_t im[kWY];
+    // one lane per plane Q of the range, row by row: the voxel (Q, y0 + r, z0 + zfire) fires if it lies in the
+    // lattice (bit zfire of the row's in-lattice lanes)
+    uint32_t nf = 0u;
 #pragma unroll
-    for (int r = 0; r < kWY; ++r)
-        im[r] = __ballot(y0 + r < int(a.R1) && z0 + l < int(a.R2));
-    if (l == 0) {
-        uint32_t nf = 0u;
-        for (int r = 0; r < kWY; ++r)
-            for (int Q = xb; Q < xe; ++Q) {
-                const uint32_t zfire = 0u - a.itime - uint32_t(Q) * uint32_t(y0 + r) - uint32_t(z0);
-                if (zfire < uint32_t(kWZ))
-                    nf += uint32_t(im[r] >> zfire) & 1u;
-            }
-        tc.fired += nf;
+    for (int r = 0; r < kWY; ++r) {
+        const uint64_t im = __ballot(y0 + r < int(a.R1) && z0 + l < int(a.R2));
+        for (int qb = xb; qb < xe; qb += 64) {
+            const int Q = qb + l;
+            const uint32_t zfire = 0u - a.itime - uint32_t(Q) * uint32_t(y0 + r) - uint32_t(z0);
+            const bool f = Q < xe && zfire < uint32_t(kWZ) && ((im >> (zfire & 63u)) & 1ull) != 0ull;
+            nf += uint32_t(__popcll(__ballot(f)));
+        }
     }
+    if (l == 0)
+        tc.fired += nf;
 }
 
 // One wave tile (tz, wy) of the lattice (a, w), marched through x-chunk `chunk`.
