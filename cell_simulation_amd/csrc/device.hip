@@ -4317,7 +4317,7 @@ float Ensemble::autotune(uint32_t trial_steps)
     const EnsembleTiling before = tiling_;
     EnsembleTiling best = before;
     float best_ms = -1.f;
-    const uint32_t xcs[] = {8, 16, 32, 48};  // up to 60 planes keep the occupancy summaries usable
+    const uint32_t xcs[] = {8, 16, 32, 48, 60};  // up to 60 planes keep the occupancy summaries usable
     const uint32_t bpcs[] = {2, 3, 4};
     const uint32_t batches[] = {0, 1, 4};
     for (uint32_t xc : xcs)

@@ -55,7 +55,8 @@ def test_ensemble_matches_plain_cells():
 
 
 @pytest.mark.parametrize("tiling", [dict(xc=8, blocks_per_cu=2, batch=0), dict(xc=32, blocks_per_cu=4, batch=4),
-                                    dict(xc=5, blocks_per_cu=1, batch=1)])
+                                    dict(xc=5, blocks_per_cu=1, batch=1), dict(xc=60, blocks_per_cu=2, batch=0),
+                                    dict(xc=61, blocks_per_cu=3, batch=1)])
 def test_ensemble_tilings_agree(tiling):
     announce(f"ensemble tiling {tiling}, 30 steps")
     cells, seeds = _cells()
