@@ -29,8 +29,8 @@ def test_every_declared_symbol_is_exported():
 
 
 def test_rust_binding_lists_every_declaration():
-    """The extern block a Rust crate would add declares every entry point of the header, each
-    once, with as many parameters as the C declaration."""
+    """The Rust extern block of the integration notes (what a crate binding this library would add) declares
+    every entry point of the header, each once, with as many parameters as the C declaration."""
     doc = (HEADER.parent.parent / "INTEGRATION.md").read_text()
     rust = {m.group(1): m.group(2) for m in re.finditer(r"pub fn (cs_\w+)\(([^)]*)\)", doc)}
     assert len(re.findall(r"pub fn cs_\w+\(", doc)) == len(rust)
