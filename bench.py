@@ -7,8 +7,13 @@ lattice.
 
 Prints ONE JSON line. `roofline` prices the step kernel against HBM: algorithmic bytes per voxel-step
 = 2*S (read and write every slot) + 1 (region label) = 33 B at S=16, divided by the kernel's average
-duration measured with HIP events on the simulation's stream. `cpu_baseline` times the CPU oracle (the
-serial restatement of the reference shaders, one core) on a bounded sample of the same workload.
+duration measured with HIP events on the simulation's stream. `cpu_baseline` times the CPU restatement of the
+reference shaders on every host core this process may use (hop draws, reactions and compaction in parallel,
+claims in the canonical order; bit-exact against the serial oracle) on a bounded sample of the same workload,
+and the 64^3 spherical cell for its full 1000 steps. `configs` times the other single-cell workloads on the
+GPU (64^3 for its 1000 steps, 128^3 gene expression for its 10000 steps), `batched` the 512-cell ensemble as
+stated (empty lattices, genes off) and started near its stationary state, against the same cells' plain
+launches, and `precision` the 512^3 precision-stress workload with its FP32-vs-FP64 audit.
 """
 from __future__ import annotations
 
@@ -45,14 +50,17 @@ def calibrate_copy(mib: int, steps: int) -> int:
     return 0
 
 
-def ensemble_leg(args) -> dict:
+def ensemble_leg(args, populated: bool = False) -> dict:
     """Replicate cells (gene-expression network, edges log-uniform 32..96, rates x LogNormal(0.2), S=8,
     seed 99): one batched Ensemble (autotuned, hipGraph blocks) against the same cells stepped one
-    plain launch sequence per cell. Values are replicate-voxel-steps per second."""
+    plain launch sequence per cell. Values are replicate-voxel-steps per second. `populated`: each cell starts
+    near its stationary state (genes on with probability k_on / (k_on + k_off), mRNA and protein at their
+    stationary means) instead of empty with its genes off."""
     from cell_simulation_amd import Ensemble, configs, default_itime_schedule
 
-    log(f"ensemble: building {args.ensemble_cells} cells")
-    cells, seeds = configs.ensemble_cells(args.ensemble_cells, seed=99)
+    log(f"ensemble: building {args.ensemble_cells} cells (populated={populated})")
+    cells, seeds = configs.ensemble_cells(args.ensemble_cells, seed=99, populated=populated)
+    particles = sum(int(c.species_counts()[1:].sum()) for c in cells)
     ens = Ensemble(cells, seeds)
     vox = ens.info()["total_voxels"]
     ens.run(2)  # warm-up (first launches, page-in)
@@ -68,10 +76,11 @@ def ensemble_leg(args) -> dict:
     if args.plain_steps <= 0:  # the batched leg alone (profiling runs)
         for sim in cells:
             sim.close()
-        return {"voxels_per_step": vox, "unit": "replicate-voxel-steps/s", "batched_value": batched_value,
+        return {"populated": populated, "particles": particles, "voxels_per_step": vox,
+                "unit": "replicate-voxel-steps/s", "batched_value": batched_value,
                 "batched_ms_per_step": ens_ms / args.ensemble_steps, "batched_steps": args.ensemble_steps,
                 "graph_block": args.graph_block, "tiling": tiling, "plain_value": None, "speedup": None,
-                "cme_events": int(c[5]), "exceptions": int(c[1])}
+                "cme_events": int(c[5]), "lattice_reactions_fired": int(c[3]), "exceptions": int(c[1])}
     log(f"plain: preparing {len(cells)} simulations")
     for sim in cells:
         sim.prepare_for_gpu(0)
@@ -91,9 +100,12 @@ def ensemble_leg(args) -> dict:
     for sim in cells:
         sim.close()
     b_alg = 2 * 8 + 1
+    start = ("started near the stationary state (genes on with probability k_on/(k_on+k_off), mRNA and protein at "
+             "their stationary means)" if populated else "started as stated: empty lattices, genes off")
     return {
         "workload": f"{len(cells)} replicate cells, gene-expression network (well-mixed gene switching, lattice "
-                    "mRNA/protein), edges log-uniform 32..96, rates x LogNormal(sigma 0.2), S=8, seed 99",
+                    f"mRNA/protein), edges log-uniform 32..96, rates x LogNormal(sigma 0.2), S=8, seed 99, {start}",
+        "populated": populated, "particles": particles,
         "voxels_per_step": vox, "unit": "replicate-voxel-steps/s",
         "batched_value": batched_value, "batched_ms_per_step": ens_ms / args.ensemble_steps,
         "batched_steps": args.ensemble_steps, "graph_block": args.graph_block, "tiling": tiling,
@@ -104,20 +116,23 @@ def ensemble_leg(args) -> dict:
         # the batched kernel skips the (tile, plane) pieces of a cell that are empty and were empty, so it moves
         # fewer bytes than b_alg per voxel-step: this dense-equivalent rate is not an HBM figure
         "batched_dense_equivalent_gbs": batched_value * b_alg / 1e9,
-        "batched_hbm": _ensemble_hbm(args, ens_ms / args.ensemble_steps),
-        "cme_events": int(c[5]), "exceptions": int(c[1]),
+        "batched_hbm": _ensemble_hbm(args, ens_ms / args.ensemble_steps, populated),
+        "cme_events": int(c[5]), "lattice_reactions_fired": int(c[3]), "exceptions": int(c[1]),
     }
 
 
-def _ensemble_hbm(args, ms_per_step: float):
+def _ensemble_hbm(args, ms_per_step: float, populated: bool):
     """HBM bytes the batched step moved (rocprofv3 FETCH_SIZE / WRITE_SIZE per step, scaled by the copy
-    calibration, from --ensemble-traffic-json) over this run's time per step, against the 8 TB/s peak."""
+    calibration, from the traffic JSON of this start state) over this run's time per step, against the 8 TB/s
+    peak."""
     path = args.ensemble_traffic_json
+    if populated:
+        path = path.replace("_traffic.json", "_populated_traffic.json")
     if not path or not os.path.exists(path):
         return None
     with open(path) as f:
         tj = json.load(f)
-    if tj.get("cells") != args.ensemble_cells:
+    if tj.get("cells") != args.ensemble_cells or bool(tj.get("populated", False)) != populated:
         return None
     gbs = tj["bytes_per_step"] / (ms_per_step * 1e-3) / 1e9
     return {"achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
@@ -125,9 +140,9 @@ def _ensemble_hbm(args, ms_per_step: float):
 
 
 def precision_leg(steps: int) -> dict:
-    """512^3 precision-stress workload (12 lattice + 40 well-mixed species, 120 reactions with rates over
-    1e-6..1e4 /s, ~10M particles, S=8): `steps` steps with the device's f64 audit of every well-mixed event
-    (compensated f32 total vs plain f64 total; choice under f64 vs the f32 choice)."""
+    """512^3 precision-stress workload (12 lattice + 40 well-mixed species, 120 reactions, ~10M particles, S=8):
+    `steps` steps with the device's f64 audit of every well-mixed event (compensated f32 total vs plain f64
+    total; choice under f64 vs the f32 choice)."""
     from cell_simulation_amd import configs, default_itime_schedule
 
     log("precision: building the 512^3 workload")
@@ -143,14 +158,93 @@ def precision_leg(steps: int) -> dict:
     a = sim.cme_audit()
     c = sim.counters()
     inf = sim.info()
-    return {"workload": "precision_stress 512^3 S=8, 12 lattice + 40 well-mixed species, 120 reactions, "
-                        "rates log-uniform 1e-6..1e4 /s, ~10M particles, dt=5us, seed 2024",
+    return {"workload": "precision_stress 512^3 S=8, 12 lattice + 40 well-mixed species, 120 reactions (40 lattice "
+                        "reactions with rate constants log-uniform 1e-6..1e1 /s; 80 well-mixed reactions whose "
+                        "propensities at the initial counts are log-uniform 1e-6..1e4 /s), ~10M particles, dt=5us, "
+                        "seed 2024",
             "steps": steps, "ms_per_step": wall * 1e3 / steps,
             "voxel_steps_per_s": inf["n_voxels"] * steps / wall,
             "cme_events": a["events"], "max_rel_propensity_error_f32_kahan_vs_f64": a["max_rel_error"],
             "selection_divergences": a["divergences"],
             "selection_divergence_rate": a["divergences"] / a["events"] if a["events"] else None,
-            "lattice_reactions_fired": int(c[3]), "near_threshold": int(c[2]), "exceptions": int(c[1])}
+            "lattice_reactions_fired": int(c[3]), "near_threshold": int(c[2]), "exceptions": int(c[1]),
+            "tight_voxel_steps": int(c[0]), "failed_claims": int(c[8]), "slot_s_aliases": int(c[9])}
+
+
+def host_cores() -> tuple[int, str]:
+    """The cores this process may run on, and the CPU model (from /proc/cpuinfo)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return n, model
+
+
+def cpu_leg(args, N: int, S: int) -> dict:
+    """The CPU restatement on every core this process may use: a bounded sample of the 256^3 workload (one
+    warm-up step, then args.cpu_steps timed), and the 64^3 spherical cell for its full 1000 steps."""
+    from cell_simulation_amd import configs, default_itime_schedule
+    from oracle.cs_oracle import Oracle
+
+    cores, model = host_cores()
+    log(f"cpu baseline: {cores} threads on {model}, {args.cpu_steps} steps of {args.edge}^3")
+    orc = Oracle.from_simulation(configs.large_cell(seed=1234, edge=args.edge))
+    ci = default_itime_schedule(1234, args.cpu_steps + 1)
+    orc.run_mt(ci[:1], cores)  # warm-up step (page-in), not timed
+    t1 = time.perf_counter()
+    orc.run_mt(ci[1:], cores)
+    ct = time.perf_counter() - t1
+    del orc
+    out = {"value": N * args.cpu_steps / ct, "unit": "voxel-steps/s", "cores": cores, "kind": "port",
+           "sample": f"{args.cpu_steps} step(s) of the {args.edge}^3 S={S} large-cell workload after one warm-up "
+                     f"step, C restatement of the reference shaders on {cores} threads (hop draws, reactions and "
+                     "compaction in parallel, claims in canonical order; bit-exact against the serial oracle)",
+           "cpu_model": model}
+    if args.cpu_cfg1:
+        log("cpu baseline: 64^3 spherical cell, 1000 steps")
+        orc = Oracle.from_simulation(configs.spherical_cell(seed=42))
+        ci = default_itime_schedule(42, 1000)
+        t1 = time.perf_counter()
+        orc.run_mt(ci, cores)
+        ct = time.perf_counter() - t1
+        out["spherical_cell_64"] = {"value": 64 ** 3 * 1000 / ct, "unit": "voxel-steps/s", "steps": 1000,
+                                    "seconds": ct}
+    return out
+
+
+def single_cell_configs(args) -> dict:
+    """The 64^3 spherical cell for its 1000 steps and the 128^3 gene-expression cell for its 10000 steps (as
+    stated, and started near its stationary state), plain one-launch-per-step path, wall time of the run."""
+    from cell_simulation_amd import configs, default_itime_schedule
+
+    out = {}
+    runs = [("spherical_cell_64", lambda: configs.spherical_cell(seed=42), 42, 1000),
+            ("gene_expression_128", lambda: configs.gene_expression(seed=7), 7, 10000),
+            ("gene_expression_128_populated", lambda: configs.gene_expression(seed=7, populated=True), 7, 10000)]
+    for name, build, seed, steps in runs:
+        sim = build()
+        n = sim.info()["n_voxels"]
+        sim.prepare_for_gpu(0)
+        its = default_itime_schedule(seed, steps + 1)
+        sim.step(0, int(its[0]), 0)  # warm-up
+        sim.synchronize()
+        t0 = time.perf_counter()
+        sim.run(1, its[1:])
+        sim.synchronize()
+        wall = time.perf_counter() - t0
+        c = sim.counters()
+        out[name] = {"voxels": n, "steps": steps, "value": n * steps / wall, "unit": "voxel-steps/s",
+                     "ms_per_step": wall * 1e3 / steps, "particles": int(sim.species_counts()[1:].sum()),
+                     "fired": int(c[3]), "cme_events": int(c[5]), "exceptions": int(c[1])}
+        log(f"{name}: {wall * 1e3 / steps:.4f} ms/step")
+        sim.close()
+    return out
 
 
 def main() -> int:
@@ -159,15 +253,20 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=2000, help="timed steps (the 256^3 workload's run length: 2000)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--edge", type=int, default=256)
-    ap.add_argument("--cpu-steps", type=int, default=11,
-                    help="oracle steps for the CPU baseline: one warm-up, the rest timed (0 = skip)")
+    ap.add_argument("--cpu-steps", type=int, default=10,
+                    help="timed steps of the CPU baseline on the 256^3 workload, after one warm-up step (0 = skip)")
+    ap.add_argument("--cpu-cfg1", type=int, default=1, help="also time the CPU baseline on the 64^3 cell (1/0)")
+    ap.add_argument("--configs", type=int, default=1,
+                    help="also time the 64^3 and 128^3 single-cell workloads on the GPU (1/0)")
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "large_cell_256_traffic.json"),
                     help="per-launch HBM bytes measured with rocprofv3 --pmc (FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--ensemble-cells", type=int, default=512,
                     help="replicate cells of the batched-vs-plain comparison (0 = skip it)")
     ap.add_argument("--ensemble-steps", type=int, default=5000,
                     help="timed steps of the batched ensemble (the workload's run length: 5000 steps in 100-step graphs)")
-    ap.add_argument("--plain-steps", type=int, default=10, help="timed steps of the per-cell plain launches")
+    ap.add_argument("--plain-steps", type=int, default=100, help="timed steps of the per-cell plain launches")
+    ap.add_argument("--populated", type=int, default=1,
+                    help="also run the batched leg with the cells started near their stationary state (1/0)")
     ap.add_argument("--ensemble-only", action="store_true",
                     help="run only the batched-ensemble leg and print its record (for profiling it)")
     ap.add_argument("--ensemble-traffic-json", default=os.path.join(ROOT, "profiles", "ensemble_512_traffic.json"),
@@ -191,7 +290,7 @@ def main() -> int:
     from cell_simulation_amd import configs, default_itime_schedule
 
     if args.ensemble_only:
-        print(json.dumps({"batched": ensemble_leg(args)}), flush=True)
+        print(json.dumps({"batched": ensemble_leg(args, populated=bool(args.populated))}), flush=True)
         return 0
 
     log(f"building large_cell edge={args.edge}")
@@ -239,25 +338,15 @@ def main() -> int:
             traffic = tj.get("bytes_per_launch")
 
     value = N * args.steps / wall
-    cpu = None
-    if args.cpu_steps > 0:
-        from oracle.cs_oracle import Oracle
-
-        log(f"cpu baseline: oracle, {args.cpu_steps} steps")
-        orc = Oracle.from_simulation(configs.large_cell(seed=1234, edge=args.edge))
-        ci = default_itime_schedule(1234, args.cpu_steps)
-        orc.step(int(ci[0]))  # warm-up step (page-in), not timed
-        t1 = time.perf_counter()
-        orc.run(ci[1:])
-        ct = time.perf_counter() - t1
-        n_cpu = args.cpu_steps - 1
-        cpu = {"value": N * n_cpu / ct if n_cpu else None, "unit": "voxel-steps/s", "cores": 1, "kind": "port",
-               "sample": f"{n_cpu} step(s) of the {args.edge}^3 S={S} large-cell workload after one warm-up step, "
-                         "serial C oracle (canonical order of the reference shaders)"}
+    sim.close()
+    cpu = cpu_leg(args, N, S) if args.cpu_steps > 0 else None
+    single = single_cell_configs(args) if args.configs else None
 
     batched = None
     if args.ensemble_cells > 0:
         batched = ensemble_leg(args)
+        if args.populated:
+            batched["populated_run"] = ensemble_leg(args, populated=True)
     precision = precision_leg(args.precision) if args.precision > 0 else None
 
     out = {
@@ -282,7 +371,9 @@ def main() -> int:
                      "avg_kernel_us": avg_kernel_s * 1e6},
         "cpu_baseline": cpu,
         "diagnostics": {"tight_voxel_steps": int(counters[0]), "exceptions": int(counters[1]),
-                        "near_threshold": int(counters[2]), "fired": int(counters[3])},
+                        "near_threshold": int(counters[2]), "fired": int(counters[3]),
+                        "failed_claims": int(counters[8]), "slot_s_aliases": int(counters[9])},
+        "configs": single,
         "batched": batched,
         "precision": precision,
     }

@@ -27,7 +27,9 @@ CS_CNT_ROLLBACK = 4
 CS_CNT_CME_EVENTS = 5
 CS_CNT_CME_NEAR = 6
 CS_CNT_EMIT_FAIL = 7
-CS_NUM_COUNTERS = 8
+CS_CNT_FAILED_CLAIM = 8
+CS_CNT_ALIAS = 9
+CS_NUM_COUNTERS = 10
 
 
 class LatticeParamsC(C.Structure):
@@ -156,6 +158,8 @@ SIGNATURES = {
     "cs_ens_cell_res": (C.c_int, [_SIM, C.c_uint32, C.POINTER(C.c_uint32)]),
     "cs_ens_read_lattice": (C.c_int, [_SIM, C.c_uint32, C.c_void_p, C.c_size_t]),
     "cs_ens_read_cme_counts": (C.c_int, [_SIM, C.c_uint32, C.c_void_p, C.c_size_t]),
+    "cs_ens_read_concentrations": (C.c_int, [_SIM, C.c_uint32, C.c_void_p, C.c_size_t]),
+    "cs_ens_read_occupancy": (C.c_int, [_SIM, C.c_uint32, C.c_void_p, C.c_size_t]),
     "cs_ens_read_counters": (C.c_int, [_SIM, C.c_void_p, C.c_size_t]),
     "cs_ens_reset_counters": (C.c_int, [_SIM]),
     "cs_sim_read_cme_audit": (C.c_int, [_SIM, C.POINTER(C.c_double), C.POINTER(C.c_uint64),

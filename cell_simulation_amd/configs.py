@@ -13,9 +13,11 @@
   32..96 voxels, every rate constant (and the diffusion constants) perturbed per replicate by a
   log-normal factor (sigma 0.2), S=8, seed 99; per-cell itime seeds for an Ensemble.
 * `precision_stress` -- 512^3 lattice, S=8, 32 nm voxels, dt = 5 us; the large-cell regions; 12 lattice
-  species (about 10M particles in the cytoplasm and nucleoid) with region-dependent D, 40 well-mixed species; 120 reactions in total
-  (40 per-voxel lattice reactions, 72 well-mixed reactions, 8 well-mixed reactions that emit lattice
-  particles) with rates log-uniform over 1e-6..1e4 /s; seed 2024. `edge` scales it down for tests.
+  species (about 10M particles in the cytoplasm and nucleoid) with region-dependent D, 40 well-mixed species
+  (10..2000 copies each); 120 reactions in total: 40 per-voxel lattice reactions with rate constants
+  log-uniform over 1e-6..1e1 /s, 72 well-mixed reactions and 8 well-mixed reactions that emit lattice particles,
+  whose propensities at the initial counts are log-uniform over 1e-6..1e4 /s; seed 2024. `edge` scales it
+  down for tests.
 * `gene_expression` -- 128^3 lattice, 16 nm voxels, S=8, dt = 20 us; extracellular space, a membrane shell
   one voxel thick, cytoplasm and a nucleoid sphere of radius 20 voxels; well-mixed gene_off / gene_on
   switching (0.01 /s on, 0.05 /s off) with transcription of mRNA into the nucleoid at 0.5 /s while on;
@@ -121,11 +123,15 @@ GENE_REGIONS = ("background", "membrane", "cytoplasm", "nucleoid")
 
 def gene_expression(seed: int = 7, edge: int = 128, slots: int = 8, tau: float = 20e-6, lam: float = 16e-9,
                     rate_scale: float = 1.0, rate_factors=None, nucleoid_radius: float = 20.0,
-                    gene_copies: int = 1) -> Simulation:
+                    gene_copies: int = 1, populated: bool = False) -> Simulation:
     """Hybrid gene expression: well-mixed gene switching, lattice mRNA and protein.
 
     `rate_factors` (optional sequence of 7 positive factors) multiplies, in order: gene on, gene off,
-    transcription, mRNA degradation, translation, protein degradation, and both diffusion constants."""
+    transcription, mRNA degradation, translation, protein degradation, and both diffusion constants.
+    `populated` starts the cell near its stationary state instead of an empty lattice with the genes off: each
+    gene copy on with probability p_on = k_on / (k_on + k_off) (drawn from `seed`), copies * p_on *
+    transcription / mRNA decay mRNA in the nucleoid and that times translation / protein decay proteins in the
+    cytoplasm (the means of the switching model)."""
     f = list(rate_factors) if rate_factors is not None else [1.0] * 7
     box = edge * lam
     sim = Simulation(LatticeParams([box] * 3, [edge] * 3, tau, lam, slots))
@@ -136,8 +142,16 @@ def gene_expression(seed: int = 7, edge: int = 128, slots: int = 8, tau: float =
     sim.add_region(RegionType.SphericalShell("membrane", "cytoplasm", c, r_int, r_ext), 0.0)
     sim.add_region(RegionType.Sphere("nucleoid", c, nucleoid_radius * lam * edge / 128), 0.0)
     sim.prepare_regions()
-    sim.add_particle_count("mRNA", "nucleoid", 0, True, False)
-    sim.add_particle_count("protein", "cytoplasm", 0, True, False)
+    n_mrna = n_prot = n_on = 0
+    if populated:
+        p_on = 0.01 * f[0] / (0.01 * f[0] + 0.05 * f[1])
+        draw = np.random.Generator(np.random.PCG64(seed + 0x5EED))
+        n_on = int(np.count_nonzero(draw.uniform(size=gene_copies) < p_on))
+        m = gene_copies * p_on * (0.5 * f[2]) / (0.01 * f[3])
+        n_mrna = int(round(m))
+        n_prot = int(round(m * (0.2 * f[4]) / (0.001 * f[5])))
+    sim.add_particle_count("mRNA", "nucleoid", n_mrna, True, False)
+    sim.add_particle_count("protein", "cytoplasm", n_prot, True, False)
     d = {"mRNA": 5e-14 * f[6], "protein": 2.5e-12 * f[6]}
     for sp, dc in d.items():
         for reg in GENE_REGIONS:
@@ -148,8 +162,8 @@ def gene_expression(seed: int = 7, edge: int = 128, slots: int = 8, tau: float =
     sim.add_reaction(["mRNA"], [], 0.01 * k * f[3])
     sim.add_reaction(["mRNA"], ["mRNA", "protein"], 0.2 * k * f[4])
     sim.add_reaction(["protein"], [], 0.001 * k * f[5])
-    sim.add_cme_species("gene_off", gene_copies)
-    sim.add_cme_species("gene_on", 0)
+    sim.add_cme_species("gene_off", gene_copies - n_on)
+    sim.add_cme_species("gene_on", n_on)
     sim.add_cme_reaction(["gene_off"], ["gene_on"], 0.01 * k * f[0])
     sim.add_cme_reaction(["gene_on"], ["gene_off"], 0.05 * k * f[1])
     sim.add_cme_reaction(["gene_on"], ["gene_on", "mRNA@nucleoid"], 0.5 * k * f[2])
@@ -157,8 +171,8 @@ def gene_expression(seed: int = 7, edge: int = 128, slots: int = 8, tau: float =
 
 
 def ensemble_cells(n_cells: int = 512, seed: int = 99, edge_range=(32, 96), sigma: float = 0.2,
-                   rate_scale: float = 1.0, slots: int = 8):
-    """Replicate cells for a batched ensemble: returns (cells, itime_seeds)."""
+                   rate_scale: float = 1.0, slots: int = 8, populated: bool = False):
+    """Replicate cells for a batched ensemble: returns (cells, itime_seeds). `populated`: see gene_expression."""
     rng = np.random.Generator(np.random.PCG64(seed))
     lo, hi = edge_range
     edges = np.exp(rng.uniform(np.log(lo), np.log(hi), size=n_cells)).round().astype(int)
@@ -166,7 +180,7 @@ def ensemble_cells(n_cells: int = 512, seed: int = 99, edge_range=(32, 96), sigm
     for i, e in enumerate(edges):
         factors = np.exp(rng.normal(0.0, sigma, size=7))
         cells.append(gene_expression(seed=seed * 100003 + i, edge=int(e), slots=slots, rate_scale=rate_scale,
-                                     rate_factors=factors))
+                                     rate_factors=factors, populated=populated))
         seeds.append(int(rng.integers(0, 2**32)))
     return cells, seeds
 
@@ -194,8 +208,19 @@ def precision_stress(seed: int = 2024, edge: int = 512, slots: int = 8, n_partic
         for reg in LARGE_CELL_REGIONS:
             sim.set_diffusion_rate_particle(nm, reg, loguniform(1e-14, 1e-11))
     cme = [f"W{i:02d}" for i in range(n_cme)]
+    init = {}
     for nm in cme:
-        sim.add_cme_species(nm, int(rng.integers(10, 2000)))
+        init[nm] = int(rng.integers(10, 2000))
+        sim.add_cme_species(nm, init[nm])
+
+    def rate_for(reac, a):
+        """k giving propensity a at the initial counts: a = k * h, h the product of binomial(count, multiplicity)
+        over distinct reactants (the well-mixed propensity's definition)."""
+        h = 1.0
+        for nm in set(reac):
+            m, n = reac.count(nm), init[nm]
+            h *= float(np.prod([n - j for j in range(m)])) / float(np.prod(range(1, m + 1)))
+        return a / h
     n_lattice = n_reactions // 3
     n_emit = n_reactions // 15
     n_wm = n_reactions - n_lattice - n_emit
@@ -206,13 +231,14 @@ def precision_stress(seed: int = 2024, edge: int = 512, slots: int = 8, n_partic
         else:  # two distinct species (k*n^2 of a homodimer row fires on a single particle)
             b = names[(names.index(a) + 1 + int(rng.integers(n_rdme - 1))) % n_rdme]
             sim.add_reaction([a, b], [names[int(rng.integers(n_rdme))]], loguniform(1e-6, 1e1))
+    # well-mixed reactions: initial propensities log-uniform over 1e-6..1e4 /s
     for _ in range(n_wm):
         order = 1 if rng.uniform() < 0.7 else 2
         reac = [cme[int(i)] for i in rng.integers(0, n_cme, size=order)]
         prods = [cme[int(i)] for i in rng.integers(0, n_cme, size=int(rng.integers(0, 3)))]
-        k = loguniform(1e-6, 1e4)
-        sim.add_cme_reaction(reac, prods, k / 1000.0 if order == 2 else k / 100.0)
+        sim.add_cme_reaction(reac, prods, rate_for(reac, loguniform(1e-6, 1e4)))
     for _ in range(n_emit):
         src = cme[int(rng.integers(n_cme))]
-        sim.add_cme_reaction([src], [src, f"{names[int(rng.integers(n_rdme))]}@cytoplasm"], loguniform(1e-6, 1e2))
+        sim.add_cme_reaction([src], [src, f"{names[int(rng.integers(n_rdme))]}@cytoplasm"],
+                             rate_for([src], loguniform(1e-6, 1e4)))
     return sim

@@ -621,12 +621,11 @@ int cs_sim_read_concentrations(cs_sim *sim, int32_t *out, size_t n)
         if (n != N * n1)
             throw cs::Error(CS_ERR_ARG, "concentration buffer must hold N*(ns+1) entries");
         std::vector<uint8_t> lat(N * S);
-        std::vector<uint32_t> vox;
-        std::vector<int32_t> rs;
+        std::vector<cs::SkewVoxel> sk;
         int32_t c0 = 0;
         if (sim->device) {
             sim->device->read_lattice(lat.data());
-            sim->device->read_reservoir_state(vox, rs);
+            sim->device->read_skew(sk);
             c0 = sim->device->steps() > 0u ? 1 : 0;  // every step sets C[v][0] = 1 (cme.wgsl:31)
         } else {
             std::memcpy(lat.data(), m.lattice().data(), lat.size());
@@ -639,14 +638,11 @@ int cs_sim_read_concentrations(cs_sim *sim, int32_t *out, size_t n)
                 if (lat[v * S + k])
                     c[lat[v * S + k]] += 1;
         }
-        if (sim->device) {
-            for (size_t i = 0; i < vox.size(); ++i) {
-                const uint32_t sp = uint32_t(rs[4 * i]), K = sp & 0xFFu, R = sp >> 8;
-                if (K)
-                    out[uint64_t(vox[i]) * n1 + K] += rs[4 * i + 2];
-                if (R)
-                    out[uint64_t(vox[i]) * n1 + R] += rs[4 * i + 1];
-            }
+        if (sim->device) {  // what the skewed voxels' counts hold beyond their slots
+            for (const cs::SkewVoxel &e : sk)
+                for (int q = 0; q < 4; ++q)
+                    if (e.sp[q] != 0u && e.sp[q] < n1)
+                        out[uint64_t(e.v) * n1 + e.sp[q]] += e.dc[q];
         } else {
             const auto &res = m.reservoir();
             for (uint64_t v = 0; v < N; ++v)
@@ -666,11 +662,10 @@ int cs_sim_read_occupancy(cs_sim *sim, uint32_t *out, size_t n)
         if (n != N)
             throw cs::Error(CS_ERR_ARG, "occupancy buffer must hold N entries");
         std::vector<uint8_t> lat(N * S);
-        std::vector<uint32_t> vox;
-        std::vector<int32_t> rs;
+        std::vector<cs::SkewVoxel> sk;
         if (sim->device) {
             sim->device->read_lattice(lat.data());
-            sim->device->read_reservoir_state(vox, rs);
+            sim->device->read_skew(sk);
         } else {
             std::memcpy(lat.data(), m.lattice().data(), lat.size());
         }
@@ -680,8 +675,8 @@ int cs_sim_read_occupancy(cs_sim *sim, uint32_t *out, size_t n)
                 c += lat[v * S + k] != 0u;
             out[v] = c;
         }
-        for (size_t i = 0; i < vox.size(); ++i)
-            out[vox[i]] += uint32_t(rs[4 * i + 3]);  // u32 wrap-around, as the reference's atomics
+        for (const cs::SkewVoxel &e : sk)
+            out[e.v] += uint32_t(e.off);  // u32 wrap-around, as the reference's atomics
     });
 }
 
@@ -935,6 +930,59 @@ int cs_ens_read_cme_counts(cs_ens *ens, uint32_t cell, int32_t *out, size_t n)
         if (n)
             need(out, "out");
         e.read_cme_counts(cell, out);
+    });
+}
+
+int cs_ens_read_concentrations(cs_ens *ens, uint32_t cell, int32_t *out, size_t n)
+{
+    return guard([&] {
+        need(out, "out");
+        cs::Ensemble &e = ens_of(ens);
+        const uint64_t N = e.cell_voxels(cell);
+        const uint32_t S = e.slots(), n1 = e.cell_species(cell) + 1u;
+        if (n != N * n1)
+            throw cs::Error(CS_ERR_ARG, "concentration buffer must hold the cell's N*(ns+1) entries");
+        std::vector<uint8_t> lat(N * S);
+        std::vector<cs::SkewVoxel> sk;
+        e.read_lattice(cell, lat.data());
+        e.read_skew(cell, sk);
+        const int32_t c0 = e.steps_done() > 0u ? 1 : 0;  // every step sets C[v][0] = 1 (cme.wgsl:31)
+        std::memset(out, 0, n * sizeof(int32_t));
+        for (uint64_t v = 0; v < N; ++v) {
+            int32_t *c = out + v * n1;
+            c[0] = c0;
+            for (uint32_t k = 0; k < S; ++k)
+                if (lat[v * S + k])
+                    c[lat[v * S + k]] += 1;
+        }
+        for (const cs::SkewVoxel &x : sk)
+            for (int q = 0; q < 4; ++q)
+                if (x.sp[q] != 0u && x.sp[q] < n1)
+                    out[uint64_t(x.v) * n1 + x.sp[q]] += x.dc[q];
+    });
+}
+
+int cs_ens_read_occupancy(cs_ens *ens, uint32_t cell, uint32_t *out, size_t n)
+{
+    return guard([&] {
+        need(out, "out");
+        cs::Ensemble &e = ens_of(ens);
+        const uint64_t N = e.cell_voxels(cell);
+        const uint32_t S = e.slots();
+        if (n != N)
+            throw cs::Error(CS_ERR_ARG, "occupancy buffer must hold the cell's N entries");
+        std::vector<uint8_t> lat(N * S);
+        std::vector<cs::SkewVoxel> sk;
+        e.read_lattice(cell, lat.data());
+        e.read_skew(cell, sk);
+        for (uint64_t v = 0; v < N; ++v) {
+            uint32_t c = 0;
+            for (uint32_t k = 0; k < S; ++k)
+                c += lat[v * S + k] != 0u;
+            out[v] = c;
+        }
+        for (const cs::SkewVoxel &x : sk)
+            out[x.v] += uint32_t(x.off);  // u32 wrap-around, as the reference's atomics
     });
 }
 

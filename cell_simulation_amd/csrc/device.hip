@@ -37,8 +37,7 @@ namespace {
 
 constexpr int kMaxRW = 4;  // reaction bitmask words: up to 255 reactions
 constexpr int kBlock = 256;
-constexpr uint32_t kMaxTight = 1024;  // tight voxels replayed exactly per step; more are counted as exceptions
-constexpr uint32_t kFixCtl = 16;      // control words: [0] tight voxels noted, [1] blocks done, [8..15] queues
+constexpr uint32_t kFixCtl = 16;      // control words: [0] voxels noted for the replay, [2] replay scratch used, [8..15] queues
 constexpr uint32_t kFixQueue = 8;
 
 void hip_check(hipError_t e, const char *what)
@@ -46,18 +45,6 @@ void hip_check(hipError_t e, const char *what)
     if (e != hipSuccess)
         throw Error(CS_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
-
-// One reservoir voxel (lattice.rs:292-300, cme.wgsl:76-110). The reference's per-voxel counts C[v][sp]
-// equal the slot contents everywhere except at voxels touched by reservoirs: a reservoir site v of species
-// R holds one unit of C[v][R] that no slot holds, and the reaction at v skips the species K =
-// reservoirs[v*S] (indexed by the lattice index v*S; a read past the end reads 0). There C - slots is kept
-// for two species, and the occupancy word's offset from the slot count.
-struct ResVoxel {
-    uint32_t sp;  // K | R << 8; R = 0 when R == K (its unit is counted in gk)
-    int32_t gr;   // C[v][R] - slots holding R
-    int32_t gk;   // C[v][K] - slots holding K
-    int32_t off;  // occupancy[v] - occupied slots (u32 arithmetic in the reference)
-};
 
 // Occupancy summary of one lattice buffer, kept for the wave kernel: byte ((y / kSumTY) * tiles_z + z / kSumTZ)
 // * planes + x is non-zero whenever that wave tile's voxels of plane x may hold particles (a non-zero byte may be
@@ -90,12 +77,8 @@ struct StepArgs {
     const uint64_t *rmask;
     const float *bound;
     unsigned long long *counters;
-    uint32_t *fix_ctl;    // [0] tight voxels noted this step, [1] blocks finished
-    uint32_t *fix_list;   // tight voxels (max_tight)
-    uint32_t *fix_F;      // affected voxels (8 * max_tight)
-    uint32_t *fix_W;      // sources acting on them (56 * max_tight)
-    uint8_t *fix_L;       // slots of the affected voxels during the replay
-    uint32_t *fix_O;      // their occupancy during the replay
+    uint32_t *fix_ctl;    // [0] voxels noted this step for the replay, [kFixQueue..] the wave kernel's queues
+    uint32_t *fix_list;   // voxels noted (max_tight), as indices among all voxels of the device state
     uint32_t max_tight;
     uint32_t vox_base;    // index of this lattice's voxel 0 among all voxels on the device (ensembles)
     uint32_t R0, R1, R2, R12;
@@ -104,11 +87,7 @@ struct StepArgs {
     uint32_t itime;
     uint32_t mod_zero;
     float tau;
-    const uint32_t *res_bits;  // [N/32] reservoir voxels (their reaction runs in reservoir_react_kernel), or null
-    uint32_t *res_skew;        // [N/32] reservoir voxels whose occupancy word differs from the slot count
-    const uint32_t *res_v;     // [res_n] reservoir voxel indices, ascending
-    ResVoxel *res_st;          // [res_n]
-    uint32_t res_n;
+    const uint32_t *live;      // bit per voxel (index vox_base + v): a skewed voxel, whose reaction the replay runs
     SumRef sum_dst;            // summary of dst, marked by the passes after the step kernel
 };
 
@@ -391,8 +370,8 @@ __device__ __forceinline__ void react(const StepArgs &a, const Tab &tb, Slots<S>
     // above the bound's reject line is above the exact one below as well: nothing fires, nothing is near.
     if (r1 > tb.bound[cnt] * a.tau * 1.001f + 1e-6f)
         return;
-    if (a.res_bits != nullptr && voxel_bit(a.res_bits, z + y * a.R2 + x * a.R12))
-        return;  // a reservoir voxel: its reaction runs in reservoir_react_kernel after the step
+    if (voxel_bit(a.live, a.vox_base + z + y * a.R2 + x * a.R12))
+        return;  // a skewed voxel: its reaction runs in the replay kernel after the step
     const uint32_t words = wave_words<S>(cnt);  // slot words holding particles in any lane still here
 
     uint64_t cand[kMaxRW];
@@ -623,69 +602,283 @@ __device__ __forceinline__ Slots<S> gather_voxel(const StepArgs &a, const Tab &t
     if (z + 1u < a.R2) nhi = gather_from<S>(a, tb, x, y, z + 1u, v + 1u, 4u, hi, nhi);  // from +z, hopped -z
     if (y + 1u < a.R1) nhi = gather_from<S>(a, tb, x, y + 1u, z, v + a.R2, 2u, hi, nhi);  // from +y, hopped -y
     if (x + 1u < a.R0) nhi = gather_from<S>(a, tb, x + 1u, y, z, v + a.R12, 0u, hi, nhi);  // from +x, hopped -x
-    // a reservoir voxel whose occupancy word is off the slot count can refuse a claim that fits (rdme.wgsl:62)
-    // and walks that many slots (rdme.wgsl:213): replayed like a tight one
-    f.tight = occ + nlo + nhi > uint32_t(S) ||
-              (a.res_skew != nullptr && occ + nlo + nhi != 0u && voxel_bit(a.res_skew, v));
+    f.tight = occ + nlo + nhi > uint32_t(S);
     uint32_t cnt = 0;
     Slots<S> out = assemble<S>(own, occ, stay, lo, nlo, hi, nhi, cnt);
     react<S>(a, tb, out, cnt, x, y, z, f.fired, f.rolled, f.near, f.bad);
     return out;
 }
 
-// ---- exact step around tight voxels -----------------------------------------------------------------
-// A tight voxel t (own particles + intended arrivals > S) is where the gather may differ from the serial
-// order: a claim can find t full (rdme.wgsl:62, 30-40). Claims into any other voxel always succeed. So
-// only F = T + the six neighbours of T + the voxel after t in memory (slot S of t aliases its slot 0) can
-// end differently, and only sources in W = F + the neighbours of F act on F. The last block of the step
-// replays W in canonical order on a private copy of F's slots (the scatter form of rdme.wgsl:45-88), then
-// applies the reaction to F and overwrites F's output. Everything outside F was exact already.
-__device__ __forceinline__ uint32_t neighbor_or_none(const StepArgs &a, uint32_t v, int k)
+// ---- exact step where the gather cannot be exact ------------------------------------------------------------
+// The reference keeps, beside the lattice, an occupancy word O[v] and per-species counts C[v][sp] for every
+// voxel (rdme.wgsl:61-71, cme.wgsl:85-114). Here both are read off the byte slots (O = occupied slots, C[sp] =
+// slots holding sp) except at "skewed" voxels, where the reference's words leave that picture and the state
+// keeps the difference:
+//  * the slot-S alias (rdme.wgsl:30-40, 61-62): a claim into a full voxel d lands in slot 0 of voxel d+1; O[d]
+//    and C[d][sp] count it while the slots of d+1 hold it;
+//  * a sweep walks O[w] slots from w*S (rdme.wgsl:212-213): past slot S it moves the next voxel's particles and
+//    counts them off w; below the occupied slots it leaves the last particles where they are;
+//  * a reaction can take a count below zero (k*n^2 of a homodimer row fires on one particle, cme.wgsl:42-44,
+//    85-91), and the negative count, which no slot holds, enters later propensities;
+//  * a reservoir site holds one unit of C[v][R] in no slot (lattice.rs:292-300), and the reaction skips the
+//    species K = reservoirs[v*S] (cme.wgsl:80), whose slots it drops while C[v][K] stays.
+// A skewed voxel has an entry (occupancy offset, up to kSkewSp count offsets, K) in an open-addressing table
+// keyed by its voxel index; entries are never removed, a voxel whose offsets return to zero keeps a clean one.
+// A bitmap marks the voxels with a live entry: the step kernels leave their reaction to the replay kernel.
+//
+// The step kernels gather every voxel as if nothing were tight or skewed, and list the voxels where that may
+// be wrong: "tight" ones (own particles + intended arrivals > S, so a claim can find them full), and those whose
+// reaction takes a count below zero. One block after the step (tight_replay_kernel, or ens_epilogue for an
+// ensemble) adds every voxel whose occupancy word is off its slot count, sorts the list and cuts it into
+// clusters at gaps of kClusterGap planes along x. Each cluster is replayed by one lane, clusters side by side:
+// F = the listed voxels, their six neighbours and the voxel after each in memory (slot S aliases its slot 0)
+// is the set whose outcome may differ from the gather; the sources W = F and its neighbours act on F. The lane
+// runs the sweeps of W in canonical order (voxels ascending, slots ascending; the scatter form of
+// rdme.wgsl:45-88) on private copies of F's slots, occupancy words and count offsets, then the reaction
+// (cme.wgsl:25-120) and the compaction of each voxel of F, and overwrites F's output; every voxel outside F was
+// exact already. The replay checks the assumptions that make this hold, and a voxel outside F that turns out
+// to matter (a source whose claim into F fails, a slot-S alias into the voxel after F, a walk into a block
+// outside F) is added to F and the cluster replayed again. Clusters kClusterGap planes apart cannot reach each
+// other: F extends a cluster by at most kEscX planes, its sources one more.
+constexpr int kSkewSp = 4;                  // species with a count offset one voxel can hold
+constexpr uint32_t kSkewFree = 0xFFFFFFFFu;  // an unused key
+constexpr uint32_t kSkewStatic = 0x100u;     // a reservoir voxel: live for good
+constexpr uint32_t kSkewDone = 0x200u;       // replayed this step: its reaction has run
+constexpr uint32_t kReplayThreads = 256;
+constexpr uint32_t kSkewMargin = kReplayThreads;  // inserts stop this far below the entry capacity
+constexpr uint32_t kSortLds = 8192;          // noted voxels sorted in LDS (more: in global memory)
+constexpr int kEscMax = 16;                  // voxels a cluster may add to F
+constexpr int kEscX = 3;                     // planes F may reach beyond its cluster's noted voxels
+constexpr uint32_t kClusterGap = 2u * kEscX + 2u;
+
+struct SkewEntry {
+    uint32_t v;               // voxel index among all voxels of the device state
+    int32_t off;              // occupancy word - occupied slots (u32 arithmetic in the reference)
+    uint32_t sp;              // four species bytes (0: unused) whose count differs from the slots
+    uint32_t kf;              // bits 0-7: K = reservoirs[v*S]; kSkewStatic, kSkewDone
+    int32_t dc[kSkewSp];      // count - slots holding the species of byte i
+};
+
+struct SkewTab {
+    uint32_t *keys;   // [mask+1] voxel index or kSkewFree (linear probing)
+    uint32_t *idx;    // [mask+1] entry of that key
+    SkewEntry *ent;   // [ecap] entries in insertion order
+    uint32_t *ctl;    // [0] entries, [1] inserts refused (table full)
+    uint32_t *live;   // bit per voxel: a live entry, whose reaction the replay kernel runs
+    uint32_t mask, ecap;
+};
+
+__host__ __device__ inline uint32_t skew_hash(uint32_t v)
 {
-    const uint32_t z = v % a.R2, y = (v / a.R2) % a.R1, x = v / a.R12;
-    switch (k) {
-    case 0: return x > 0u ? v - a.R12 : 0xFFFFFFFFu;
-    case 1: return x + 1u < a.R0 ? v + a.R12 : 0xFFFFFFFFu;
-    case 2: return y > 0u ? v - a.R2 : 0xFFFFFFFFu;
-    case 3: return y + 1u < a.R1 ? v + a.R2 : 0xFFFFFFFFu;
-    case 4: return z > 0u ? v - 1u : 0xFFFFFFFFu;
-    case 5: return z + 1u < a.R2 ? v + 1u : 0xFFFFFFFFu;
-    case 6: return v;
-    default: return v + 1u < a.N ? v + 1u : 0xFFFFFFFFu;  // the voxel that slot S of v aliases
-    }
+    v ^= v >> 16;
+    v *= 0x85EBCA6Bu;
+    v ^= v >> 13;
+    v *= 0xC2B2AE35u;
+    v ^= v >> 16;
+    return v;
 }
 
-// out = sorted unique union over `nstreams` monotone maps of a sorted input. Returns the length, or
-// cap + 1 on overflow.
-__device__ __noinline__ uint32_t expand_sorted(const StepArgs &a, const uint32_t *in, uint32_t n, int nstreams, uint32_t *out,
-                                  uint32_t cap)
+__device__ inline int skew_find(const SkewTab &k, uint32_t v)
 {
-    uint32_t pos[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    uint32_t m = 0, last = 0xFFFFFFFFu;
-    for (;;) {
-        uint32_t best = 0xFFFFFFFFu;
-        int bk = -1;
-        for (int k = 0; k < nstreams; ++k) {
-            while (pos[k] < n && neighbor_or_none(a, in[pos[k]], k) == 0xFFFFFFFFu)
-                ++pos[k];
-            if (pos[k] < n) {
-                const uint32_t c = neighbor_or_none(a, in[pos[k]], k);
-                if (c < best) {
-                    best = c;
-                    bk = k;
-                }
+    uint32_t h = skew_hash(v) & k.mask;
+    for (uint32_t p = 0; p <= k.mask; ++p) {
+        const uint32_t key = k.keys[h];
+        if (key == v)
+            return int(k.idx[h]);
+        if (key == kSkewFree)
+            return -1;
+        h = (h + 1u) & k.mask;
+    }
+    return -1;
+}
+
+// The entry of voxel v, created clean when missing; -1 when the table is full. Keys are never removed, so a
+// probe for a key present stops at it; concurrent inserts are of different voxels.
+__device__ inline int skew_insert(const SkewTab &k, uint32_t v)
+{
+    uint32_t h = skew_hash(v) & k.mask;
+    for (uint32_t p = 0; p <= k.mask; ++p) {
+        uint32_t key = k.keys[h];
+        if (key == v)
+            return int(k.idx[h]);
+        if (key == kSkewFree) {
+            if (atomicAdd(k.ctl, 0u) + kSkewMargin >= k.ecap) {
+                atomicAdd(k.ctl + 1, 1u);
+                return -1;
             }
+            key = atomicCAS(k.keys + h, kSkewFree, v);
+            if (key == kSkewFree) {
+                const uint32_t i = atomicAdd(k.ctl, 1u);
+                SkewEntry e;
+                e.v = v;
+                e.off = 0;
+                e.sp = 0u;
+                e.kf = 0u;
+                for (int q = 0; q < kSkewSp; ++q)
+                    e.dc[q] = 0;
+                k.ent[i] = e;
+                k.idx[h] = i;
+                return int(i);
+            }
+            if (key == v)
+                return int(k.idx[h]);
         }
-        if (bk < 0)
-            return m;
-        ++pos[bk];
-        if (best != last) {
-            if (m >= cap)
-                return cap + 1u;
-            out[m++] = best;
-            last = best;
+        h = (h + 1u) & k.mask;
+    }
+    return -1;
+}
+
+__device__ __forceinline__ bool skew_live(const SkewEntry &e)
+{
+    return e.off != 0 || e.sp != 0u || (e.kf & (0xFFu | kSkewStatic)) != 0u;
+}
+
+__device__ __forceinline__ void set_voxel_bit(uint32_t *bits, uint32_t v, bool on)
+{
+    const uint32_t b = 1u << (v & 31u);
+    if (on)
+        atomicOr(bits + (v >> 5), b);
+    else
+        atomicAnd(bits + (v >> 5), ~b);
+}
+
+// Count offsets: species bytes `sp` with values dc[0..kSkewSp).
+__device__ __forceinline__ int32_t dc_get(uint32_t sp, const int32_t *dc, uint32_t s)
+{
+    int32_t d = 0;
+    for (int q = 0; q < kSkewSp; ++q)
+        if (((sp >> (8 * q)) & 0xFFu) == s && s != 0u)
+            d = dc[q];
+    return d;
+}
+
+// Adds d to the offset of species s; false when all kSkewSp places hold other species.
+__device__ inline bool dc_add(uint32_t &sp, int32_t *dc, uint32_t s, int32_t d)
+{
+    if (d == 0 || s == 0u)
+        return true;
+    for (int q = 0; q < kSkewSp; ++q)
+        if (((sp >> (8 * q)) & 0xFFu) == s) {
+            dc[q] += d;
+            if (dc[q] == 0)
+                sp &= ~(0xFFu << (8 * q));
+            return true;
+        }
+    for (int q = 0; q < kSkewSp; ++q)
+        if (((sp >> (8 * q)) & 0xFFu) == 0u) {
+            sp |= s << (8 * q);
+            dc[q] = d;
+            return true;
+        }
+    return false;
+}
+
+template <int S>
+__device__ __forceinline__ Slots<S> set_byte(const Slots<S> &s, uint32_t j, uint32_t val)
+{
+    return from_bv<S>(bv_set(to_bv<S>(s), j, val));
+}
+
+// Stable partition of the non-zero slots to the front (final_stage.wgsl:25-33).
+template <int S>
+__device__ __forceinline__ Slots<S> compact(const Slots<S> &s)
+{
+    Slots<S> out = zero_slots<S>();
+    uint32_t j = 0;
+    for (uint32_t i = 0; i < uint32_t(S); ++i) {
+        const uint32_t b = get_byte_dyn(s, i);
+        if (b) {
+            put_byte_dyn(out, j, b);
+            ++j;
         }
     }
+    return out;
+}
+
+// One voxel with the reference's counters: its block of slots, occupancy word and count offsets.
+template <int S>
+struct XVox {
+    Slots<S> sl;
+    uint32_t occ;
+    uint32_t sp;
+    int32_t dc[kSkewSp];
+};
+
+// cme.wgsl:25-120 then final_stage.wgsl:25-33 on one voxel, with C[s] = slots holding s + offset, C[0] = 1, the
+// occupancy word, and K (reservoirs[v*S]) skipped by the rewrite. The voxel leaves compacted, its offsets
+// updated: after a rewrite K keeps its whole count in no slot and every other species keeps only a count below
+// zero; a rollback subtracts the stoichiometry up to the species being written, K included (cme.wgsl:102-106).
+// `bad` is set when the offsets need more than kSkewSp species.
+template <int S>
+__device__ void exact_react(const StepArgs &a, XVox<S> &vx, uint32_t K, uint32_t x, uint32_t y, uint32_t z, bool &fired,
+                            bool &rolled, bool &near, bool &bad)
+{
+    fired = rolled = near = false;
+    auto C = [&](uint32_t s) -> int32_t {
+        return s == 0u ? 1 : int32_t(count_species(vx.sl, s)) + dc_get(vx.sp, vx.dc, s);
+    };
+    auto prop = [&](uint32_t r) -> float {
+        const uint32_t ix = reinterpret_cast<const uint32_t *>(a.ridx)[r];
+        const uint32_t p = uint32_t(C(ix & 0xFFu)) * uint32_t(C((ix >> 8) & 0xFFu)) * uint32_t(C((ix >> 16) & 0xFFu));
+        return a.rates[r] * float(int32_t(p));  // i32 product (cme.wgsl:42-44), wrapping
+    };
+    float tot = 0.0f;
+    for (uint32_t r = 1; r <= a.nr; ++r)
+        tot = tot + prop(r);
+    const uint32_t P = pcg_hash(a.itime + x * y + z);
+    const float r1 = uniform_float(P);
+    const float xt = tot * a.tau;
+    const float thr = 1.0f - float(exp(double(-xt)));
+    near = xt != 0.0f && fabsf(r1 - thr) <= 5.9604644775390625e-08f;  // exp(-0) = 1 exactly
+    if (r1 <= thr) {
+        fired = true;
+        // first i with r2 <= cum[i] / tot, i < nr (cme.wgsl:50-52, 68-73): the same f32 sums in the same order
+        const float r2 = uniform_float(P + 1u);
+        uint32_t i = 0;
+        float cum = 0.0f;
+        while (i < a.nr && r2 > cum / tot) {
+            ++i;
+            cum = cum + prop(i);
+        }
+        const int8_t *row = a.stoich + size_t(i) * (a.ns + 1u);
+        Slots<S> out = zero_slots<S>();
+        uint32_t j = 0, sp_stop = 0;
+        int32_t occ_new = int32_t(vx.occ);
+        for (uint32_t s = 1; s <= a.ns; ++s) {
+            if (s == K)
+                continue;  // reservoirs[idx_lattice] == species (cme.wgsl:80-83)
+            const int32_t c = C(s) + int32_t(row[s]);
+            occ_new += int32_t(row[s]);
+            if (c > 0) {
+                if (j + uint32_t(c) >= uint32_t(S)) {  // the write that reaches S slots rolls back (cme.wgsl:95)
+                    sp_stop = s;
+                    break;
+                }
+                for (int32_t t = 0; t < c; ++t)
+                    put_byte_dyn(out, j++, s);
+            }
+        }
+        if (sp_stop != 0u) {
+            rolled = true;
+            if (K != 0u && K <= sp_stop && !dc_add(vx.sp, vx.dc, K, -int32_t(row[K])))
+                bad = true;
+        } else {
+            uint32_t nsp = 0u;
+            int32_t ndc[kSkewSp] = {0, 0, 0, 0};
+            for (uint32_t s = 1; s <= a.ns; ++s) {
+                const int32_t c = C(s);
+                const int32_t d = s == K ? c : min(c + int32_t(row[s]), 0);
+                if (!dc_add(nsp, ndc, s, d))
+                    bad = true;
+            }
+            vx.sl = out;
+            vx.occ = uint32_t(occ_new);
+            vx.sp = nsp;
+            for (int q = 0; q < kSkewSp; ++q)
+                vx.dc[q] = ndc[q];
+        }
+    }
+    vx.sl = compact<S>(vx.sl);
 }
 
 __device__ __forceinline__ int find_sorted(const uint32_t *arr, uint32_t n, uint32_t v)
@@ -701,13 +894,50 @@ __device__ __forceinline__ int find_sorted(const uint32_t *arr, uint32_t n, uint
     return (lo < n && arr[lo] == v) ? int(lo) : -1;
 }
 
-// Offset of the reference's occupancy word from the slot count of voxel v (non-zero at reservoir voxels only).
-__device__ __forceinline__ int32_t res_offset(const StepArgs &a, uint32_t v)
+__device__ __forceinline__ uint32_t neighbor_or_none(const StepArgs &a, uint32_t v, int k)
 {
-    if (a.res_n == 0u)
-        return 0;
-    const int i = find_sorted(a.res_v, a.res_n, v);
-    return i < 0 ? 0 : a.res_st[i].off;
+    const uint32_t z = v % a.R2, y = (v / a.R2) % a.R1, x = v / a.R12;
+    switch (k) {
+    case 0: return x > 0u ? v - a.R12 : 0xFFFFFFFFu;
+    case 1: return x + 1u < a.R0 ? v + a.R12 : 0xFFFFFFFFu;
+    case 2: return y > 0u ? v - a.R2 : 0xFFFFFFFFu;
+    case 3: return y + 1u < a.R1 ? v + a.R2 : 0xFFFFFFFFu;
+    case 4: return z > 0u ? v - 1u : 0xFFFFFFFFu;
+    case 5: return z + 1u < a.R2 ? v + 1u : 0xFFFFFFFFu;
+    case 6: return v;
+    default: return v + 1u < a.N ? v + 1u : 0xFFFFFFFFu;  // the voxel that slot S of v aliases
+    }
+}
+
+// Ascending, duplicate-free union of the monotone maps k = 0 .. nstreams-1 of neighbor_or_none over the sorted
+// list `in`, merged with the sorted list `extra`. Writes it to `out` when given; returns its length.
+__device__ uint32_t merge_hood(const StepArgs &a, const uint32_t *in, uint32_t n, int nstreams, const uint32_t *extra,
+                               uint32_t n_extra, uint32_t *out)
+{
+    uint32_t pos[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    uint32_t pe = 0, m = 0, last = 0xFFFFFFFFu;
+    for (;;) {
+        uint32_t best = pe < n_extra ? extra[pe] : 0xFFFFFFFFu;
+        for (int k = 0; k < nstreams; ++k) {
+            while (pos[k] < n && neighbor_or_none(a, in[pos[k]], k) == 0xFFFFFFFFu)
+                ++pos[k];
+            if (pos[k] < n)
+                best = min(best, neighbor_or_none(a, in[pos[k]], k));
+        }
+        if (best == 0xFFFFFFFFu)
+            return m;
+        for (int k = 0; k < nstreams; ++k)
+            if (pos[k] < n && neighbor_or_none(a, in[pos[k]], k) == best)
+                ++pos[k];
+        if (pe < n_extra && extra[pe] == best)
+            ++pe;
+        if (best != last) {
+            if (out)
+                out[m] = best;
+            ++m;
+            last = best;
+        }
+    }
 }
 
 __device__ __forceinline__ void counter_add(unsigned long long *c, int idx, long long d)
@@ -716,135 +946,384 @@ __device__ __forceinline__ void counter_add(unsigned long long *c, int idx, long
         atomicAdd(c + idx, (unsigned long long)d);
 }
 
-// Replays the `nt_all` tight voxels listed at a.fix_list (voxel indices of the lattice in `a`).
+// Scratch of the replay kernel (one set per device state).
+struct ReplayBuf {
+    uint32_t *ctl;      // [0] voxels noted this step, [2] F pool used, [kFixQueue..] the wave kernel's queues
+    uint32_t *T;        // [tcap] noted voxels: by the step kernels (note_tight), then the skewed ones; sorted here
+    uint32_t tcap;      // power of two
+    uint4 *clus;        // [tcap] per cluster: first noted voxel, count, cell, lowest plane
+    uint32_t *F;        // [fcap] the voxels each cluster replays, ascending
+    uint8_t *FL;        // [fcap * S] their slots during the replay
+    uint32_t *FO;       // [fcap] their occupancy words during the replay
+    uint32_t *FW;       // [fcap] their occupancy words when the step began: the sweep's walk (rdme.wgsl:212-213)
+    uint32_t *FS;       // [fcap] species bytes of their count offsets
+    int32_t *FD;        // [fcap * kSkewSp] the count offsets
+    uint32_t fcap;
+};
+
+struct ReplayTally {
+    long long fired = 0, rolled = 0, near = 0, bad = 0, failed = 0, alias = 0;
+};
+
+// Replays cluster c (see the section comment). Returns false when the cluster could not be replayed exactly
+// (scratch or table exhausted, or F reaching further than kEscX planes): its gathered output then stands.
 template <int S>
-__device__ __noinline__ void fixup_tight(const StepArgs &a, uint32_t nt_all)
+__device__ bool replay_cluster(const StepArgs &a, const SkewTab &k, const ReplayBuf &b, const uint32_t *T, uint32_t nt,
+                               uint32_t xlo_t, ReplayTally &tally)
 {
     const Tab tb = global_tab(a);
-    if (nt_all > a.max_tight) {
-        counter_add(a.counters, CS_CNT_EXCEPTION, (long long)nt_all);  // too many to replay
-        return;
-    }
-    uint32_t *T = a.fix_list;
-    const uint32_t nt = nt_all;
-    for (uint32_t i = 1; i < nt; ++i) {  // insertion sort (a handful of entries in practice)
-        const uint32_t x = T[i];
-        uint32_t j = i;
-        while (j > 0u && T[j - 1u] > x) {
-            T[j] = T[j - 1u];
-            --j;
+    const uint32_t base = a.vox_base;
+    const int xlo = int(xlo_t) - kEscX, xhi = int(T[nt - 1u] / a.R12) + kEscX;
+    uint32_t ext[kEscMax];
+    uint32_t n_ext = 0;
+    const uint32_t nF0 = merge_hood(a, T, nt, 8, ext, 0u, nullptr);
+    const uint32_t fb = atomicAdd(b.ctl + 2, nF0 + uint32_t(kEscMax));
+    if (fb + nF0 + uint32_t(kEscMax) > b.fcap)
+        return false;
+    uint32_t *F = b.F + fb;
+    uint8_t *FL = b.FL + size_t(fb) * S;
+    uint32_t *FO = b.FO + fb, *FW = b.FW + fb, *FS = b.FS + fb;
+    int32_t *FD = b.FD + size_t(fb) * kSkewSp;
+    uint32_t nF = 0;
+    long long failed = 0, alias = 0;
+    bool done = false, bad = false;
+    for (int round = 0; round <= kEscMax && !done && !bad; ++round) {
+        nF = merge_hood(a, T, nt, 8, ext, n_ext, F);
+        if (int(F[0] / a.R12) < xlo || int(F[nF - 1u] / a.R12) > xhi)
+            return false;
+        for (uint32_t i = 0; i < nF; ++i) {
+            const Slots<S> s = load_slots<S>(a.src, F[i]);
+            for (int q = 0; q < S; ++q)
+                FL[size_t(i) * S + q] = uint8_t(get_byte(s, q));
+            const int ei = skew_find(k, base + F[i]);
+            const SkewEntry e = ei >= 0 ? k.ent[ei] : SkewEntry{0u, 0, 0u, 0u, {0, 0, 0, 0}};
+            FO[i] = FW[i] = count_nonzero(s) + uint32_t(e.off);
+            FS[i] = e.sp;
+            for (int q = 0; q < kSkewSp; ++q)
+                FD[size_t(i) * kSkewSp + q] = e.dc[q];
         }
-        T[j] = x;
-    }
-    const uint32_t capF = 8u * a.max_tight, capW = 7u * capF;
-    const uint32_t nF = expand_sorted(a, T, nt, 8, a.fix_F, capF);
-    const uint32_t nW = nF > capF ? capW + 1u : expand_sorted(a, a.fix_F, nF, 7, a.fix_W, capW);
-    if (nF > capF || nW > capW) {
-        counter_add(a.counters, CS_CNT_EXCEPTION, (long long)nt);
-        return;
-    }
-    const uint32_t *F = a.fix_F, *W = a.fix_W;
-    uint8_t *Lr = a.fix_L;
-    uint32_t *Or = a.fix_O;
-    long long d_fired = 0, d_rolled = 0, d_near = 0, d_bad = 0;
-    for (uint32_t k = 0; k < nF; ++k) {
-        // undo the counts the gathered result of this voxel contributed
-        VoxelFlags g;
-        (void)gather_voxel<S>(a, tb, F[k], g);
-        d_fired -= g.fired;
-        d_rolled -= g.rolled;
-        d_near -= g.near;
-        d_bad -= g.bad;
-        const Slots<S> s = load_slots<S>(a.src, F[k]);
-        for (int q = 0; q < S; ++q)
-            Lr[size_t(k) * S + q] = uint8_t(get_byte(s, q));
-        Or[k] = count_nonzero(s) + uint32_t(res_offset(a, F[k]));
-    }
-    // canonical replay of the sources in W
-    for (uint32_t wi = 0; wi < nW; ++wi) {
-        const uint32_t w = W[wi];
-        const Slots<S> sw = load_slots<S>(a.src, w);
-        uint32_t occ = count_nonzero(sw);
-        if (const int32_t offw = res_offset(a, w)) {
-            // The sweep walks as many slots as the occupancy word says (rdme.wgsl:213), with the reference's
-            // u32 end index: below the occupied count the last particles stay put; an end at or below the
-            // start walks nothing. Past slot S it would walk the next voxel's particles as this voxel's,
-            // which the byte-slot state cannot follow (flagged).
-            const uint32_t base = w * uint32_t(S), end = base + occ + uint32_t(offw);
-            const uint32_t n = end > base ? end - base : 0u;
-            if (n > uint32_t(S) &&
-                (n > 2u * uint32_t(S) || (w + 1u < a.N && count_nonzero(load_slots<S>(a.src, w + 1u)) != 0u)))
-                d_bad += 1;
-            occ = min(occ, n);
-        }
-        if (!occ)
-            continue;
-        const int wF = find_sorted(F, nF, w);
-        const uint32_t z = w % a.R2, y = (w / a.R2) % a.R1, x = w / a.R12;
-        const Hood h = make_hood(a, x, y, z, w, S);
-        for (uint32_t s = 0; s < occ; ++s) {
-            const uint32_t sp = get_byte(sw, int(s));
-            const uint32_t d = hop(a, tb, h, sp, s);
-            if (d >= 6u)
+        failed = alias = 0;
+        uint32_t esc = 0xFFFFFFFFu;
+        // sources W = F and its six neighbours, ascending (the sweep order of rdme.wgsl:194-229)
+        uint32_t pos[7] = {0, 0, 0, 0, 0, 0, 0};
+        for (;;) {
+            uint32_t w = 0xFFFFFFFFu;
+            for (int q = 0; q < 7; ++q) {
+                while (pos[q] < nF && neighbor_or_none(a, F[pos[q]], q) == 0xFFFFFFFFu)
+                    ++pos[q];
+                if (pos[q] < nF)
+                    w = min(w, neighbor_or_none(a, F[pos[q]], q));
+            }
+            if (w == 0xFFFFFFFFu)
+                break;
+            for (int q = 0; q < 7; ++q)
+                if (pos[q] < nF && neighbor_or_none(a, F[pos[q]], q) == w)
+                    ++pos[q];
+            const int wF = find_sorted(F, nF, w);
+            const Slots<S> sw = load_slots<S>(a.src, w);
+            // walk: occupancy word at the step's start, with the reference's u32 end index (an end at or below
+            // the start walks nothing)
+            const uint32_t occw = wF >= 0 ? FW[wF] : count_nonzero(sw);
+            const uint32_t start = w * uint32_t(S), end = start + occw;
+            uint32_t walk = end > start ? end - start : 0u;
+            if (walk == 0u)
                 continue;
-            const uint32_t t = neighbor_or_none(a, w, int(d));
-            const int tF = find_sorted(F, nF, t);
-            if (wF < 0 && tF < 0)
-                continue;
-            if (tF >= 0) {
-                const uint32_t old = Or[tF];
-                Or[tF] = old + 1u;
-                bool ok = false;
-                if (old <= uint32_t(S)) {
-                    for (int j = 0; j < S && !ok; ++j)
-                        if (Lr[size_t(tF) * S + j] == 0u) {
-                            Lr[size_t(tF) * S + j] = uint8_t(sp);
-                            ok = true;
-                        }
-                    if (!ok) {  // slot S: the next voxel's slot 0 (out of the lattice counts as taken)
-                        const int nF1 = (t + 1u < a.N) ? find_sorted(F, nF, t + 1u) : -1;
-                        if (nF1 >= 0 && Lr[size_t(nF1) * S] == 0u) {
-                            Lr[size_t(nF1) * S] = uint8_t(sp);
-                            ok = true;
-                            d_bad += 1;  // occupancy S+1 with S slots: the byte-slot state cannot hold it
+            if (walk > 2u * uint32_t(S)) {  // a word wrapped below zero at the first voxels: unrepresentable
+                bad = true;
+                walk = 2u * uint32_t(S);
+            }
+            const uint32_t z = w % a.R2, y = (w / a.R2) % a.R1, x = w / a.R12;
+            const Hood h = make_hood(a, x, y, z, w, S);
+            const Slots<S> sn = (walk > uint32_t(S) && w + 1u < a.N) ? load_slots<S>(a.src, w + 1u) : zero_slots<S>();
+            for (uint32_t s = 0; s < walk && esc == 0xFFFFFFFFu; ++s) {
+                const uint32_t blk = w + s / uint32_t(S), j = s % uint32_t(S);
+                const uint32_t sp = blk >= a.N ? 0u : (blk == w ? get_byte_dyn(sw, j) : get_byte_dyn(sn, j));
+                if (sp == 0u)
+                    continue;
+                const uint32_t d = hop(a, tb, h, sp, s);
+                if (d >= 6u)
+                    continue;
+                const uint32_t t = neighbor_or_none(a, w, int(d));
+                const int tF = find_sorted(F, nF, t);
+                const int bF = blk == w ? wF : find_sorted(F, nF, blk);
+                if (tF < 0 && wF < 0)
+                    continue;  // neither end in F: the gather has it
+                if (tF >= 0) {
+                    const uint32_t old = FO[tF];
+                    FO[tF] = old + 1u;
+                    bool ok = false;
+                    if (old <= uint32_t(S)) {  // rdme.wgsl:62: admits a claim into a voxel holding S
+                        for (int q = 0; q < S && !ok; ++q)
+                            if (FL[size_t(tF) * S + q] == 0u) {
+                                FL[size_t(tF) * S + q] = uint8_t(sp);
+                                ok = true;
+                            }
+                        if (!ok && t + 1u < a.N) {  // slot S: slot 0 of the next voxel (past the lattice: taken)
+                            const int t1 = find_sorted(F, nF, t + 1u);
+                            if (t1 < 0) {
+                                esc = t + 1u;
+                                break;
+                            }
+                            if (FL[size_t(t1) * S] == 0u) {
+                                FL[size_t(t1) * S] = uint8_t(sp);
+                                ok = true;
+                                ++alias;
+                                // C[t][sp] counts it, the slots of t+1 hold it
+                                bad |= !dc_add(FS[tF], FD + size_t(tF) * kSkewSp, sp, 1);
+                                bad |= !dc_add(FS[t1], FD + size_t(t1) * kSkewSp, sp, -1);
+                            }
                         }
                     }
+                    if (!ok) {
+                        FO[tF] = old;
+                        ++failed;
+                        if (wF < 0) {  // a source outside F keeps a particle the gather let go
+                            esc = w;
+                            break;
+                        }
+                        continue;
+                    }
                 }
-                if (!ok) {
-                    Or[tF] = old;
+                if (wF < 0)
                     continue;
+                // the source: O[w] -= 1, lattice[i_part] = 0, C[w][sp] -= 1 (rdme.wgsl:69-71)
+                FO[wF] -= 1u;
+                if (bF < 0) {  // a walk past slot S into a block outside F
+                    esc = blk;
+                    break;
+                }
+                const uint32_t cur = FL[size_t(bF) * S + j];
+                FL[size_t(bF) * S + j] = 0u;
+                if (!(bF == wF && cur == sp)) {
+                    bad |= !dc_add(FS[bF], FD + size_t(bF) * kSkewSp, cur, 1);
+                    bad |= !dc_add(FS[wF], FD + size_t(wF) * kSkewSp, sp, -1);
                 }
             }
-            if (wF >= 0) {
-                Or[wF] -= 1u;
-                Lr[size_t(wF) * S + s] = 0u;
+            if (esc != 0xFFFFFFFFu)
+                break;
+        }
+        if (esc == 0xFFFFFFFFu) {
+            done = true;
+        } else {
+            if (n_ext >= uint32_t(kEscMax))
+                return false;
+            uint32_t q = n_ext++;  // keep ext sorted
+            while (q > 0u && ext[q - 1u] > esc) {
+                ext[q] = ext[q - 1u];
+                --q;
             }
+            ext[q] = esc;
         }
     }
-    // reaction and compaction of F, overwriting the gathered output
-    for (uint32_t k = 0; k < nF; ++k) {
-        const uint32_t v = F[k];
-        Slots<S> out = zero_slots<S>();
-        uint32_t cnt = 0;
-        for (int j = 0; j < S; ++j)
-            if (Lr[size_t(k) * S + j]) {
-                put_byte_dyn(out, cnt, Lr[size_t(k) * S + j]);
-                ++cnt;
-            }
-        bool fired, rolled, near, bad;
-        react<S>(a, tb, out, cnt, v / a.R12, (v / a.R2) % a.R1, v % a.R2, fired, rolled, near, bad);
-        d_fired += fired;
-        d_rolled += rolled;
-        d_near += near;
-        d_bad += bad;
-        store_slots<S>(a.dst, v, out);
-        if (count_nonzero(out) != 0u)
+    if (!done)
+        return false;
+    // the counts the step kernel took for F's gathered outputs
+    for (uint32_t i = 0; i < nF; ++i) {
+        VoxelFlags g;
+        (void)gather_voxel<S>(a, tb, F[i], g);
+        tally.fired -= g.fired;
+        tally.rolled -= g.rolled;
+        tally.near -= g.near;
+    }
+    // reaction and compaction of F; outputs, offsets and live bits
+    for (uint32_t i = 0; i < nF; ++i) {
+        const uint32_t v = F[i];
+        int ei = skew_find(k, base + v);
+        const uint32_t kf = ei >= 0 ? k.ent[ei].kf : 0u;
+        XVox<S> vx;
+        vx.sl = zero_slots<S>();
+        for (int q = 0; q < S; ++q)
+            put_byte_dyn(vx.sl, uint32_t(q), FL[size_t(i) * S + q]);
+        vx.occ = FO[i];
+        vx.sp = FS[i];
+        for (int q = 0; q < kSkewSp; ++q)
+            vx.dc[q] = FD[size_t(i) * kSkewSp + q];
+        bool fired, rolled, near, xbad = false;
+        exact_react<S>(a, vx, kf & 0xFFu, v / a.R12, (v / a.R2) % a.R1, v % a.R2, fired, rolled, near, xbad);
+        tally.fired += fired;
+        tally.rolled += rolled;
+        tally.near += near;
+        bad |= xbad;
+        store_slots<S>(a.dst, v, vx.sl);
+        const int32_t off = int32_t(vx.occ - count_nonzero(vx.sl));
+        if (count_nonzero(vx.sl) != 0u)
+            sum_mark(a.sum_dst, v);
+        if (ei < 0 && (off != 0 || vx.sp != 0u))
+            ei = skew_insert(k, base + v);
+        if (ei >= 0) {
+            SkewEntry &e = k.ent[ei];
+            e.off = off;
+            e.sp = vx.sp;
+            for (int q = 0; q < kSkewSp; ++q)
+                e.dc[q] = vx.dc[q];
+            e.kf = kf | kSkewDone;
+            set_voxel_bit(k.live, base + v, skew_live(e));
+        } else if (off != 0 || vx.sp != 0u) {
+            bad = true;  // table full
+        }
+    }
+    tally.failed += failed;
+    tally.alias += alias;
+    tally.bad += bad;
+    return true;
+}
+
+// The reaction of a live entry that no cluster replayed this step: the step kernel stored the voxel's
+// post-diffusion slots and left its reaction here.
+template <int S>
+__device__ void skew_react(const StepArgs &a, const SkewTab &k, uint32_t ei, ReplayTally &tally)
+{
+    SkewEntry e = k.ent[ei];
+    if (e.kf & kSkewDone) {
+        k.ent[ei].kf = e.kf & ~kSkewDone;
+        return;
+    }
+    if (!skew_live(e))
+        return;
+    const uint32_t v = e.v - a.vox_base;
+    XVox<S> vx;
+    vx.sl = load_slots<S>(a.dst, v);
+    vx.occ = count_nonzero(vx.sl) + uint32_t(e.off);
+    vx.sp = e.sp;
+    for (int q = 0; q < kSkewSp; ++q)
+        vx.dc[q] = e.dc[q];
+    bool fired, rolled, near, bad = false;
+    exact_react<S>(a, vx, e.kf & 0xFFu, v / a.R12, (v / a.R2) % a.R1, v % a.R2, fired, rolled, near, bad);
+    tally.fired += fired;
+    tally.rolled += rolled;
+    tally.near += near;
+    tally.bad += bad;
+    if (fired && !rolled) {
+        store_slots<S>(a.dst, v, vx.sl);
+        if (count_nonzero(vx.sl) != 0u)
             sum_mark(a.sum_dst, v);
     }
-    counter_add(a.counters, CS_CNT_FIRED, d_fired);
-    counter_add(a.counters, CS_CNT_ROLLBACK, d_rolled);
-    counter_add(a.counters, CS_CNT_NEAR_THRESHOLD, d_near);
-    counter_add(a.counters, CS_CNT_EXCEPTION, d_bad);
+    e.off = int32_t(vx.occ - count_nonzero(vx.sl));
+    e.sp = vx.sp;
+    for (int q = 0; q < kSkewSp; ++q)
+        e.dc[q] = vx.dc[q];
+    k.ent[ei] = e;
+    if (!skew_live(e))
+        set_voxel_bit(k.live, e.v, false);
+}
+
+// Ascending sort of arr[0..n) in place by bitonic merging, the whole block taking part (arr padded to n2, a
+// power of two, with 0xFFFFFFFF).
+__device__ void block_bitonic(uint32_t *arr, uint32_t n, uint32_t n2)
+{
+    for (uint32_t i = n + threadIdx.x; i < n2; i += blockDim.x)
+        arr[i] = 0xFFFFFFFFu;
+    __syncthreads();
+    for (uint32_t kk = 2; kk <= n2; kk <<= 1)
+        for (uint32_t jj = kk >> 1; jj > 0; jj >>= 1) {
+            for (uint32_t i = threadIdx.x; i < n2; i += blockDim.x) {
+                const uint32_t l = i ^ jj;
+                if (l > i) {
+                    const uint32_t p = arr[i], q = arr[l];
+                    const bool up = (i & kk) == 0u;
+                    if (up ? p > q : p < q) {
+                        arr[i] = q;
+                        arr[l] = p;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+}
+
+// The replay kernel's work for one step (see the section comment). `cells` maps a voxel index to the cell that
+// holds it: find(v) -> cell, args(cell) -> its StepArgs (vox_base, extents, tables, dst).
+template <int S, class Cells>
+__device__ void replay_step(const Cells &cells, const SkewTab &k, const ReplayBuf &b, unsigned long long *counters)
+{
+    __shared__ uint32_t s_n, s_ncl;
+    __shared__ uint32_t s_sort[kSortLds];
+    __shared__ unsigned long long s_tally[6];  // fired, rolled back, near, exceptions, failed claims, aliases
+    const uint32_t tid = threadIdx.x, nth = blockDim.x;
+    const uint32_t ne = k.ctl[0];
+    if (tid == 0)
+        s_n = b.ctl[0];
+    if (tid < 6u)
+        s_tally[tid] = 0ull;
+    __syncthreads();
+    ReplayTally tally;
+    // every voxel whose occupancy word is off its slot count: its walk and its claims differ from the gather
+    for (uint32_t i = tid; i < ne; i += nth)
+        if (k.ent[i].off != 0) {
+            const uint32_t j = atomicAdd(&s_n, 1u);
+            if (j < b.tcap)
+                b.T[j] = k.ent[i].v;
+        }
+    __syncthreads();
+    const uint32_t n = s_n;
+    if (n > b.tcap) {
+        if (tid == 0)
+            tally.bad += n;  // more voxels than the list holds: none replayed
+    } else if (n > 0u) {
+        uint32_t n2 = 1u;
+        while (n2 < n)
+            n2 <<= 1;
+        uint32_t *arr = b.T;
+        if (n2 <= kSortLds) {
+            for (uint32_t i = tid; i < n; i += nth)
+                s_sort[i] = b.T[i];
+            arr = s_sort;
+        }
+        block_bitonic(arr, n, n2);
+        if (tid == 0) {
+            // clusters: runs of one cell whose consecutive noted voxels are fewer than kClusterGap planes apart
+            uint32_t m = 0, ncl = 0, prev = 0xFFFFFFFFu, cell = 0, cbase = 0, cend = 0, last_x = 0;
+            uint32_t R12 = 1;
+            for (uint32_t i = 0; i < n; ++i) {
+                const uint32_t v = arr[i];
+                if (v == prev)
+                    continue;
+                prev = v;
+                bool fresh = ncl == 0u;
+                if (fresh || v >= cend) {
+                    cell = cells.find(v);
+                    const StepArgs ca = cells.args(cell);
+                    cbase = ca.vox_base;
+                    cend = cbase + ca.N;
+                    R12 = ca.R12;
+                    fresh = true;
+                }
+                const uint32_t local = v - cbase, x = local / R12;
+                if (!fresh && x - last_x >= kClusterGap)
+                    fresh = true;
+                if (fresh)
+                    b.clus[ncl++] = make_uint4(m, 0u, cell, x);
+                b.T[m++] = local;
+                b.clus[ncl - 1u].y += 1u;
+                last_x = x;
+            }
+            s_ncl = ncl;
+            b.ctl[2] = 0u;
+        }
+        __syncthreads();
+        for (uint32_t c = tid; c < s_ncl; c += nth) {
+            const uint4 cl = b.clus[c];
+            if (!replay_cluster<S>(cells.args(cl.z), k, b, b.T + cl.x, cl.y, cl.w, tally))
+                tally.bad += cl.y;
+        }
+        __syncthreads();
+    }
+    // the reactions of the live entries no cluster replayed (and the replayed ones re-armed)
+    const uint32_t ne2 = k.ctl[0];
+    for (uint32_t i = tid; i < ne2; i += nth)
+        skew_react<S>(cells.args(cells.find(k.ent[i].v)), k, i, tally);
+    const long long mine[6] = {tally.fired, tally.rolled, tally.near, tally.bad, tally.failed, tally.alias};
+    for (int q = 0; q < 6; ++q)
+        if (mine[q])
+            atomicAdd(&s_tally[q], (unsigned long long)mine[q]);
+    __syncthreads();
+    if (tid == 0) {
+        const int idx[6] = {CS_CNT_FIRED, CS_CNT_ROLLBACK, CS_CNT_NEAR_THRESHOLD, CS_CNT_EXCEPTION, CS_CNT_FAILED_CLAIM,
+                            CS_CNT_ALIAS};
+        for (int q = 0; q < 6; ++q)
+            counter_add(counters, idx[q], (long long)s_tally[q]);
+        b.ctl[0] = 0u;
+    }
 }
 
 __device__ __forceinline__ void note_tight(const StepArgs &a, uint32_t v)
@@ -854,145 +1333,33 @@ __device__ __forceinline__ void note_tight(const StepArgs &a, uint32_t v)
         a.fix_list[i] = a.vox_base + v;
 }
 
-// One block launched after every step kernel on the same stream: replays the tight neighbourhoods noted
-// by the step (if any) and re-arms the count. It returns at once when nothing was tight, which is the
-// normal case; a per-block "last block done" atomic in the step kernel would serialise thousands of
-// same-address device-scope atomics per step instead.
+struct PlainCells {
+    StepArgs a;
+    __device__ uint32_t find(uint32_t) const { return 0u; }
+    __device__ StepArgs args(uint32_t) const { return a; }
+};
+
+// One block launched after every step kernel on the same stream: replays what the step noted (if anything) and
+// the skewed voxels, runs the reactions of the live entries, and re-arms the wave kernel's work queues. With
+// nothing noted and no entries it returns at once, the normal case.
 template <int S>
-__global__ __launch_bounds__(64) void tight_replay_kernel(StepArgs a)
+__global__ __launch_bounds__(kReplayThreads) void tight_replay_kernel(StepArgs a, SkewTab k, ReplayBuf b)
 {
     if (threadIdx.x < 8u)
-        a.fix_ctl[kFixQueue + threadIdx.x] = 0u;  // the wave kernel's work queues, for the next step
-    if (threadIdx.x != 0 || a.fix_ctl[0] == 0u)
+        b.ctl[kFixQueue + threadIdx.x] = 0u;  // the wave kernel's work queues, for the next step
+    if (b.ctl[0] == 0u && k.ctl[0] == 0u)
         return;
-    fixup_tight<S>(a, a.fix_ctl[0]);
-    a.fix_ctl[0] = 0u;
-}
-
-// ---- reservoir voxels: the reaction with the reference's per-voxel counts (cme.wgsl:25-120) ----------
-// C[v][sp] = slots holding sp + the ghost units of ResVoxel (species R and K only).
-template <int S>
-__device__ __forceinline__ int32_t res_count(const Slots<S> &s, uint32_t sp, const ResVoxel &e)
-{
-    if (sp == 0u)
-        return 1;  // C[v][0] = 1 (cme.wgsl:31)
-    int32_t c = int32_t(count_species(s, sp));
-    if (sp == (e.sp & 0xFFu))
-        c += e.gk;
-    if (sp == (e.sp >> 8))
-        c += e.gr;
-    return c;
-}
-
-// k_r * f32(C[i0] * C[i1] * C[i2]) with the i32 product of cme.wgsl:41-45 (wrapping, signed).
-template <int S>
-__device__ __forceinline__ float res_propensity(const StepArgs &a, const Slots<S> &s, const ResVoxel &e, uint32_t r)
-{
-    const uint32_t ix = reinterpret_cast<const uint32_t *>(a.ridx)[r];
-    const uint32_t p = uint32_t(res_count<S>(s, ix & 0xFFu, e)) * uint32_t(res_count<S>(s, (ix >> 8) & 0xFFu, e)) *
-                       uint32_t(res_count<S>(s, (ix >> 16) & 0xFFu, e));
-    return a.rates[r] * float(int32_t(p));
-}
-
-// Launched after tight_replay_kernel when the model has reservoir voxels: one lane per reservoir voxel runs
-// cme.wgsl:25-120 on the compacted post-diffusion slots plus the ghost units (the step kernels skipped
-// these voxels), then keeps C - slots and the occupancy offset for the next step:
-//  * fired, kept: every species but K is rewritten as C copies, so its ghost is 0 (R keeps a negative C,
-//    which writes no slot); K is skipped, so its slots vanish and C[K] stays: ghost K = old C[K];
-//    occupancy += sum of the stoichiometry over the species but K (cme.wgsl:78-84, 114);
-//  * rolled back (cme.wgsl:95-108): slots and occupancy stay; C[q] -= stoich[q] for every q up to the
-//    species being written, K included although it was never added: ghost K -= stoich[K] then.
-template <int S>
-__global__ __launch_bounds__(64) void reservoir_react_kernel(StepArgs a)
-{
-    const uint32_t i = blockIdx.x * 64u + threadIdx.x;
-    bool fired = false, rolled = false, near = false, bad = false;
-    if (i < a.res_n) {
-        const uint32_t v = a.res_v[i];
-        ResVoxel e = a.res_st[i];
-        const uint32_t K = e.sp & 0xFFu, R = e.sp >> 8;
-        const uint32_t z = v % a.R2, y = (v / a.R2) % a.R1, x = v / a.R12;
-        const Slots<S> s = load_slots<S>(a.dst, v);
-        float tot = 0.0f;
-        for (uint32_t r = 1; r <= a.nr; ++r)
-            tot = tot + res_propensity<S>(a, s, e, r);
-        const uint32_t P = pcg_hash(a.itime + x * y + z);
-        const float r1 = uniform_float(P);
-        const float xt = tot * a.tau;
-        const float thr = 1.0f - float(exp(double(-xt)));
-        near = xt != 0.0f && fabsf(r1 - thr) <= 5.9604644775390625e-08f;  // exp(-0) = 1 exactly
-        if (r1 <= thr) {
-            fired = true;
-            // first q with r2 <= cum[q] / tot, q < nr (cme.wgsl:50-52, 68-73)
-            const float r2 = uniform_float(P + 1u);
-            uint32_t q = 0;
-            float cum = 0.0f;
-            while (q < a.nr && r2 > cum / tot) {
-                ++q;
-                cum = cum + res_propensity<S>(a, s, e, q);
-            }
-            const int8_t *row = a.stoich + size_t(q) * (a.ns + 1u);
-            Slots<S> out = zero_slots<S>();
-            uint32_t j = 0, sp_stop = 0;
-            int32_t docc = 0, gr = 0;
-            bool neg = false;
-            for (uint32_t sp = 1; sp <= a.ns; ++sp) {
-                if (sp == K)
-                    continue;  // reservoirs[v*S] == sp (cme.wgsl:80-83)
-                const int32_t c = res_count<S>(s, sp, e) + int32_t(row[sp]);
-                docc += int32_t(row[sp]);
-                if (c > 0) {
-                    if (j + uint32_t(c) >= uint32_t(S)) {
-                        sp_stop = sp;
-                        break;
-                    }
-                    for (int32_t t = 0; t < c; ++t)
-                        put_byte_dyn(out, j++, sp);
-                } else if (c < 0) {
-                    if (sp == R)
-                        gr = c;
-                    else
-                        neg = true;  // a negative count of a third species has no place to live
-                }
-            }
-            if (sp_stop != 0u) {
-                rolled = true;
-                if (K != 0u && K <= sp_stop)
-                    e.gk -= int32_t(row[K]);
-            } else {
-                bad = neg;
-                const int32_t occ = int32_t(count_nonzero(s)) + e.off;
-                if (K != 0u)
-                    e.gk = res_count<S>(s, K, e);
-                if (R != 0u)
-                    e.gr = gr;
-                e.off = occ + docc - int32_t(j);
-                store_slots<S>(a.dst, v, out);
-                if (j != 0u)
-                    sum_mark(a.sum_dst, v);
-            }
-            a.res_st[i] = e;
-            const uint32_t bit = 1u << (v & 31u);
-            if (e.off != 0)
-                atomicOr(a.res_skew + (v >> 5), bit);
-            else
-                atomicAnd(a.res_skew + (v >> 5), ~bit);
-        }
-    }
-    count_event(a.counters, CS_CNT_EXCEPTION, bad);
-    count_event(a.counters, CS_CNT_NEAR_THRESHOLD, near);
-    count_event(a.counters, CS_CNT_FIRED, fired);
-    count_event(a.counters, CS_CNT_ROLLBACK, rolled);
+    replay_step<S>(PlainCells{a}, k, b, a.counters);
 }
 
 // ---- view (final_stage.wgsl:35-67): one value per voxel for a renderer -----------------------------
 // Voxel order (v = z + y*R2 + x*R2*R1). Hidden (0) when res[axis] - coord[axis] < slice. View 0: the species
 // found most often among slots 0..occupancy-1 in lattice-index order (slot S and on are the next voxel's,
 // past the lattice 0; ties keep the first in slot order), over 255; view 1: region label / 255; view 2:
-// reservoir species / 255. The occupancy word is the slot count plus the reservoir offset; a word above 2S
+// reservoir species / 255. The occupancy word is the slot count plus the skewed voxel's offset; a word above 2S
 // (only one wrapped below zero, whose loop in the reference walks the whole buffer) gives 0.
 template <int S>
-__global__ __launch_bounds__(kBlock) void view_kernel(StepArgs a, const uint8_t *resv, uint32_t view,
+__global__ __launch_bounds__(kBlock) void view_kernel(StepArgs a, SkewTab k, const uint8_t *resv, uint32_t view,
                                                       uint32_t axis, uint32_t slice, float *out)
 {
     const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
@@ -1009,7 +1376,12 @@ __global__ __launch_bounds__(kBlock) void view_kernel(StepArgs a, const uint8_t 
             val = float(resv != nullptr ? resv[v] : 0u) / 255.0f;
         } else {
             const Slots<S> s = load_slots<S>(a.src, v);
-            const uint32_t occ = count_nonzero(s) + uint32_t(res_offset(a, v));
+            int32_t off = 0;
+            if (voxel_bit(a.live, a.vox_base + v)) {
+                const int ei = skew_find(k, a.vox_base + v);
+                off = ei >= 0 ? k.ent[ei].off : 0;
+            }
+            const uint32_t occ = count_nonzero(s) + uint32_t(off);
             uint32_t best = 0u, maxc = 0u;
             if (occ <= 2u * uint32_t(S)) {
                 const Slots<S> nx = (occ > uint32_t(S) && v + 1u < a.N) ? load_slots<S>(a.src, v + 1u) : zero_slots<S>();
@@ -1039,11 +1411,10 @@ __global__ __launch_bounds__(kBlock) void rdme_step_kernel(StepArgs a)
     if (v < a.N) {
         const Slots<S> out = gather_voxel<S>(a, global_tab(a), v, f);
         store_slots<S>(a.dst, v, out);
-        if (f.tight)
+        if (f.tight || f.bad)  // a tight voxel, or a reaction that took a count below zero: replayed
             note_tight(a, v);
     }
     count_event(a.counters, CS_CNT_TIGHT, f.tight);
-    count_event(a.counters, CS_CNT_EXCEPTION, f.bad);
     count_event(a.counters, CS_CNT_NEAR_THRESHOLD, f.near);
     count_event(a.counters, CS_CNT_FIRED, f.fired);
     count_event(a.counters, CS_CNT_ROLLBACK, f.rolled);
@@ -1283,7 +1654,7 @@ __device__ __forceinline__ uint32_t take_arrivals(const typename SlotVec<S>::T &
 
 // Event counts a block accumulates over the tiles it marches.
 struct TileCounts {
-    uint32_t tight = 0, bad = 0, near = 0, fired = 0, rolled = 0;
+    uint32_t tight = 0, near = 0, fired = 0, rolled = 0;
 };
 
 // One tile (tz, ty) of the lattice described by (a, t), marched through x-chunk `chunk`. Every lane of the
@@ -1382,7 +1753,7 @@ __device__ __forceinline__ void tile_march(TileLds<S> &L, uint32_t *tab_lds, con
     st0.hal = load_slots<S>(a.src, plane_base(xb - 1) + hal_yz);
     fetch_labels(xb, st0.lab);
 
-    uint32_t n_tight = 0, n_bad = 0, n_near = 0, n_fired = 0, n_rolled = 0;
+    uint32_t n_tight = 0, n_near = 0, n_fired = 0, n_rolled = 0;
     __syncthreads();  // the tables and the first labels are in LDS
 
     auto plane = [&](int p, Staged &use, Staged &fill) __attribute__((always_inline)) {
@@ -1459,7 +1830,8 @@ __device__ __forceinline__ void tile_march(TileLds<S> &L, uint32_t *tab_lds, con
         if (gather_here && !slow) {
             // empty and nothing arrives: the output is empty; only the zero-propensity draw at seed 0 "fires"
             store_slots<S>(a.dst, mine_yz + uint32_t(x) * a.R12, zero_slots<S>());
-            n_fired += (a.itime + uint32_t(x) * uint32_t(Y) + uint32_t(Z)) == 0u;
+            n_fired += (a.itime + uint32_t(x) * uint32_t(Y) + uint32_t(Z)) == 0u &&
+                       !voxel_bit(a.live, a.vox_base + mine_yz + uint32_t(x) * a.R12);
         }
         const uint32_t n_slow = wave_list(slow, uint32_t(lane), L.slow_list[wv]);
         if (wl == 0) {
@@ -1540,15 +1912,8 @@ __device__ __forceinline__ void tile_march(TileLds<S> &L, uint32_t *tab_lds, con
                 nhi = take_arrivals<S>(sy1, sel(4u, cy1, 2u), hi, nhi);   // from +y, hopped -y
                 nhi = take_arrivals<S>(sx1, sel(1u, cx1, 0u), hi, nhi);   // from +x, hopped -x
                 const uint32_t V = gz + gy * a.R2 + uint32_t(x) * a.R12;
-                if (occ + nlo + nhi > uint32_t(S) ||
-                    (a.res_skew != nullptr && occ + nlo + nhi != 0u && voxel_bit(a.res_skew, V))) {
-                    ++n_tight;
-                    const uint32_t k = atomicAdd(&L.n_tight, 1u);
-                    if (k < uint32_t(kTightLds))
-                        L.tight[k] = V;
-                    else
-                        note_tight(a, V);  // block list full: straight to the global list
-                }
+                const bool tight = occ + nlo + nhi > uint32_t(S);
+                n_tight += tight;
                 uint32_t cnt = 0;
                 Slots<S> out = assemble<S>(own, occ, occ ? stay : 0ull, lo, nlo, hi, nhi, cnt);
                 bool fired, rolled, near, bad;
@@ -1556,7 +1921,14 @@ __device__ __forceinline__ void tile_march(TileLds<S> &L, uint32_t *tab_lds, con
                 n_fired += fired;
                 n_rolled += rolled;
                 n_near += near;
-                n_bad += bad;
+                // a tight voxel, or a reaction that took a count below zero: replayed after the step
+                if (tight || bad) {
+                    const uint32_t k = atomicAdd(&L.n_tight, 1u);
+                    if (k < uint32_t(kTightLds))
+                        L.tight[k] = V;
+                    else
+                        note_tight(a, V);  // block list full: straight to the global list
+                }
                 store_slots<S>(a.dst, V, out);
             }
         }
@@ -1573,7 +1945,6 @@ __device__ __forceinline__ void tile_march(TileLds<S> &L, uint32_t *tab_lds, con
     for (uint32_t i = uint32_t(lane); i < nt; i += kBlock)
         note_tight(a, L.tight[i]);
     tc.tight += n_tight;
-    tc.bad += n_bad;
     tc.near += n_near;
     tc.fired += n_fired;
     tc.rolled += n_rolled;
@@ -1582,10 +1953,10 @@ __device__ __forceinline__ void tile_march(TileLds<S> &L, uint32_t *tab_lds, con
 
 __device__ __forceinline__ void flush_counts(unsigned long long *counters, const TileCounts &tc)
 {
-    const uint32_t vals[5] = {tc.tight, tc.bad, tc.near, tc.fired, tc.rolled};
-    const int idx[5] = {CS_CNT_TIGHT, CS_CNT_EXCEPTION, CS_CNT_NEAR_THRESHOLD, CS_CNT_FIRED, CS_CNT_ROLLBACK};
+    const uint32_t vals[4] = {tc.tight, tc.near, tc.fired, tc.rolled};
+    const int idx[4] = {CS_CNT_TIGHT, CS_CNT_NEAR_THRESHOLD, CS_CNT_FIRED, CS_CNT_ROLLBACK};
 #pragma unroll
-    for (int k = 0; k < 5; ++k)  // events are rare: one atomic per lane that saw any
+    for (int k = 0; k < 4; ++k)  // events are rare: one atomic per lane that saw any
         if (vals[k])
             atomicAdd(counters + idx[k], (unsigned long long)vals[k]);
 }
@@ -1917,7 +2288,9 @@ __device__ __forceinline__ void count_idle_fires(const StepArgs &a, uint32_t tz,
         for (int qb = xb; qb < xe; qb += 64) {
             const int Q = qb + l;
             const uint32_t zfire = 0u - a.itime - uint32_t(Q) * uint32_t(y0 + r) - uint32_t(z0);
-            const bool f = Q < xe && zfire < uint32_t(kWZ) && ((im >> (zfire & 63u)) & 1ull) != 0ull;
+            bool f = Q < xe && zfire < uint32_t(kWZ) && ((im >> (zfire & 63u)) & 1ull) != 0ull;
+            if (f)  // a skewed voxel's reaction runs in the replay kernel
+                f = !voxel_bit(a.live, a.vox_base + z0 + zfire + uint32_t(y0 + r) * a.R2 + uint32_t(Q) * a.R12);
             nf += uint32_t(__popcll(__ballot(f)));
         }
     }
@@ -2044,7 +2417,7 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
     for (int q = 0; q < 3 * kWY / 2; ++q)
         L.head[q / (kWY / 2)][(q % (kWY / 2)) * 64 + l] = kWEvEnd | (kWEvEnd << 16);
 
-    uint32_t n_tight = 0, n_bad = 0, n_near = 0, n_fired = 0, n_rolled = 0;
+    uint32_t n_tight = 0, n_near = 0, n_fired = 0, n_rolled = 0;
     // bit r of this lane: its voxel of interior row r holds particles, in the plane gathered next and in the
     // plane listed now (per-lane bits rather than wave masks: these live through the march, and scalar
     // registers are the scarcer ones here)
@@ -2070,7 +2443,8 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
             n_fired += fired;
             n_rolled += rolled;
             n_near += near;
-            n_bad += bad;
+            if (bad)  // a count taken below zero: the replay kernel redoes this voxel
+                note_tight(a, V);
             store_slots<S>(a.dst, V, out);
         }
         rb_head += k;
@@ -2242,7 +2616,8 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
                 const uint64_t m = inm & (__ballot((occ_prev >> r) & 1u) | L.arrive[rq][r] | every);
                 // an empty voxel's zero-propensity draw fires at seed 0 (itime + x*y + z == 0)
                 const uint32_t zfire = 0u - a.itime - uint32_t(Q) * uint32_t(y0 + r) - uint32_t(z0);
-                if (l == 0 && zfire < uint32_t(kWZ))
+                if (l == 0 && zfire < uint32_t(kWZ) &&
+                    !voxel_bit(a.live, a.vox_base + z0 + zfire + uint32_t(y0 + r) * a.R2 + uint32_t(Q) * a.R12))
                     n_fired += uint32_t((inm & ~m) >> zfire) & 1u;
                 const bool need = (m >> l) & 1ull;
                 const bool occ = (occ_prev >> r) & 1u;
@@ -2279,7 +2654,7 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
                 const uint32_t lv = act ? lvq[me] : 0u;
                 // Most voxels end with at most four particles: then the arrivals and the assembled slots fit
                 // one 32-bit word, and the wave takes that path when every lane fits it.
-                uint32_t cnt = 0u, narr = 0u;
+                uint32_t cnt = 0u;
                 Slots<S> out = zero_slots<S>();
                 bool done = false, tight = false;
                 if constexpr (S > 4) {
@@ -2310,7 +2685,6 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
                             const uint32_t stay4 = ((1u << (4u * occ)) - 1u) & 0x1111u & ~sp4;
                             out.w[0] = assemble4(own.w[0], occ, stay4, arr & b4_below(nlo4),
                                                  nlo4, b4_shr(arr, nlo4), tot - nlo4, cnt);
-                            narr = tot;
                             done = true;
                         }
                     }
@@ -2353,14 +2727,9 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
                         ++n_tight;
                         note_tight(a, V);
                     }
-                    narr = nlo + nhi;
                     const uint64_t stay = first_nibbles(occ) & ~spread_nibbles(lv);
                     out = assemble<S>(own, occ, occ ? stay : 0ull, lo, min(nlo, 16u), hi, min(nhi, 16u), cnt);
                 }
-                // a reservoir voxel whose occupancy word is off the slot count can refuse a claim that fits
-                // (rdme.wgsl:62) and walks that many slots (rdme.wgsl:213): replayed like a tight one
-                if (a.res_skew != nullptr && act && !tight && occ + narr != 0u && voxel_bit(a.res_skew, V))
-                    note_tight(a, V);
                 const bool may = act && react_may_fire<S>(a, tb, cnt, uint32_t(Q), Y, Z);
                 // the result goes back to the list entry; the voxel's own lane stores whole rows below, so no
                 // output line is written in pieces at different times (a reaction still to run is stored
@@ -2432,7 +2801,6 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
     while (rb_n != 0u)
         react_flush(min(rb_n, 64u));
     tc.tight += n_tight;
-    tc.bad += n_bad;
     tc.near += n_near;
     tc.fired += n_fired;
     tc.rolled += n_rolled;
@@ -2693,9 +3061,9 @@ struct EnsArgs {
     uint32_t *itime;           // [n_cells], written by the prologue of each step
     uint32_t *queue;           // [0] next item, [1] step counter
     unsigned long long *counters;
-    uint32_t *fix_ctl, *fix_list, *fix_F, *fix_W, *fix_O;
-    uint8_t *fix_L;
+    uint32_t *fix_ctl, *fix_list;
     uint32_t max_tight;
+    const uint32_t *live;         // skewed voxels (bit per voxel of every cell)
     uint8_t *sum_src, *sum_dst;   // occupancy summaries of src and dst (every cell's, at EnsCell::sum_off)
     const uint32_t *item_cell;    // [total_items] the cell of each work item
 };
@@ -2729,10 +3097,6 @@ __device__ __forceinline__ StepArgs cell_step_args(const EnsArgs &e, const EnsCe
     a.counters = e.counters;
     a.fix_ctl = e.fix_ctl;
     a.fix_list = e.fix_list;
-    a.fix_F = e.fix_F;
-    a.fix_W = e.fix_W;
-    a.fix_L = e.fix_L;
-    a.fix_O = e.fix_O;
     a.max_tight = e.max_tight;
     a.vox_base = c.vox_off;
     a.R0 = c.R0;
@@ -2747,11 +3111,7 @@ __device__ __forceinline__ StepArgs cell_step_args(const EnsArgs &e, const EnsCe
     a.itime = itime;
     a.mod_zero = c.mod_zero;
     a.tau = c.tau;
-    a.res_bits = nullptr;  // ensembles take no reservoir models
-    a.res_skew = nullptr;
-    a.res_v = nullptr;
-    a.res_st = nullptr;
-    a.res_n = 0u;
+    a.live = e.live;
     a.sum_dst = cell_sum(e.sum_dst, c);
     return a;
 }
@@ -2867,52 +3227,35 @@ __global__ __launch_bounds__(kWaveBlock) void rdme_step_ensemble(EnsArgs e)
     flush_counts(e.counters, tc);
 }
 
-// After each step: replays the tight voxels cell by cell, re-arms the queue, advances the step counter.
-template <int S>
-__global__ __launch_bounds__(64) void ens_epilogue(EnsArgs e)
-{
-    if (threadIdx.x != 0)
-        return;
-    const uint32_t nt = e.fix_ctl[0];
-    if (nt > e.max_tight) {
-        counter_add(e.counters, CS_CNT_EXCEPTION, (long long)nt);
-    } else if (nt > 0u) {
-        uint32_t *T = e.fix_list;
-        for (uint32_t i = 1; i < nt; ++i) {  // insertion sort: cells' voxel ranges become contiguous
-            const uint32_t x = T[i];
-            uint32_t j = i;
-            while (j > 0u && T[j - 1u] > x) {
-                T[j] = T[j - 1u];
-                --j;
-            }
-            T[j] = x;
+// The cells of an ensemble as the replay sees them: a voxel index among all cells' voxels -> its cell.
+struct EnsCells {
+    EnsArgs e;
+    __device__ uint32_t find(uint32_t v) const
+    {
+        uint32_t lo = 0, hi = e.n_cells;
+        while (hi - lo > 1u) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (e.cells[mid].vox_off <= v)
+                lo = mid;
+            else
+                hi = mid;
         }
-        uint32_t i = 0;
-        while (i < nt) {
-            uint32_t lo = 0, hi = e.n_cells;
-            while (hi - lo > 1u) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (e.cells[mid].vox_off <= T[i])
-                    lo = mid;
-                else
-                    hi = mid;
-            }
-            const EnsCell &c = e.cells[lo];
-            const uint32_t end = c.vox_off + c.R0 * c.R1 * c.R2;
-            uint32_t j = i;
-            while (j < nt && T[j] < end) {
-                T[j] -= c.vox_off;
-                ++j;
-            }
-            StepArgs a = cell_step_args(e, c, e.itime[lo]);
-            a.fix_list = T + i;
-            fixup_tight<S>(a, j - i);
-            i = j;
-        }
+        return lo;
     }
-    e.fix_ctl[0] = 0u;
-    e.queue[0] = 0u;
-    e.queue[1] += 1u;
+    __device__ StepArgs args(uint32_t c) const { return cell_step_args(e, e.cells[c], e.itime[c]); }
+};
+
+// After each step: the replay of every cell's noted and skewed voxels (tight_replay_kernel's work, clusters
+// never spanning two cells), then the queue re-armed and the step counter advanced.
+template <int S>
+__global__ __launch_bounds__(kReplayThreads) void ens_epilogue(EnsArgs e, SkewTab k, ReplayBuf b)
+{
+    if (b.ctl[0] != 0u || k.ctl[0] != 0u)
+        replay_step<S>(EnsCells{e}, k, b, e.counters);
+    if (threadIdx.x == 0) {
+        e.queue[0] = 0u;
+        e.queue[1] += 1u;
+    }
 }
 }  // namespace
 
@@ -3006,8 +3349,6 @@ static bool summary_pays(const std::vector<uint8_t> &bytes)
 // Checks that the model fits the device encodings (one-byte labels and slots, int8 stoichiometry).
 static void check_device_model(const Model &m)
 {
-    if (m.has_reservoir() && m.num_cme_species() > 0u)
-        throw Error(CS_ERR_UNSUPPORTED, "reservoir particles together with well-mixed species are not supported");
     if (m.num_regions() > 255)
         throw Error(CS_ERR_UNSUPPORTED, "at most 255 regions fit the one-byte label");
     if (m.num_reactions() > 255)
@@ -3146,6 +3487,188 @@ static CmeArgs cme_args_at(const uint8_t *base, const CmeBlob &b, const Model &m
     return c;
 }
 
+// Device table of skewed voxels (SkewTab) and the replay kernel's scratch (ReplayBuf) of one device state: a
+// plain simulation, or every cell of an ensemble (voxel indices among all cells' voxels). The table starts with
+// the reservoir voxels, rebuilt from the host copy whenever the lattice is replaced.
+struct SkewStore {
+    SkewTab k{};
+    ReplayBuf b{};
+    uint64_t nvox = 0;
+    std::vector<uint32_t> keys0, idx0;  // the table as it starts: reservoir voxels only
+    std::vector<SkewEntry> ent0;
+    std::vector<uint32_t> live0;        // their bits
+
+    static uint32_t pow2_at_least(uint64_t n)
+    {
+        uint64_t p = 1;
+        while (p < n)
+            p <<= 1;
+        return uint32_t(p);
+    }
+
+    // Host-side insert of a reservoir voxel's entry, in the probing order of skew_insert.
+    void add_static(uint32_t v, uint32_t R, uint32_t K)
+    {
+        uint32_t h = skew_hash(v) & k.mask;
+        while (keys0[h] != kSkewFree && keys0[h] != v)
+            h = (h + 1u) & k.mask;
+        if (keys0[h] == kSkewFree) {
+            keys0[h] = v;
+            idx0[h] = uint32_t(ent0.size());
+            SkewEntry e{v, 0, 0u, kSkewStatic | K, {0, 0, 0, 0}};
+            ent0.push_back(e);
+        }
+        SkewEntry &e = ent0[idx0[h]];
+        e.kf = kSkewStatic | K;
+        if (R != 0u) {  // the reservoir unit: C[v][R] = 1 with no slot (lattice.rs:297)
+            int32_t dc[kSkewSp];
+            std::memcpy(dc, e.dc, sizeof(dc));
+            uint32_t sp = e.sp;
+            bool placed = false;
+            for (int q = 0; q < kSkewSp && !placed; ++q)
+                if (((sp >> (8 * q)) & 0xFFu) == R || ((sp >> (8 * q)) & 0xFFu) == 0u) {
+                    sp |= R << (8 * q);
+                    dc[q] += 1;
+                    placed = true;
+                }
+            e.sp = sp;
+            std::memcpy(e.dc, dc, sizeof(dc));
+        }
+        live0[v >> 5] |= 1u << (v & 31u);
+    }
+
+    // Sizes the table for every voxel of a state of up to 2^20 voxels (a voxel keeps its entry once it has been
+    // skewed), 2^20 entries above that (at least twice the reservoir voxels), and the replay scratch: up to tcap
+    // noted voxels per step (every voxel twice, at most 65536), and
+    // F sets for `max_clusters` clusters (a cluster's F is disjoint from every other's, so all of them hold at
+    // most every voxel once, plus kEscMax voxels each for the voxels they may add).
+    void alloc(uint64_t nvox_, uint32_t S, uint64_t n_static, uint32_t *fix_ctl, uint64_t max_clusters)
+    {
+        nvox = nvox_;
+        uint64_t want = std::max<uint64_t>(2 * n_static + 8192, std::min<uint64_t>(nvox, uint64_t(1) << 20));
+        want = std::min<uint64_t>(want, nvox + 64) + kSkewMargin;
+        k.ecap = uint32_t(want);
+        const uint32_t cap = pow2_at_least(2 * want);
+        k.mask = cap - 1u;
+        keys0.assign(cap, kSkewFree);
+        idx0.assign(cap, 0u);
+        ent0.clear();
+        live0.assign((nvox + 31u) / 32u, 0u);
+        hip_check(hipMalloc(&k.keys, size_t(cap) * sizeof(uint32_t)), "hipMalloc skew keys");
+        hip_check(hipMalloc(&k.idx, size_t(cap) * sizeof(uint32_t)), "hipMalloc skew index");
+        hip_check(hipMalloc(&k.ent, size_t(k.ecap) * sizeof(SkewEntry)), "hipMalloc skew entries");
+        hip_check(hipMalloc(&k.ctl, 4 * sizeof(uint32_t)), "hipMalloc skew control");
+        hip_check(hipMalloc(&k.live, std::max<size_t>(live0.size(), 1) * sizeof(uint32_t)), "hipMalloc skew bits");
+        b.ctl = fix_ctl;
+        b.tcap = pow2_at_least(std::max<uint64_t>(std::min<uint64_t>(2 * nvox + 64, 65536), 1024));
+        b.fcap = uint32_t(std::min<uint64_t>(12ull * b.tcap, nvox) + (kEscMax + 8) * (max_clusters + 1));
+        hip_check(hipMalloc(&b.T, size_t(b.tcap) * sizeof(uint32_t)), "hipMalloc replay list");
+        hip_check(hipMalloc(&b.clus, size_t(max_clusters + 1) * sizeof(uint4)), "hipMalloc replay clusters");
+        hip_check(hipMalloc(&b.F, size_t(b.fcap) * sizeof(uint32_t)), "hipMalloc replay F");
+        hip_check(hipMalloc(&b.FL, size_t(b.fcap) * S), "hipMalloc replay slots");
+        hip_check(hipMalloc(&b.FO, size_t(b.fcap) * sizeof(uint32_t)), "hipMalloc replay occupancy");
+        hip_check(hipMalloc(&b.FW, size_t(b.fcap) * sizeof(uint32_t)), "hipMalloc replay walks");
+        hip_check(hipMalloc(&b.FS, size_t(b.fcap) * sizeof(uint32_t)), "hipMalloc replay offsets");
+        hip_check(hipMalloc(&b.FD, size_t(b.fcap) * kSkewSp * sizeof(int32_t)), "hipMalloc replay offsets");
+    }
+
+    // Resets the device table to its starting entries.
+    void upload(hipStream_t st)
+    {
+        const uint32_t ctl[4] = {uint32_t(ent0.size()), 0u, 0u, 0u};
+        hip_check(hipMemcpyAsync(k.keys, keys0.data(), keys0.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st),
+                  "upload skew keys");
+        hip_check(hipMemcpyAsync(k.idx, idx0.data(), idx0.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st),
+                  "upload skew index");
+        if (!ent0.empty())
+            hip_check(hipMemcpyAsync(k.ent, ent0.data(), ent0.size() * sizeof(SkewEntry), hipMemcpyHostToDevice, st),
+                      "upload skew entries");
+        hip_check(hipMemcpyAsync(k.ctl, ctl, sizeof(ctl), hipMemcpyHostToDevice, st), "upload skew control");
+        if (!live0.empty())
+            hip_check(hipMemcpyAsync(k.live, live0.data(), live0.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st),
+                      "upload skew bits");
+        hip_check(hipStreamSynchronize(st), "upload skew table");  // ctl lives on this stack frame
+    }
+
+    // The live entries (voxel indices among all voxels of the state), in insertion order.
+    void read(hipStream_t st, std::vector<SkewVoxel> &out) const
+    {
+        uint32_t ctl[4];
+        hip_check(hipMemcpyAsync(ctl, k.ctl, sizeof(ctl), hipMemcpyDeviceToHost, st), "read skew control");
+        hip_check(hipStreamSynchronize(st), "read skew control");
+        std::vector<SkewEntry> e(ctl[0]);
+        if (!e.empty())
+            hip_check(hipMemcpyAsync(e.data(), k.ent, e.size() * sizeof(SkewEntry), hipMemcpyDeviceToHost, st),
+                      "read skew entries");
+        hip_check(hipStreamSynchronize(st), "read skew entries");
+        out.clear();
+        for (const SkewEntry &x : e) {
+            SkewVoxel s;
+            s.v = x.v;
+            s.off = x.off;
+            s.K = x.kf & 0xFFu;
+            for (int q = 0; q < kSkewSp; ++q) {
+                s.sp[q] = (x.sp >> (8 * q)) & 0xFFu;
+                s.dc[q] = s.sp[q] ? x.dc[q] : 0;
+            }
+            if (x.off != 0 || x.sp != 0u || s.K != 0u || (x.kf & kSkewStatic))
+                out.push_back(s);
+        }
+    }
+
+    uint32_t refused(hipStream_t st) const
+    {
+        uint32_t ctl[4];
+        hip_check(hipMemcpyAsync(ctl, k.ctl, sizeof(ctl), hipMemcpyDeviceToHost, st), "read skew control");
+        hip_check(hipStreamSynchronize(st), "read skew control");
+        return ctl[1];
+    }
+
+    size_t table_bytes() const { return size_t(k.mask + 1u) * 2u * sizeof(uint32_t); }
+
+    void release()
+    {
+        void *bufs[] = {k.keys, k.idx, k.ent, k.ctl, k.live, b.T, b.clus, b.F, b.FL, b.FO, b.FW, b.FS, b.FD};
+        for (void *p : bufs)
+            if (p)
+                (void)hipFree(p);
+        k = SkewTab{};
+        b = ReplayBuf{};
+    }
+};
+
+// The reservoir voxels of a model, as static entries with voxel indices offset by `base`: sites of a reservoir
+// species R (one unit of C[v][R] in no slot) and voxels v whose reservoirs[v*S] names a species K (cme.wgsl:80
+// indexes the reservoir buffer with the lattice index; an index past the end reads 0).
+static void add_reservoir_entries(SkewStore &sk, const Model &m, uint32_t base)
+{
+    if (!m.has_reservoir())
+        return;
+    const std::vector<uint32_t> &rs = m.reservoir();
+    const uint64_t N = m.n_voxels(), S = m.slots();
+    for (uint64_t v = 0; v < N; ++v) {
+        const uint32_t R = rs[v];
+        const uint32_t K = v * S < N ? rs[v * S] : 0u;
+        if (R == 0u && K == 0u)
+            continue;
+        if (R > 255u || K > 255u)
+            throw Error(CS_ERR_UNSUPPORTED, "reservoir species index beyond one byte");
+        sk.add_static(base + uint32_t(v), R, K);
+    }
+}
+
+static uint64_t count_reservoir_voxels(const Model &m)
+{
+    if (!m.has_reservoir())
+        return 0;
+    const std::vector<uint32_t> &rs = m.reservoir();
+    const uint64_t N = m.n_voxels(), S = m.slots();
+    uint64_t n = 0;
+    for (uint64_t v = 0; v < N; ++v)
+        n += rs[v] != 0u || (v * S < N && rs[v * S] != 0u);
+    return n;
+}
+
 Device::Device(const Model &m, int hip_device) : dev_(hip_device)
 {
     const LatticeParams &lp = m.params();
@@ -3261,12 +3784,12 @@ Device::Device(const Model &m, int hip_device) : dev_(hip_device)
     hip_check(hipMalloc(&counters_, kNumCounters * sizeof(unsigned long long)), "hipMalloc counters");
     hip_check(hipMalloc(&hist_, size_t(nreg_) * n1 * sizeof(int32_t)), "hipMalloc hist");
     hip_check(hipMalloc(&fix_ctl_, kFixCtl * sizeof(uint32_t)), "hipMalloc fix_ctl");
-    hip_check(hipMalloc(&fix_list_, size_t(kMaxTight) * sizeof(uint32_t)), "hipMalloc fix_list");
-    hip_check(hipMalloc(&fix_F_, size_t(8 * kMaxTight) * sizeof(uint32_t)), "hipMalloc fix_F");
-    hip_check(hipMalloc(&fix_W_, size_t(56 * kMaxTight) * sizeof(uint32_t)), "hipMalloc fix_W");
-    hip_check(hipMalloc(&fix_L_, size_t(8 * kMaxTight) * S_), "hipMalloc fix_L");
-    hip_check(hipMalloc(&fix_O_, size_t(8 * kMaxTight) * sizeof(uint32_t)), "hipMalloc fix_O");
     hip_check(hipMemsetAsync(fix_ctl_, 0, kFixCtl * sizeof(uint32_t), st), "clear fix_ctl");
+    // skewed voxels: the reservoir voxels to start with (their table and bits uploaded below)
+    skew_ = new SkewStore();
+    skew_->alloc(N_, S_, count_reservoir_voxels(m), fix_ctl_, res_[0] / kClusterGap + 1u);
+    add_reservoir_entries(*skew_, m, 0u);
+    reg_host_ = reg8;
 
     // Well-mixed (CME) pool and its tables in one blob (8-byte aligned sections), plus the voxel lists of
     // the regions that receive lattice products.
@@ -3296,46 +3819,6 @@ Device::Device(const Model &m, int hip_device) : dev_(hip_device)
                   "upload cme tables");
     }
 
-    // Reservoir voxels: sites of a reservoir species, and voxels v whose reservoirs[v*S] names one
-    // (cme.wgsl:80 indexes the reservoir buffer with the lattice index).
-    std::vector<ResVoxel> rst;
-    std::vector<uint32_t> rbits;
-    if (m.has_reservoir()) {
-        const std::vector<uint32_t> &rs = m.reservoir();
-        rbits.assign((N_ + 31u) / 32u, 0u);
-        for (uint64_t v = 0; v < N_; ++v) {
-            const uint32_t R = rs[v];
-            const uint64_t vk = v * S_;
-            const uint32_t K = vk < N_ ? rs[vk] : 0u;
-            if (R == 0u && K == 0u)
-                continue;
-            if (R > 255u || K > 255u)
-                throw Error(CS_ERR_UNSUPPORTED, "reservoir species index beyond one byte");
-            ResVoxel e;
-            e.sp = K | ((R == K ? 0u : R) << 8);
-            e.gr = (R != 0u && R != K) ? 1 : 0;  // the reservoir unit (lattice.rs:297)
-            e.gk = (R != 0u && R == K) ? 1 : 0;
-            e.off = 0;
-            res_vh_.push_back(uint32_t(v));
-            res_reg_.push_back(reg8[v]);
-            rst.push_back(e);
-            rbits[v >> 5] |= 1u << (v & 31u);
-        }
-        res_n_ = uint32_t(res_vh_.size());
-    }
-    if (res_n_ > 0u) {
-        const size_t nb = rbits.size() * sizeof(uint32_t);
-        hip_check(hipMalloc(&res_bits_, nb), "hipMalloc reservoir bits");
-        hip_check(hipMalloc(&res_skew_, nb), "hipMalloc reservoir skew bits");
-        hip_check(hipMalloc(&res_v_, res_n_ * sizeof(uint32_t)), "hipMalloc reservoir voxels");
-        hip_check(hipMalloc(&res_st_, res_n_ * sizeof(ResVoxel)), "hipMalloc reservoir state");
-        hip_check(hipMemcpyAsync(res_bits_, rbits.data(), nb, hipMemcpyHostToDevice, st), "upload reservoir bits");
-        hip_check(hipMemsetAsync(res_skew_, 0, nb, st), "clear reservoir skew bits");
-        hip_check(hipMemcpyAsync(res_v_, res_vh_.data(), res_n_ * sizeof(uint32_t), hipMemcpyHostToDevice, st),
-                  "upload reservoir voxels");
-        hip_check(hipMemcpyAsync(res_st_, rst.data(), res_n_ * sizeof(ResVoxel), hipMemcpyHostToDevice, st),
-                  "upload reservoir state");
-    }
     std::vector<uint8_t> rvol;
     if (m.has_reservoir()) {  // the reservoir view's labels (final_stage.wgsl:63-66)
         rvol.resize(N_);
@@ -3347,6 +3830,7 @@ Device::Device(const Model &m, int hip_device) : dev_(hip_device)
 
     hip_check(hipMemcpyAsync(lat_[0], m.lattice().data(), N_ * S_, hipMemcpyHostToDevice, st), "upload lattice");
     hip_check(hipMemsetAsync(lat_[1], 0, N_ * S_, st), "clear lattice");
+    skew_->upload(st);
     // occupancy summaries of both buffers for the wave kernel (none with zero-order reactions, which can
     // fire in any voxel)
     summ_ = !simple_ && !tiled_ && !has_zero_order_;
@@ -3390,18 +3874,13 @@ Device::~Device()
     hipFree(counters_);
     hipFree(hist_);
     hipFree(fix_ctl_);
-    hipFree(fix_list_);
-    hipFree(fix_F_);
-    hipFree(fix_W_);
-    hipFree(fix_L_);
-    hipFree(fix_O_);
+    if (skew_) {
+        skew_->release();
+        delete skew_;
+    }
     hipFree(cme_counts_);
     hipFree(cme_tab_);
     hipFree(audit_);
-    hipFree(res_bits_);
-    hipFree(res_skew_);
-    hipFree(res_v_);
-    hipFree(res_st_);
     hipFree(res_vol_);
     hipFree(view_);
     hipFree(sum_[0]);
@@ -3424,12 +3903,8 @@ void Device::step(uint32_t itime)
     a.bound = bound_;
     a.counters = counters_;
     a.fix_ctl = fix_ctl_;
-    a.fix_list = fix_list_;
-    a.fix_F = fix_F_;
-    a.fix_W = fix_W_;
-    a.fix_L = fix_L_;
-    a.fix_O = fix_O_;
-    a.max_tight = kMaxTight;
+    a.fix_list = skew_->b.T;
+    a.max_tight = skew_->b.tcap;
     a.vox_base = 0;
     a.R0 = res_[0];
     a.R1 = res_[1];
@@ -3443,11 +3918,7 @@ void Device::step(uint32_t itime)
     a.itime = itime;
     a.mod_zero = mod_zero_;
     a.tau = tau_;
-    a.res_bits = res_bits_;
-    a.res_skew = res_skew_;
-    a.res_v = res_v_;
-    a.res_st = static_cast<ResVoxel *>(res_st_);
-    a.res_n = res_n_;
+    a.live = skew_->k.live;
     a.sum_dst = SumRef{summ_ ? sum_[cur_ ^ 1] : nullptr, res_[0], wtiles_z_, res_[1], res_[2]};
     const SumRef sum_src{summ_ ? sum_[cur_] : nullptr, res_[0], wtiles_z_, res_[1], res_[2]};
     hipStream_t st = static_cast<hipStream_t>(stream_);
@@ -3542,21 +4013,14 @@ void Device::step(uint32_t itime)
         }
     }
     hip_check(hipGetLastError(), "rdme step launch");
+    const SkewTab k = skew_->k;
+    const ReplayBuf rb = skew_->b;
     switch (S_) {
-    case 4: hipLaunchKernelGGL(tight_replay_kernel<4>, dim3(1), dim3(64), 0, st, a); break;
-    case 8: hipLaunchKernelGGL(tight_replay_kernel<8>, dim3(1), dim3(64), 0, st, a); break;
-    default: hipLaunchKernelGGL(tight_replay_kernel<16>, dim3(1), dim3(64), 0, st, a); break;
+    case 4: hipLaunchKernelGGL(tight_replay_kernel<4>, dim3(1), dim3(kReplayThreads), 0, st, a, k, rb); break;
+    case 8: hipLaunchKernelGGL(tight_replay_kernel<8>, dim3(1), dim3(kReplayThreads), 0, st, a, k, rb); break;
+    default: hipLaunchKernelGGL(tight_replay_kernel<16>, dim3(1), dim3(kReplayThreads), 0, st, a, k, rb); break;
     }
     hip_check(hipGetLastError(), "tight replay launch");
-    if (res_n_ > 0u) {
-        const uint32_t blocks = (res_n_ + 63u) / 64u;
-        switch (S_) {
-        case 4: hipLaunchKernelGGL(reservoir_react_kernel<4>, dim3(blocks), dim3(64), 0, st, a); break;
-        case 8: hipLaunchKernelGGL(reservoir_react_kernel<8>, dim3(blocks), dim3(64), 0, st, a); break;
-        default: hipLaunchKernelGGL(reservoir_react_kernel<16>, dim3(blocks), dim3(64), 0, st, a); break;
-        }
-        hip_check(hipGetLastError(), "reservoir reaction launch");
-    }
     ++steps_;
     cur_ ^= 1;
 }
@@ -3577,31 +4041,24 @@ void Device::render_view(uint32_t view, uint32_t axis, uint32_t slice, float *ou
     a.R2 = res_[2];
     a.R12 = res_[1] * res_[2];
     a.N = uint32_t(N_);
-    a.res_v = res_v_;
-    a.res_st = static_cast<ResVoxel *>(res_st_);
-    a.res_n = res_n_;
+    a.vox_base = 0u;
+    a.live = skew_->k.live;
+    const SkewTab k = skew_->k;
     const uint32_t blocks = uint32_t((N_ + kBlock - 1) / kBlock);
     switch (S_) {
-    case 4: hipLaunchKernelGGL(view_kernel<4>, dim3(blocks), dim3(kBlock), 0, st, a, res_vol_, view, axis, slice, view_); break;
-    case 8: hipLaunchKernelGGL(view_kernel<8>, dim3(blocks), dim3(kBlock), 0, st, a, res_vol_, view, axis, slice, view_); break;
-    default: hipLaunchKernelGGL(view_kernel<16>, dim3(blocks), dim3(kBlock), 0, st, a, res_vol_, view, axis, slice, view_); break;
+    case 4: hipLaunchKernelGGL(view_kernel<4>, dim3(blocks), dim3(kBlock), 0, st, a, k, res_vol_, view, axis, slice, view_); break;
+    case 8: hipLaunchKernelGGL(view_kernel<8>, dim3(blocks), dim3(kBlock), 0, st, a, k, res_vol_, view, axis, slice, view_); break;
+    default: hipLaunchKernelGGL(view_kernel<16>, dim3(blocks), dim3(kBlock), 0, st, a, k, res_vol_, view, axis, slice, view_); break;
     }
     hip_check(hipGetLastError(), "view launch");
     hip_check(hipMemcpyAsync(out, view_, N_ * sizeof(float), hipMemcpyDeviceToHost, st), "read view");
     hip_check(hipStreamSynchronize(st), "read view");
 }
 
-void Device::read_reservoir_state(std::vector<uint32_t> &voxels, std::vector<int32_t> &state)
+void Device::read_skew(std::vector<SkewVoxel> &out)
 {
-    voxels = res_vh_;
-    state.assign(size_t(res_n_) * 4u, 0);
-    if (res_n_ == 0u)
-        return;
     hip_check(hipSetDevice(dev_), "hipSetDevice");
-    hipStream_t st = static_cast<hipStream_t>(stream_);
-    hip_check(hipMemcpyAsync(state.data(), res_st_, res_n_ * sizeof(ResVoxel), hipMemcpyDeviceToHost, st),
-              "read reservoir state");
-    hip_check(hipStreamSynchronize(st), "read reservoir state");
+    skew_->read(static_cast<hipStream_t>(stream_), out);
 }
 
 float Device::run_timed(const uint32_t *itime, uint32_t nsteps)
@@ -3662,6 +4119,7 @@ void Device::write_lattice(const uint8_t *in)
     hip_check(hipSetDevice(dev_), "hipSetDevice");
     hipStream_t st = static_cast<hipStream_t>(stream_);
     hip_check(hipMemcpyAsync(lat_[cur_], in, N_ * S_, hipMemcpyHostToDevice, st), "write lattice");
+    skew_->upload(st);  // the new lattice starts with the reference's counters on its slots (reservoir units aside)
     std::vector<uint8_t> bytes;
     if (summ_) {
         bytes.assign(size_t(wtiles_y_) * wtiles_z_ * res_[0], 0u);
@@ -3815,19 +4273,14 @@ void Device::region_counts(int32_t *out)
     hip_check(hipGetLastError(), "region_count_kernel launch");
     hip_check(hipMemcpyAsync(out, hist_, nbins * sizeof(int32_t), hipMemcpyDeviceToHost, st), "read hist");
     hip_check(hipStreamSynchronize(st), "region counts");
-    if (res_n_ > 0u) {  // the reference's counts hold the reservoir units that no slot holds
-        std::vector<uint32_t> vox;
-        std::vector<int32_t> rs;
-        read_reservoir_state(vox, rs);
-        for (uint32_t i = 0; i < res_n_; ++i) {
-            const uint32_t sp = uint32_t(rs[4 * i]), K = sp & 0xFFu, R = sp >> 8;
-            int32_t *row = out + size_t(res_reg_[i]) * (ns_ + 1u);
-            if (K)
-                row[K] += rs[4 * i + 2];
-            if (R)
-                row[R] += rs[4 * i + 1];
-        }
-    }
+    // the reference's counts hold what the slots of skewed voxels do not (reservoir units, slot-S aliases,
+    // counts below zero)
+    std::vector<SkewVoxel> sk;
+    read_skew(sk);
+    for (const SkewVoxel &e : sk)
+        for (int q = 0; q < 4; ++q)
+            if (e.sp[q] != 0u && e.sp[q] <= ns_)
+                out[size_t(reg_host_[e.v]) * (ns_ + 1u) + e.sp[q]] += e.dc[q];
 }
 
 void Device::read_counters(uint64_t *out)
@@ -3867,8 +4320,6 @@ Ensemble::Ensemble(const std::vector<const Model *> &cells, const std::vector<ui
         if (m->slots() != S_)
             throw Error(CS_ERR_ARG, "all cells of an ensemble have the same slots per voxel");
         check_device_model(*m);
-        if (m->has_reservoir())
-            throw Error(CS_ERR_UNSUPPORTED, "ensemble cells cannot hold reservoir particles");
     }
     int ndev = 0;
     hip_check(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
@@ -3894,6 +4345,7 @@ Ensemble::Ensemble(const std::vector<const Model *> &cells, const std::vector<ui
     vox_off_.resize(n);
     res_.resize(3 * size_t(n));
     cell_nc_.resize(n);
+    cell_ns_.resize(n);
     cme_counts_off_.resize(n);
     uint64_t vox = 0;
     int32_t n_cme = 0;
@@ -3939,6 +4391,7 @@ Ensemble::Ensemble(const std::vector<const Model *> &cells, const std::vector<ui
         d.tau = lp.tau;
         d.itime_seed = itime_seeds[c];
         cell_nc_[c] = m.num_cme_species();
+        cell_ns_[c] = m.num_species();
         cme_counts_off_[c] = uint32_t(cme_init.size());
         d.cme_counts_off = cme_counts_off_[c];
         cme_init.insert(cme_init.end(), m.cme_init().begin(), m.cme_init().end());
@@ -3992,11 +4445,18 @@ Ensemble::Ensemble(const std::vector<const Model *> &cells, const std::vector<ui
     hip_check(hipMalloc(&audit_, 3 * sizeof(unsigned long long)), "hipMalloc audit");
     hip_check(hipMalloc(&cme_counts_, std::max<size_t>(cme_init.size(), 1) * sizeof(int32_t)), "hipMalloc cme counts");
     hip_check(hipMalloc(&fix_ctl_, kFixCtl * sizeof(uint32_t)), "hipMalloc fix_ctl");
-    hip_check(hipMalloc(&fix_list_, size_t(kMaxTight) * sizeof(uint32_t)), "hipMalloc fix_list");
-    hip_check(hipMalloc(&fix_F_, size_t(8 * kMaxTight) * sizeof(uint32_t)), "hipMalloc fix_F");
-    hip_check(hipMalloc(&fix_W_, size_t(56 * kMaxTight) * sizeof(uint32_t)), "hipMalloc fix_W");
-    hip_check(hipMalloc(&fix_L_, size_t(8 * kMaxTight) * S_), "hipMalloc fix_L");
-    hip_check(hipMalloc(&fix_O_, size_t(8 * kMaxTight) * sizeof(uint32_t)), "hipMalloc fix_O");
+    {
+        // skewed voxels of every cell, starting with the cells' reservoir voxels
+        uint64_t n_static = 0, max_clusters = 0;
+        for (const Model *m : cells) {
+            n_static += count_reservoir_voxels(*m);
+            max_clusters += m->params().res[0] / kClusterGap + 1u;
+        }
+        skew_ = new SkewStore();
+        skew_->alloc(total_vox_, S_, n_static, fix_ctl_, max_clusters);
+        for (uint32_t c = 0; c < n; ++c)
+            add_reservoir_entries(*skew_, *cells[c], uint32_t(vox_off_[c]));
+    }
     if (!cme_bytes.empty())
         hip_check(hipMalloc(&cme_tab_, cme_bytes.size()), "hipMalloc ensemble cme tables");
     if (n_cme > 0)
@@ -4064,6 +4524,7 @@ Ensemble::Ensemble(const std::vector<const Model *> &cells, const std::vector<ui
     hip_check(hipMemsetAsync(counters_, 0, kNumCounters * sizeof(unsigned long long), st), "clear counters");
     hip_check(hipMemsetAsync(audit_, 0, 3 * sizeof(unsigned long long), st), "clear audit");
     hip_check(hipMemsetAsync(fix_ctl_, 0, kFixCtl * sizeof(uint32_t), st), "clear fix_ctl");
+    skew_->upload(st);
     build_descriptors();
     hip_check(hipStreamSynchronize(st), "ensemble upload");
 }
@@ -4075,11 +4536,14 @@ Ensemble::~Ensemble()
         hipStreamSynchronize(static_cast<hipStream_t>(stream_));
     destroy_graph();
     void *bufs[] = {lat_[0], lat_[1], reg_, regp_, tab_, cells_dev_, cme_args_dev_, cme_tab_, cme_counts_, itime_, queue_,
-                    counters_, audit_, fix_ctl_, fix_list_, fix_F_, fix_W_, fix_L_, fix_O_, sum_[0], sum_[1],
-                    item_cell_};
+                    counters_, audit_, fix_ctl_, sum_[0], sum_[1], item_cell_};
     for (void *b : bufs)
         if (b)
             (void)hipFree(b);
+    if (skew_) {
+        skew_->release();
+        delete skew_;
+    }
     if (stream_)
         (void)hipStreamDestroy(static_cast<hipStream_t>(stream_));
 }
@@ -4156,12 +4620,11 @@ void Ensemble::enqueue_step()
     e.queue = queue_;
     e.counters = counters_;
     e.fix_ctl = fix_ctl_;
-    e.fix_list = fix_list_;
-    e.fix_F = fix_F_;
-    e.fix_W = fix_W_;
-    e.fix_O = fix_O_;
-    e.fix_L = fix_L_;
-    e.max_tight = kMaxTight;
+    e.fix_list = skew_->b.T;
+    e.max_tight = skew_->b.tcap;
+    e.live = skew_->k.live;
+    const SkewTab k = skew_->k;
+    const ReplayBuf rb = skew_->b;
     const uint32_t pro_blocks = (n_cells() + 63u) / 64u;
     const CmeArgs *ca = static_cast<const CmeArgs *>(cme_args_dev_);
     const uint32_t grid = std::min(tiling_.blocks_per_cu * n_cu_, std::max(total_items_, 1u));
@@ -4170,17 +4633,17 @@ void Ensemble::enqueue_step()
     case 4:
         hipLaunchKernelGGL(ens_prologue<4>, dim3(pro_blocks), dim3(64), 0, st, e, ca, cme_counts_);
         hipLaunchKernelGGL(rdme_step_ensemble<4>, dim3(grid), dim3(kWaveBlock), shm, st, e);
-        hipLaunchKernelGGL(ens_epilogue<4>, dim3(1), dim3(64), 0, st, e);
+        hipLaunchKernelGGL(ens_epilogue<4>, dim3(1), dim3(kReplayThreads), 0, st, e, k, rb);
         break;
     case 8:
         hipLaunchKernelGGL(ens_prologue<8>, dim3(pro_blocks), dim3(64), 0, st, e, ca, cme_counts_);
         hipLaunchKernelGGL(rdme_step_ensemble<8>, dim3(grid), dim3(kWaveBlock), shm, st, e);
-        hipLaunchKernelGGL(ens_epilogue<8>, dim3(1), dim3(64), 0, st, e);
+        hipLaunchKernelGGL(ens_epilogue<8>, dim3(1), dim3(kReplayThreads), 0, st, e, k, rb);
         break;
     default:
         hipLaunchKernelGGL(ens_prologue<16>, dim3(pro_blocks), dim3(64), 0, st, e, ca, cme_counts_);
         hipLaunchKernelGGL(rdme_step_ensemble<16>, dim3(grid), dim3(kWaveBlock), shm, st, e);
-        hipLaunchKernelGGL(ens_epilogue<16>, dim3(1), dim3(64), 0, st, e);
+        hipLaunchKernelGGL(ens_epilogue<16>, dim3(1), dim3(kReplayThreads), 0, st, e, k, rb);
         break;
     }
     hip_check(hipGetLastError(), "ensemble step launch");
@@ -4284,6 +4747,18 @@ float Ensemble::autotune(uint32_t trial_steps)
     hip_check(hipMalloc(&cme_snap, nc_total * sizeof(int32_t)), "hipMalloc snapshot");
     hip_check(hipMalloc(&q_snap, 2 * sizeof(uint32_t)), "hipMalloc snapshot");
     hip_check(hipMalloc(&cnt_snap, kNumCounters * sizeof(unsigned long long)), "hipMalloc snapshot");
+    // the skew table: keys and their entry indices, entries, control words and live bits
+    const SkewTab &kt = skew_->k;
+    const size_t key_bytes = size_t(kt.mask + 1u) * sizeof(uint32_t), ent_bytes = size_t(kt.ecap) * sizeof(SkewEntry);
+    const size_t live_bytes = std::max<size_t>((total_vox_ + 31u) / 32u, 1) * sizeof(uint32_t);
+    uint8_t *sk_snap = nullptr;
+    hip_check(hipMalloc(&sk_snap, 2 * key_bytes + ent_bytes + live_bytes + 4 * sizeof(uint32_t)), "hipMalloc snapshot");
+    struct Part {
+        void *dev;
+        size_t bytes;
+    };
+    const Part parts[5] = {{kt.keys, key_bytes}, {kt.idx, key_bytes}, {kt.ent, ent_bytes}, {kt.live, live_bytes},
+                           {kt.ctl, 4 * sizeof(uint32_t)}};
     const int cur0 = cur_;
     const uint64_t steps0 = steps_;
     // both buffers: the one a step writes is also the one whose emptiness its summary vouches for
@@ -4297,6 +4772,11 @@ float Ensemble::autotune(uint32_t trial_steps)
         hip_check(hipMemcpyAsync(q_snap, queue_, 2 * sizeof(uint32_t), hipMemcpyDeviceToDevice, st), "snapshot");
         hip_check(hipMemcpyAsync(cnt_snap, counters_, kNumCounters * sizeof(unsigned long long), hipMemcpyDeviceToDevice, st),
                   "snapshot");
+        size_t o = 0;
+        for (const Part &p : parts) {
+            hip_check(hipMemcpyAsync(sk_snap + o, p.dev, p.bytes, hipMemcpyDeviceToDevice, st), "snapshot");
+            o += p.bytes;
+        }
     };
     auto restore = [&] {
         synchronize();
@@ -4311,6 +4791,11 @@ float Ensemble::autotune(uint32_t trial_steps)
         hip_check(hipMemcpyAsync(queue_, q_snap, 2 * sizeof(uint32_t), hipMemcpyDeviceToDevice, st), "restore");
         hip_check(hipMemcpyAsync(counters_, cnt_snap, kNumCounters * sizeof(unsigned long long), hipMemcpyDeviceToDevice, st),
                   "restore");
+        size_t o = 0;
+        for (const Part &p : parts) {
+            hip_check(hipMemcpyAsync(p.dev, sk_snap + o, p.bytes, hipMemcpyDeviceToDevice, st), "restore");
+            o += p.bytes;
+        }
         synchronize();
     };
     save();
@@ -4338,6 +4823,7 @@ float Ensemble::autotune(uint32_t trial_steps)
     (void)hipFree(cme_snap);
     (void)hipFree(q_snap);
     (void)hipFree(cnt_snap);
+    (void)hipFree(sk_snap);
     return best_ms;
 }
 
@@ -4367,6 +4853,20 @@ void Ensemble::read_lattice(uint32_t cell, uint8_t *out)
     const uint64_t nv = cell_voxels(cell);
     synchronize();
     hip_check(hipMemcpy(out, lat_[cur_] + lat_off_[cell], nv * S_, hipMemcpyDeviceToHost), "read cell lattice");
+}
+
+void Ensemble::read_skew(uint32_t cell, std::vector<SkewVoxel> &out)
+{
+    const uint64_t nv = cell_voxels(cell), base = vox_off_[cell];
+    synchronize();
+    std::vector<SkewVoxel> all;
+    skew_->read(static_cast<hipStream_t>(stream_), all);
+    out.clear();
+    for (SkewVoxel e : all)
+        if (e.v >= base && e.v < base + nv) {
+            e.v -= uint32_t(base);
+            out.push_back(e);
+        }
 }
 
 void Ensemble::read_cme_counts(uint32_t cell, int32_t *out)

@@ -13,7 +13,19 @@
 
 namespace cs {
 
-constexpr int kNumCounters = 8;
+constexpr int kNumCounters = 10;
+
+// A voxel whose reference counters differ from its slots: the occupancy word is the occupied slots + off, the
+// count of species sp[q] the slots holding it + dc[q] (sp[q] = 0: unused), and the reaction skips species K.
+struct SkewVoxel {
+    uint32_t v;       // voxel index (among all cells' voxels for an ensemble)
+    int32_t off;
+    uint32_t sp[4];
+    int32_t dc[4];
+    uint32_t K;
+};
+
+struct SkewStore;  // the device table of skewed voxels and the replay scratch (device.hip)
 
 class Device {
 public:
@@ -39,9 +51,9 @@ public:
     uint32_t slots() const { return S_; }
     uint64_t steps() const { return steps_; }
 
-    // Reservoir voxels (ascending indices) and, per voxel, four int32: K | R << 8, the units of R and of K
-    // that the reference's per-voxel counts hold beyond the slots, and occupancy - occupied slots.
-    void read_reservoir_state(std::vector<uint32_t> &voxels, std::vector<int32_t> &state);
+    // The skewed voxels of the current state (reservoir voxels and those the capacity hazards or negative counts
+    // of the reference made), with their offsets.
+    void read_skew(std::vector<SkewVoxel> &out);
     // The renderer's view of the current state (final_stage.wgsl:35-67), f32[N] in voxel order.
     void render_view(uint32_t view, uint32_t axis, uint32_t slice, float *out);
 
@@ -91,12 +103,8 @@ private:
     float *bound_ = nullptr;       // [S+1]: upper bound of the total propensity of a voxel with c particles
     unsigned long long *counters_ = nullptr;
     int32_t *hist_ = nullptr;      // [nreg][ns+1]
-    uint32_t *fix_ctl_ = nullptr;  // tight-voxel replay: control words, lists and slot copies
-    uint32_t *fix_list_ = nullptr;
-    uint32_t *fix_F_ = nullptr;
-    uint32_t *fix_W_ = nullptr;
-    uint8_t *fix_L_ = nullptr;
-    uint32_t *fix_O_ = nullptr;
+    uint32_t *fix_ctl_ = nullptr;  // replay control words and the wave kernel's queues
+    SkewStore *skew_ = nullptr;    // skewed voxels and the replay scratch
 
     bool has_cme_ = false;         // some well-mixed reaction exists
     uint32_t nc_ = 0, ncr_ = 0, cme_seed_ = 0;
@@ -108,13 +116,7 @@ private:
     unsigned long long *audit_ = nullptr;  // [3]
 
     uint64_t steps_ = 0;           // steps enqueued
-    uint32_t res_n_ = 0;           // reservoir voxels
-    uint32_t *res_bits_ = nullptr; // [N/32] bit per reservoir voxel
-    uint32_t *res_skew_ = nullptr; // [N/32] bit per reservoir voxel whose occupancy offset is non-zero
-    uint32_t *res_v_ = nullptr;    // [res_n] voxel indices, ascending
-    void *res_st_ = nullptr;       // [res_n] per-voxel ghost units and occupancy offset
-    std::vector<uint32_t> res_vh_; // host copies: voxel indices and their region labels
-    std::vector<uint8_t> res_reg_;
+    std::vector<uint8_t> reg_host_;  // region label per voxel (per-region counts of skewed voxels' offsets)
     uint8_t *res_vol_ = nullptr;   // [N] reservoir species (reservoir models only), for the view
     bool summ_ = false;            // occupancy summaries kept (wave kernel, no zero-order reactions)
     uint8_t *sum_[2] = {nullptr, nullptr};  // summary of lat_[0], lat_[1]: a byte per wave tile and plane

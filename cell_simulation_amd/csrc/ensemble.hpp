@@ -12,6 +12,7 @@
 #include <cstdint>
 #include <vector>
 
+#include "device.hpp"
 #include "model.hpp"
 
 namespace cs {
@@ -49,7 +50,10 @@ public:
     uint64_t cell_voxels(uint32_t c) const;
     void cell_res(uint32_t c, uint32_t res[3]) const;
     uint32_t cell_cme_species(uint32_t c) const { return cell_nc_.at(c); }
+    uint32_t cell_species(uint32_t c) const { return cell_ns_.at(c); }
     void read_lattice(uint32_t cell, uint8_t *out);
+    // The skewed voxels of one cell (voxel indices within the cell), with their offsets.
+    void read_skew(uint32_t cell, std::vector<SkewVoxel> &out);
     void read_cme_counts(uint32_t cell, int32_t *out);
     void read_counters(uint64_t *out);
     void reset_counters();
@@ -77,6 +81,7 @@ private:
     std::vector<uint64_t> regp_off_;  // per cell, words of the padded label buffer
     std::vector<uint32_t> res_;       // per cell, 3 entries
     std::vector<uint32_t> cell_nc_;   // well-mixed species per cell
+    std::vector<uint32_t> cell_ns_;   // lattice species per cell
     std::vector<uint32_t> cme_counts_off_;
     std::vector<uint8_t> cells_;      // host copy of the device descriptors (n_cells_ EnsCell), rebuilt per tiling
 
@@ -95,8 +100,8 @@ private:
     uint32_t *queue_ = nullptr;        // [0] next item, [1] step counter
     unsigned long long *counters_ = nullptr;
     unsigned long long *audit_ = nullptr;
-    uint32_t *fix_ctl_ = nullptr, *fix_list_ = nullptr, *fix_F_ = nullptr, *fix_W_ = nullptr, *fix_O_ = nullptr;
-    uint8_t *fix_L_ = nullptr;
+    uint32_t *fix_ctl_ = nullptr;
+    SkewStore *skew_ = nullptr;        // skewed voxels of every cell and the replay scratch
     void *graph_exec_ = nullptr;       // hipGraphExec_t
     uint32_t graph_block_ = 0;
     int graph_parity_ = 0;

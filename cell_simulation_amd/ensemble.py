@@ -30,6 +30,7 @@ class Ensemble:
         check(lib().cs_ens_create(handles, n, seeds, int(device), C.byref(h)))
         self._h = h
         self._nc = [c.info()["num_cme_species"] for c in cells]
+        self._ns = [c.info()["num_species"] for c in cells]
 
     def close(self) -> None:
         if getattr(self, "_h", None):
@@ -94,6 +95,22 @@ class Ensemble:
         out = np.zeros(max(n, 1), dtype=np.int32)
         check(lib().cs_ens_read_cme_counts(self._h, int(cell), out.ctypes.data, n))
         return out[:n]
+
+    def concentrations(self, cell: int) -> np.ndarray:
+        """The cell's per-voxel counts C[v][sp], int32[N, ns+1] (see Simulation.concentrations)."""
+        res = self.cell_res(cell)
+        n = res[0] * res[1] * res[2]
+        n1 = self._ns[cell] + 1
+        out = np.empty(n * n1, dtype=np.int32)
+        check(lib().cs_ens_read_concentrations(self._h, int(cell), out.ctypes.data, out.size))
+        return out.reshape(n, n1)
+
+    def occupancy(self, cell: int) -> np.ndarray:
+        """The cell's occupancy words, uint32[N] (see Simulation.occupancy)."""
+        res = self.cell_res(cell)
+        out = np.empty(res[0] * res[1] * res[2], dtype=np.uint32)
+        check(lib().cs_ens_read_occupancy(self._h, int(cell), out.ctypes.data, out.size))
+        return out
 
     def counters(self) -> np.ndarray:
         out = np.zeros(_lib.CS_NUM_COUNTERS, dtype=np.uint64)

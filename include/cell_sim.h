@@ -225,15 +225,19 @@ typedef struct {
 int cs_sim_pop_stat(cs_sim *sim, cs_stat_sample *out);
 
 /* Step diagnostics accumulated on the device since prepare (uint64[CS_NUM_COUNTERS]). */
-#define CS_CNT_TIGHT 0           /* voxel-steps whose capacity could be reached (not gathered exactly) */
-#define CS_CNT_EXCEPTION 1       /* voxel-steps the byte-slot state cannot represent (negative count) */
+#define CS_CNT_TIGHT 0           /* voxel-steps whose capacity could be reached (replayed in canonical order) */
+#define CS_CNT_EXCEPTION 1       /* voxel-steps not followed exactly: replay scratch or skew table exhausted, more than
+                                    4 species with count offsets in one voxel, or an occupancy word wrapped below
+                                    zero at the first voxels (whose sweep walks the whole lattice) */
 #define CS_CNT_NEAR_THRESHOLD 2  /* voxel-steps with |r1 - (1-exp(-a0*tau))| <= 2^-24 */
 #define CS_CNT_FIRED 3           /* voxel-steps where a reaction fired */
 #define CS_CNT_ROLLBACK 4        /* fired reactions undone because S slots would fill */
 #define CS_CNT_CME_EVENTS 5      /* well-mixed reactions fired */
 #define CS_CNT_CME_NEAR 6        /* CME steps whose last waiting time ended within 1e-12*tau of tau */
 #define CS_CNT_EMIT_FAIL 7       /* lattice products of CME reactions dropped: no free slot in 100 draws */
-#define CS_NUM_COUNTERS 8
+#define CS_CNT_FAILED_CLAIM 8    /* hops rejected because the destination was full (rdme.wgsl:62, 73-81) */
+#define CS_CNT_ALIAS 9           /* claims that landed in slot S, the next voxel's slot 0 (rdme.wgsl:30-40) */
+#define CS_NUM_COUNTERS 10
 int cs_sim_read_counters(cs_sim *sim, uint64_t *out, size_t n);
 int cs_sim_reset_counters(cs_sim *sim);
 
@@ -275,13 +279,17 @@ int cs_ens_run_timed(cs_ens *ens, uint32_t nsteps, uint32_t block, float *ms);
 int cs_ens_synchronize(cs_ens *ens);
 int cs_ens_set_tiling(cs_ens *ens, const cs_ens_tiling *t);
 int cs_ens_get_tiling(const cs_ens *ens, cs_ens_tiling *t);
-/* Tries tilings (planes per item 8/16/32, 2-4 blocks per CU, queue batch 0/1/4) for `trial_steps` steps
- * each from a snapshot of the state (restored afterwards) and keeps the fastest. */
+/* Tries tilings (planes per item 8/16/32/48/60, 2-4 blocks per CU, queue batch 0/1/4) for `trial_steps`
+ * steps each from a snapshot of the state (restored afterwards) and keeps the fastest. */
 int cs_ens_autotune(cs_ens *ens, uint32_t trial_steps, float *ms_per_step);
 int cs_ens_get_info(const cs_ens *ens, cs_ens_info *out);
 int cs_ens_cell_res(const cs_ens *ens, uint32_t cell, uint32_t res[3]);
 int cs_ens_read_lattice(cs_ens *ens, uint32_t cell, uint8_t *out, size_t nbytes);
 int cs_ens_read_cme_counts(cs_ens *ens, uint32_t cell, int32_t *out, size_t n);
+/* One cell's per-voxel counts int32[N][ns+1] and occupancy words uint32[N], as cs_sim_read_concentrations and
+ * cs_sim_read_occupancy give them for a simulation. */
+int cs_ens_read_concentrations(cs_ens *ens, uint32_t cell, int32_t *out, size_t n);
+int cs_ens_read_occupancy(cs_ens *ens, uint32_t cell, uint32_t *out, size_t n);
 int cs_ens_read_counters(cs_ens *ens, uint64_t *out, size_t n);
 int cs_ens_reset_counters(cs_ens *ens);
 

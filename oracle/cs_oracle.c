@@ -8,6 +8,7 @@
 #include "cs_oracle.h"
 
 #include <math.h>
+#include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -144,6 +145,34 @@ static void move_particle_site(const cso_model *m, cso_state *st, uint32_t sp, u
         cnt->failed_claims++;
 }
 
+/* rdme.wgsl:205-226 for the particle at lattice index i_part walked by voxel (x, y, z): its hop draw and, with
+ * move_particle (:91-130), the destination voxel, or UINT32_MAX when it stays (direction 6 or a reflecting
+ * boundary). */
+static uint32_t hop_target(const cso_model *m, uint32_t x, uint32_t y, uint32_t z, uint32_t sp, uint32_t i_part,
+                           uint32_t itime)
+{
+    const uint32_t R0 = m->res[0], R1 = m->res[1], R2 = m->res[2];
+    uint32_t xmy = (y != 0u) ? (x % y) : m->mod_zero;  /* hazard H4 */
+    float p[7];
+    cso_probability_vector(m, x, y, z, sp, p);
+    uint32_t state = cso_pcg(itime + xmy + z + i_part);
+    float r = cso_uniform_float(state);
+    int i = 0;
+    while (i < 6 && r > p[i])
+        ++i;
+    uint32_t dx = x, dy = y, dz = z;
+    switch (i) {
+    case 0: if (x == 0u) return UINT32_MAX; dx = x - 1u; break;
+    case 1: if (x == R0 - 1u) return UINT32_MAX; dx = x + 1u; break;
+    case 2: if (y == 0u) return UINT32_MAX; dy = y - 1u; break;
+    case 3: if (y == R1 - 1u) return UINT32_MAX; dy = y + 1u; break;
+    case 4: if (z == 0u) return UINT32_MAX; dz = z - 1u; break;
+    case 5: if (z == R2 - 1u) return UINT32_MAX; dz = z + 1u; break;
+    default: return UINT32_MAX;
+    }
+    return vidx(m, dx, dy, dz);
+}
+
 /* rdme.wgsl:194-229 (with move_particle :91-130), all voxels in ascending order. */
 void cso_phase_r(const cso_model *m, const uint32_t *lat_src, const uint32_t *occ_src,
                  cso_state *st, uint32_t itime, cso_counters *cnt)
@@ -156,39 +185,19 @@ void cso_phase_r(const cso_model *m, const uint32_t *lat_src, const uint32_t *oc
                 uint32_t v = vidx(m, x, y, z);
                 uint32_t idx_lattice = v * m->S;             /* u32, functions.wgsl:42-46 */
                 uint32_t occ = occ_src[v];
-                uint32_t xmy = (y != 0u) ? (x % y) : m->mod_zero;  /* hazard H4 */
                 for (uint32_t i_part = idx_lattice; i_part < idx_lattice + occ; ++i_part) {
                     uint32_t sp = (i_part < total) ? lat_src[i_part] : 0u;
                     if (sp == 0u)
                         continue;
-                    float p[7];
-                    cso_probability_vector(m, x, y, z, sp, p);
-                    uint32_t state = cso_pcg(itime + xmy + z + i_part);
-                    float r = cso_uniform_float(state);
-                    int i = 0;
-                    while (i < 6 && r > p[i])
-                        ++i;
-                    /* move_particle: reflecting boundary returns without moving; 6 = stay */
-                    uint32_t dx = x, dy = y, dz = z;
-                    int ok = 1;
-                    switch (i) {
-                    case 0: if (x == 0u) ok = 0; else dx = x - 1u; break;
-                    case 1: if (x == R0 - 1u) ok = 0; else dx = x + 1u; break;
-                    case 2: if (y == 0u) ok = 0; else dy = y - 1u; break;
-                    case 3: if (y == R1 - 1u) ok = 0; else dy = y + 1u; break;
-                    case 4: if (z == 0u) ok = 0; else dz = z - 1u; break;
-                    case 5: if (z == R2 - 1u) ok = 0; else dz = z + 1u; break;
-                    default: ok = 0; break;
-                    }
-                    if (!ok)
-                        continue;
-                    move_particle_site(m, st, sp, v, vidx(m, dx, dy, dz), i_part, cnt);
+                    uint32_t d = hop_target(m, x, y, z, sp, i_part, itime);
+                    if (d != UINT32_MAX)
+                        move_particle_site(m, st, sp, v, d, i_part, cnt);
                 }
             }
 }
 
 /* cme.wgsl:25-120 for one voxel. */
-static void cme_voxel(const cso_model *m, cso_state *st, uint32_t x, uint32_t y, uint32_t z,
+static void cme_voxel(const cso_model *m, cso_state *st, int32_t *stat, uint32_t x, uint32_t y, uint32_t z,
                       uint32_t itime, float *cumm, cso_counters *cnt)
 {
     const uint32_t n1 = m->ns + 1u, S = m->S, nr = m->nr;
@@ -242,7 +251,7 @@ static void cme_voxel(const cso_model *m, cso_state *st, uint32_t x, uint32_t y,
             continue;
         C[sp] += row[sp];
         new_occ += row[sp];
-        st->stat[sp] += row[sp];
+        stat[sp] += row[sp];
         int32_t cc = C[sp];
         while (cc > 0) {
             L[j] = sp;
@@ -253,7 +262,7 @@ static void cme_voxel(const cso_model *m, cso_state *st, uint32_t x, uint32_t y,
                     L[q] = cell_copy[q];
                 for (uint32_t q = 1; q <= sp; ++q) {
                     C[q] -= row[q];
-                    st->stat[q] -= row[q];
+                    stat[q] -= row[q];
                 }
                 if (cnt)
                     cnt->rollbacks++;
@@ -275,7 +284,7 @@ void cso_phase_c(const cso_model *m, cso_state *st, uint32_t itime, cso_counters
     for (uint32_t x = 0; x < m->res[0]; ++x)
         for (uint32_t y = 0; y < m->res[1]; ++y)
             for (uint32_t z = 0; z < m->res[2]; ++z)
-                cme_voxel(m, st, x, y, z, itime, cumm, cnt);
+                cme_voxel(m, st, st->stat, x, y, z, itime, cumm, cnt);
     free(cumm);
 }
 
@@ -329,6 +338,197 @@ int cso_run(const cso_model *m, cso_state *st, const uint32_t *itime, uint32_t n
             return rc;
     }
     return 0;
+}
+
+/* ---- the same step on several host threads (the CPU baseline) ------------------------------------------
+ * The hop draws of phase R depend only on the source state, so the threads draw them for the first S walked
+ * slots of every voxel (x-planes split among the threads) into a direction table; one thread then applies the
+ * claims in the canonical order exactly as cso_phase_r does (a walk past slot S draws its extra hops there).
+ * Phase C and phase F touch one voxel each and run on all threads, and so does the copy of the source state;
+ * the global counts and the event counters are summed per thread. The result equals cso_step's bit for bit. */
+typedef struct {
+    const cso_model *m;
+    cso_state *st;
+    uint32_t *lat_src, *occ_src;
+    uint8_t *dir;            /* [N*S] 0..5: hop direction; 6: stays or no particle */
+    uint32_t *walkers;       /* this thread's voxels with a non-zero occupancy word, ascending */
+    uint32_t n_walkers;
+    uint32_t itime, x0, x1;
+    int phase;               /* 0: source copy, 1: hop draws, 2: phase C, 3: phase F */
+    int32_t *stat;           /* [ns+1] this thread's changes of the global counts */
+    float *cumm;             /* [nr+1] */
+    cso_counters cnt;
+    char pad[64];            /* keeps the next thread's counters off this cache line */
+} mt_job;
+
+static uint8_t hop_dir(const cso_model *m, uint32_t x, uint32_t y, uint32_t z, uint32_t d)
+{
+    const uint32_t R1 = m->res[1], R2 = m->res[2];
+    if (d == UINT32_MAX)
+        return 6u;
+    const uint32_t dx = d / (R1 * R2), dy = (d / R2) % R1, dz = d % R2;
+    if (dx != x)
+        return dx < x ? 0u : 1u;
+    if (dy != y)
+        return dy < y ? 2u : 3u;
+    return dz < z ? 4u : 5u;
+}
+
+static void *mt_worker(void *arg)
+{
+    mt_job *j = (mt_job *)arg;
+    const cso_model *m = j->m;
+    const uint32_t R1 = m->res[1], R2 = m->res[2], S = m->S;
+    const uint64_t total = nvox(m) * S;
+    if (j->phase == 0) {  /* this slab's copy of the source state (the next voxel's slots come from the next slab) */
+        const uint64_t v0 = (uint64_t)j->x0 * R1 * R2, v1 = (uint64_t)j->x1 * R1 * R2;
+        memcpy(j->lat_src + v0 * S, j->st->lattice + v0 * S, sizeof(uint32_t) * (v1 - v0) * S);
+        memcpy(j->occ_src + v0, j->st->occupancy + v0, sizeof(uint32_t) * (v1 - v0));
+        return NULL;
+    }
+    if (j->phase == 1)
+        j->n_walkers = 0;
+    for (uint32_t x = j->x0; x < j->x1; ++x)
+        for (uint32_t y = 0; y < R1; ++y)
+            for (uint32_t z = 0; z < R2; ++z) {
+                const uint32_t v = vidx(m, x, y, z);
+                if (j->phase == 1) {
+                    const uint32_t start = v * S, end = start + j->occ_src[v];
+                    const uint32_t walk = end > start ? end - start : 0u;
+                    if (j->occ_src[v] == 0u)
+                        continue;
+                    j->walkers[j->n_walkers++] = v;
+                    for (uint32_t s = 0; s < S; ++s) {
+                        const uint32_t ip = start + s;
+                        const uint32_t sp = (s < walk && ip < total) ? j->lat_src[ip] : 0u;
+                        j->dir[ip] = sp ? hop_dir(m, x, y, z, hop_target(m, x, y, z, sp, ip, j->itime)) : 6u;
+                    }
+                } else if (j->phase == 2) {
+                    cme_voxel(m, j->st, j->stat, x, y, z, j->itime, j->cumm, &j->cnt);
+                } else {
+                    uint32_t *L = j->st->lattice + (uint64_t)v * S;
+                    uint32_t k = 0;
+                    for (uint32_t i = 0; i < S; ++i)
+                        if (L[i] != 0u) {
+                            uint32_t t = L[i];
+                            L[i] = L[k];
+                            L[k] = t;
+                            k += 1;
+                        }
+                }
+            }
+    return NULL;
+}
+
+static void mt_phase(mt_job *jobs, uint32_t nt, int phase)
+{
+    pthread_t th[256];
+    for (uint32_t t = 0; t < nt; ++t) {
+        jobs[t].phase = phase;
+        if (t > 0 && pthread_create(&th[t], NULL, mt_worker, &jobs[t]) != 0) {
+            mt_worker(&jobs[t]);  /* no thread to spare: run it here */
+            th[t] = 0;
+        }
+    }
+    mt_worker(&jobs[0]);
+    for (uint32_t t = 1; t < nt; ++t)
+        if (th[t])
+            pthread_join(th[t], NULL);
+}
+
+int cso_run_mt(const cso_model *m, cso_state *st, const uint32_t *itime, uint32_t nsteps, cso_counters *cnt,
+               uint32_t nthreads)
+{
+    if (!m || !st || m->S == 0u || m->S > 16u || m->res[0] == 0u || m->res[1] == 0u || m->res[2] == 0u)
+        return 1;
+    const uint64_t n = nvox(m), total = n * m->S;
+    uint32_t nt = nthreads < 1u ? 1u : (nthreads > 256u ? 256u : nthreads);
+    if (nt > m->res[0])
+        nt = m->res[0];
+    uint32_t *lat_src = (uint32_t *)malloc(sizeof(uint32_t) * total);
+    uint32_t *occ_src = (uint32_t *)malloc(sizeof(uint32_t) * n);
+    uint8_t *dir = (uint8_t *)malloc(total);
+    uint32_t *walkers = (uint32_t *)malloc(sizeof(uint32_t) * n);
+    /* per-thread scratch on cache lines of its own (cme_voxel writes cumm for every voxel) */
+    const size_t stat_stride = ((m->ns + 1u + 15u) / 16u + 1u) * 16u, cumm_stride = ((m->nr + 1u + 15u) / 16u + 1u) * 16u;
+    int32_t *stats = (int32_t *)calloc((size_t)nt * stat_stride, sizeof(int32_t));
+    float *cumm = (float *)malloc(sizeof(float) * (size_t)nt * cumm_stride);
+    mt_job jobs[256];
+    int rc = 0;
+    if (!lat_src || !occ_src || !dir || !walkers || !stats || !cumm) {
+        rc = 2;
+        goto done;
+    }
+    for (uint32_t t = 0; t < nt; ++t) {
+        memset(&jobs[t], 0, sizeof(mt_job));
+        jobs[t].m = m;
+        jobs[t].st = st;
+        jobs[t].lat_src = lat_src;
+        jobs[t].occ_src = occ_src;
+        jobs[t].dir = dir;
+        jobs[t].x0 = (uint32_t)((uint64_t)m->res[0] * t / nt);
+        jobs[t].x1 = (uint32_t)((uint64_t)m->res[0] * (t + 1u) / nt);
+        jobs[t].walkers = walkers + (uint64_t)jobs[t].x0 * m->res[1] * m->res[2];
+        jobs[t].stat = stats + (size_t)t * stat_stride;
+        jobs[t].cumm = cumm + (size_t)t * cumm_stride;
+    }
+    for (uint32_t k = 0; k < nsteps; ++k) {
+        for (uint32_t t = 0; t < nt; ++t)
+            jobs[t].itime = itime[k];
+        mt_phase(jobs, nt, 0);
+        mt_phase(jobs, nt, 1);
+        /* phase R's claims in the canonical order (as cso_phase_r), over the voxels that walk any slot */
+        const uint32_t R12 = m->res[1] * m->res[2];
+        for (uint32_t t = 0; t < nt; ++t)
+            for (uint32_t q = 0; q < jobs[t].n_walkers; ++q) {
+                    const uint32_t v = jobs[t].walkers[q], start = v * m->S, occ = occ_src[v];
+                    const uint32_t x = v / R12, y = (v / m->res[2]) % m->res[1], z = v % m->res[2];
+                    for (uint32_t i_part = start; i_part < start + occ; ++i_part) {
+                        const uint32_t sp = (i_part < total) ? lat_src[i_part] : 0u;
+                        if (sp == 0u)
+                            continue;
+                        uint32_t d;
+                        if (i_part - start < m->S) {
+                            const uint8_t dd = dir[i_part];
+                            if (dd >= 6u)
+                                continue;
+                            d = v;
+                            switch (dd) {
+                            case 0: d = v - m->res[1] * m->res[2]; break;
+                            case 1: d = v + m->res[1] * m->res[2]; break;
+                            case 2: d = v - m->res[2]; break;
+                            case 3: d = v + m->res[2]; break;
+                            case 4: d = v - 1u; break;
+                            default: d = v + 1u; break;
+                            }
+                        } else {
+                            d = hop_target(m, x, y, z, sp, i_part, itime[k]);
+                            if (d == UINT32_MAX)
+                                continue;
+                        }
+                        move_particle_site(m, st, sp, v, d, i_part, cnt);
+                    }
+            }
+        mt_phase(jobs, nt, 2);
+        mt_phase(jobs, nt, 3);
+    }
+    for (uint32_t t = 0; t < nt; ++t) {
+        for (uint32_t sp = 0; sp <= m->ns; ++sp)
+            st->stat[sp] += jobs[t].stat[sp];
+        if (cnt) {
+            cnt->fired += jobs[t].cnt.fired;
+            cnt->rollbacks += jobs[t].cnt.rollbacks;
+            cnt->near_threshold += jobs[t].cnt.near_threshold;
+        }
+    }
+done:
+    free(lat_src);
+    free(occ_src);
+    free(dir);
+    free(walkers);
+    free(stats);
+    free(cumm);
+    return rc;
 }
 
 /* ---- well-mixed (CME) phase: the build's own definition (cs_oracle.h), no reference counterpart ---- */
@@ -464,10 +664,13 @@ void cso_cme_phase(const cso_model *m, const cso_cme_model *c, cso_state *st, in
             for (uint32_t att = 0; att < 100u && n > 0u && !placed; ++att) {
                 uint32_t u = cso_pcg(itime + cso_pcg((c->seed ^ 0x9E3779B9u) + ((e * 4u + j) << 7) + att));
                 uint32_t v = list[(uint32_t)(((uint64_t)u * n) >> 32)];
-                uint32_t occ = st->occupancy[v];
+                uint32_t *blk = st->lattice + (uint64_t)v * m->S;
+                uint32_t occ = 0u;                   /* the voxel's particles (its slots are compacted) */
+                while (occ < m->S && blk[occ] != 0u)
+                    ++occ;
                 if (occ < m->S) {
-                    st->lattice[(uint64_t)v * m->S + occ] = sp;
-                    st->occupancy[v] = occ + 1u;
+                    blk[occ] = sp;
+                    st->occupancy[v] += 1u;
                     st->counts[(uint64_t)v * (m->ns + 1u) + sp] += 1;
                     st->stat[sp] += 1;
                     placed = 1;

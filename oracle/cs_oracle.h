@@ -83,6 +83,11 @@ void cso_probability_vector(const cso_model *m, uint32_t x, uint32_t y, uint32_t
 /* Multi-step driver: steps t = 0..nsteps-1 use itime[t]. */
 int cso_run(const cso_model *m, cso_state *st, const uint32_t *itime, uint32_t nsteps, cso_counters *cnt);
 
+/* cso_run on `nthreads` host threads (the CPU baseline): hop draws, phase C and phase F in parallel over
+ * x-planes, the claims of phase R applied serially in the canonical order. Same result as cso_run. */
+int cso_run_mt(const cso_model *m, cso_state *st, const uint32_t *itime, uint32_t nsteps, cso_counters *cnt,
+               uint32_t nthreads);
+
 /* final_stage.wgsl:35-67: the r32float view of the current state, one value per voxel in voxel order
  * (out[v], v = z + y*R2 + x*R2*R1). Voxels with res[axis] - coord[axis] < slice are 0. view 0: the
  * species found most often among slots 0..occupancy-1 (lattice index order, so slots past S are the next
@@ -103,7 +108,8 @@ void cso_view(const cso_model *m, const cso_state *st, uint32_t view, uint32_t a
  *            (the last r with a_r > 0 if rounding leaves none);
  *   lattice product j of event e goes to region voxel list[(u64)u * len >> 32] with
  *            u = pcg(itime + pcg((seed ^ 0x9E3779B9) + ((4e + j) << 7) + attempt)), attempts 0..99,
- *            appended after the voxel's particles when it has a free slot. */
+ *            appended after the voxel's particles (slot = particles held, its slots being compacted) when it
+ *            has a free slot; the voxel's occupancy word and count of that species go up by one. */
 typedef struct {
     uint32_t nc, ncr;                      /* well-mixed species, reactions */
     const int32_t *reactants;              /* [ncr][3], -1 = none */

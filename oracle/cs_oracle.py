@@ -79,6 +79,9 @@ def lib() -> C.CDLL:
         d.cso_step.argtypes = [C.POINTER(Model), C.POINTER(State), C.c_uint32, C.POINTER(Counters)]
         d.cso_run.restype = C.c_int
         d.cso_run.argtypes = [C.POINTER(Model), C.POINTER(State), C.c_void_p, C.c_uint32, C.POINTER(Counters)]
+        d.cso_run_mt.restype = C.c_int
+        d.cso_run_mt.argtypes = [C.POINTER(Model), C.POINTER(State), C.c_void_p, C.c_uint32, C.POINTER(Counters),
+                                 C.c_uint32]
         d.cso_probability_vector.restype = None
         d.cso_probability_vector.argtypes = [C.POINTER(Model), C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                              C.POINTER(C.c_float)]
@@ -226,6 +229,16 @@ class Oracle:
                            C.byref(self.counters))
         if rc:
             raise RuntimeError(f"cso_run failed ({rc})")
+
+    def run_mt(self, itimes, threads: int) -> None:
+        """run() on `threads` host threads (cso_run_mt: the same result); lattice-only models."""
+        if self.cme is not None and self.cme.ncr > 0:
+            raise ValueError("run_mt steps lattice models only")
+        arr = np.ascontiguousarray(np.asarray(itimes, dtype=np.uint32))
+        rc = lib().cso_run_mt(C.byref(self.model), C.byref(self.state), arr.ctypes.data, arr.size,
+                              C.byref(self.counters), int(threads))
+        if rc:
+            raise RuntimeError(f"cso_run_mt failed ({rc})")
 
     def probability_vector(self, x, y, z, sp) -> np.ndarray:
         p = (C.c_float * 7)()

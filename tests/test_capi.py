@@ -80,9 +80,9 @@ def test_device_preparation_fails_loudly_without_a_gpu():
 
 
 def test_unsupported_models_are_rejected_before_any_device_work():
-    # reservoir particles together with well-mixed species have no device form: refused before any HIP call
-    sim = Simulation_with_reservoir()
-    sim.add_cme_species("pool", 10)
+    # a stoichiometry entry beyond int8 has no device form: refused before any HIP call
+    sim = configs.spherical_cell(seed=1, edge=16, counts=(10, 10))
+    sim.add_reaction(["A"], ["B"] * 130, 1.0)
     with pytest.raises(CellSimError) as e:
         sim.prepare_for_gpu(0)
     assert e.value.code == _lib.CS_ERR_UNSUPPORTED

@@ -1,16 +1,21 @@
 """Bit-exact parity of the HIP step against the serial CPU oracle, through the C ABI.
 
 Each case builds one model with the Simulation API, hands the same initial lattice, region labels and
-tables to the oracle, runs both with the same per-step seeds and compares the integer lattice (species
-per slot) byte for byte. The HIP path also reports how many voxel-steps were "tight" (could fill up, so
-the capacity hazards of rdme.wgsl:57-87 could matter); parity is asserted where that count is zero.
+tables to the oracle, runs both with the same per-step seeds and compares the reference's whole state: the
+integer lattice (species per slot) byte for byte, the per-voxel counts, the occupancy words and the global
+counts, plus the event counters (reactions fired, near-threshold draws, rejected claims, slot-S aliases). The
+HIP path also reports how many voxel-steps were "tight" (could fill up, so the capacity hazards of
+rdme.wgsl:57-87 apply); those are replayed in canonical order, and the voxels whose counters leave the slots
+(slot-S aliases, rdme.wgsl:30-40; counts below zero, cme.wgsl:42-44) keep their offsets, so every case runs
+to its end.
 """
 import numpy as np
 import pytest
 
 from cell_simulation_amd import LatticeParams, RegionType, Simulation, configs, default_itime_schedule
-from cell_simulation_amd._lib import CS_CNT_EXCEPTION, CS_CNT_FIRED, CS_CNT_NEAR_THRESHOLD, CS_CNT_TIGHT
-from tests.conftest import announce
+from cell_simulation_amd._lib import (CS_CNT_ALIAS, CS_CNT_EXCEPTION, CS_CNT_FAILED_CLAIM, CS_CNT_FIRED,
+                                      CS_CNT_NEAR_THRESHOLD, CS_CNT_ROLLBACK, CS_CNT_TIGHT)
+from tests.conftest import announce, check_state, skewed_voxels
 
 pytestmark = pytest.mark.gpu
 
@@ -24,6 +29,17 @@ def _compare(sim, orc, tag):
         raise AssertionError(f"{tag}: {len(diff)} voxels differ; first {v}: gpu {g[v]} oracle {o[v]}")
 
 
+def check_counters(sim, orc):
+    c = sim.counters()
+    assert c[CS_CNT_EXCEPTION] == 0
+    assert c[CS_CNT_NEAR_THRESHOLD] == orc.counters.near_threshold
+    assert c[CS_CNT_FIRED] == orc.counters.fired
+    assert c[CS_CNT_ROLLBACK] == orc.counters.rollbacks
+    assert c[CS_CNT_FAILED_CLAIM] == orc.counters.failed_claims
+    assert c[CS_CNT_ALIAS] == orc.counters.alias_writes
+    return c
+
+
 def run_parity(sim, orc, itimes, check_every, tag):
     sim.prepare_for_gpu(0)
     done = 0
@@ -32,16 +48,8 @@ def run_parity(sim, orc, itimes, check_every, tag):
         sim.run(done, itimes[done:done + n])
         orc.run(itimes[done:done + n])
         done += n
-        _compare(sim, orc, f"{tag} after {done} steps")
-    # tight voxels (if any) were replayed in canonical order; only a slot-S alias is unrepresentable
-    assert orc.counters.alias_writes == 0, "choose a case without slot-S aliases for a multi-step comparison"
-    c = sim.counters()
-    assert c[CS_CNT_EXCEPTION] == 0
-    assert c[CS_CNT_NEAR_THRESHOLD] == orc.counters.near_threshold
-    assert c[CS_CNT_FIRED] == orc.counters.fired
-    counts = sim.species_counts()
-    assert np.array_equal(counts[1:], orc.stat[1:]), (counts, orc.stat)
-    return c
+        check_state(sim, orc, f"{tag} after {done} steps")
+    return check_counters(sim, orc)
 
 
 def test_spherical_cell_64(oracle_mod):
@@ -176,10 +184,9 @@ def test_region_counts_match_lattice(oracle_mod):
 
 @pytest.mark.parametrize("kernel", ["wave", "wave-overflow", "wave-summary", "tiled", "simple"])
 def test_dense_capacity_replay(oracle_mod, kernel):
-    """S=4 at high density: claims into full voxels are rejected or alias into the next voxel's slot 0
-    (rdme.wgsl:30-40, 62). The tight neighbourhoods are replayed in canonical order, so the lattice stays
-    bit-exact step by step; the comparison runs up to and including the first alias, after which the
-    reference's occupancy (S+1 with S slots) is outside the byte-slot state."""
+    """S=4 at high density: claims into full voxels are rejected (rdme.wgsl:62, 73-81) or land in the next
+    voxel's slot 0 (rdme.wgsl:30-40), after which the reference's occupancy word and counts leave the slots.
+    Step by step to the end against the oracle's whole state and event counters."""
     import os
 
     for walled, dens, nsteps in ((False, 1.4, 30), (True, 0.7, 40)):
@@ -197,43 +204,89 @@ def test_dense_capacity_replay(oracle_mod, kernel):
             os.environ.pop("CS_STEP_KERNEL", None)
             os.environ.pop("CS_WAVE_EVENT_CAP", None)
             os.environ.pop("CS_OCC_SUMMARY", None)
-        compared = 0
+        skew = 0
         for t, it in enumerate(default_itime_schedule(31, nsteps)):
             sim.step(t, int(it), 0)
             orc.step(int(it))
-            _compare(sim, orc, f"dense replay step {t}")
-            compared += 1
-            if orc.counters.alias_writes:
-                break
-        c = sim.counters()
+            check_state(sim, orc, f"dense replay step {t}")
+            skew = max(skew, skewed_voxels(orc))
+        c = check_counters(sim, orc)
         assert c[CS_CNT_TIGHT] > 0 and orc.counters.failed_claims > 0, "the case must exercise the replay"
-        if not orc.counters.alias_writes:
-            assert compared == nsteps
-            assert c[CS_CNT_EXCEPTION] == 0
-            assert c[CS_CNT_FIRED] == orc.counters.fired
-            assert c[CS_CNT_NEAR_THRESHOLD] == orc.counters.near_threshold
-        print(f"compared {compared} steps; oracle {orc.counters.as_dict()}; tight {int(c[CS_CNT_TIGHT])}")
+        if not walled:
+            assert orc.counters.alias_writes > 0 and skew > 0, "the case must leave slot-S aliases"
+        print(f"oracle {orc.counters.as_dict()}; tight {int(c[CS_CNT_TIGHT])}; skewed voxels up to {skew}")
 
 
 def test_dense_s16_full_heads(oracle_mod):
     """S=16 at about 7 particles per voxel: many voxels hold 8 or more, past the 8-slot head the wave kernel
-    stages, so their hops and gathers take the whole record from memory. Step by step against the oracle, up to
-    and including the first slot-S alias (rdme.wgsl:30-40)."""
+    stages, so their hops and gathers take the whole record from memory. Step by step to the end against the
+    oracle's whole state (slot-S aliases included)."""
     announce("dense S=16 16^3 density 7 (full 8-slot heads), step by step")
     sim = _dense_model(16, 16, 7.0, seed=17)
     orc = oracle_mod.Oracle.from_simulation(sim)
     sim.prepare_for_gpu(0)
     lat = sim.lattice()
     assert (np.count_nonzero(lat, axis=-1) >= 8).sum() > 100, "the case must hold voxels past the head"
-    compared = 0
     for t, it in enumerate(default_itime_schedule(17, 12)):
         sim.step(t, int(it), 0)
         orc.step(int(it))
-        _compare(sim, orc, f"dense S=16 step {t}")
-        compared += 1
-        if orc.counters.alias_writes:
-            break
-    print(f"compared {compared} steps; oracle {orc.counters.as_dict()}")
+        check_state(sim, orc, f"dense S=16 step {t}")
+    check_counters(sim, orc)
+    print(f"oracle {orc.counters.as_dict()}")
+
+
+def test_more_than_1024_tight_voxels_in_one_step(oracle_mod):
+    """A 32^3 S=4 lattice at 2.2 particles per voxel: thousands of tight voxels per step (the replay's list
+    and its clusters), slot-S aliases and rejected claims, against the oracle's whole state step by step."""
+    announce("32^3 S=4 density 2.2: >1024 tight voxels per step, 8 steps")
+    sim = _dense_model(32, 4, 2.2, seed=5)
+    orc = oracle_mod.Oracle.from_simulation(sim)
+    sim.prepare_for_gpu(0)
+    most = 0
+    for t, it in enumerate(default_itime_schedule(5, 8)):
+        before = int(sim.counters()[CS_CNT_TIGHT])
+        sim.step(t, int(it), 0)
+        orc.step(int(it))
+        most = max(most, int(sim.counters()[CS_CNT_TIGHT]) - before)
+        check_state(sim, orc, f"32^3 dense step {t}")
+    check_counters(sim, orc)
+    assert most > 1024, most
+    assert orc.counters.alias_writes > 0
+    print(f"most tight voxels in a step: {most}; oracle {orc.counters.as_dict()}")
+
+
+def test_homodimer_negative_counts(oracle_mod):
+    """k*n^2 of a homodimer row (cme.wgsl:42-44) fires on a single particle and takes its count to -1
+    (cme.wgsl:85-91); the negative count stays in C[v][sp], enters later propensities (a negative product) and
+    the occupancy word drops below the slot count. The HIP path keeps both as offsets of the voxel."""
+    announce("homodimer A + A -> B on single particles, 24^3 S=8, 80 steps")
+    lam = 32e-9
+    edge = 24
+    sim = Simulation(LatticeParams([edge * lam] * 3, [edge] * 3, 50e-6, lam, 8))
+    sim.set_seed(44)
+    sim.prepare_regions()
+    sim.add_particle_count("A", "background", 3000, True, False)
+    sim.add_particle_count("B", "background", 500, True, False)
+    sim.add_particle_count("C", "background", 200, True, False)
+    sim.set_diffusion_rate_particle("A", "background", 2e-12)
+    sim.set_diffusion_rate_particle("B", "background", 1e-12)
+    sim.set_diffusion_rate_particle("C", "background", 3e-12)
+    sim.add_reaction(["A", "A"], ["B"], 900.0)
+    sim.add_reaction(["B"], ["A", "C"], 300.0)
+    sim.add_reaction(["A", "C"], ["C", "C"], 400.0)
+    sim.add_reaction(["C", "C"], ["A"], 200.0)
+    orc = oracle_mod.Oracle.from_simulation(sim)
+    sim.prepare_for_gpu(0)
+    neg = 0
+    for t, it in enumerate(default_itime_schedule(44, 80)):
+        sim.step(t, int(it), 0)
+        orc.step(int(it))
+        check_state(sim, orc, f"homodimer step {t}")
+        neg = max(neg, int(np.count_nonzero(orc.counts_array()[:, 1:] < 0)))
+    c = check_counters(sim, orc)
+    assert neg > 10, "the case must take counts below zero"
+    assert c[CS_CNT_FIRED] > 0
+    print(f"negative per-voxel counts: up to {neg}; oracle {orc.counters.as_dict()}")
 
 
 def _walled_model(edge, slots, density, seed):

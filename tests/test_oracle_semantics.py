@@ -361,3 +361,45 @@ def test_view_follows_the_occupancy_word(oracle_mod):
     o.step(100)
     assert o.occupancy[1] == 2 and list(o.lattice[4:8]) == [3, 0, 0, 0]
     assert o.view(0)[1] == np.float32(3) / np.float32(255)
+
+
+def _run_pair(oracle_mod, sim, seed, nsteps, threads):
+    """The serial oracle and its multi-threaded driver from the same state: identical whole state and counters."""
+    from cell_simulation_amd import default_itime_schedule
+
+    a = oracle_mod.Oracle.from_simulation(sim)
+    b = oracle_mod.Oracle.from_simulation(sim)
+    its = default_itime_schedule(seed, nsteps)
+    a.run(its)
+    b.run_mt(its, threads)
+    assert np.array_equal(a.lattice, b.lattice)
+    assert np.array_equal(a.occupancy, b.occupancy)
+    assert np.array_equal(a.counts, b.counts)
+    assert np.array_equal(a.stat, b.stat)
+    assert a.counters.as_dict() == b.counters.as_dict()
+    return a
+
+
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_threaded_oracle_matches_serial(oracle_mod, threads):
+    """cso_run_mt (hop draws, reactions and compaction on several threads, claims serial) gives the serial
+    oracle's state bit for bit: a dense S=4 lattice with rejected claims and slot-S aliases, a reservoir model
+    and a homodimer row that takes counts below zero."""
+    from cell_simulation_amd import LatticeParams, RegionType, Simulation
+
+    lam = 32e-9
+    sim = Simulation(LatticeParams([14 * lam] * 3, [14] * 3, 50e-6, lam, 4))
+    sim.set_seed(3)
+    sim.add_region(RegionType.Cube("left", [0, 0, 0], [7 * lam, 14 * lam, 14 * lam]), 2e-12)
+    sim.prepare_regions()
+    sim.add_particle_count("A", "background", 1200, True, False)
+    sim.add_particle_count("B", "left", 1200, True, False)
+    sim.add_particle_count("R", "left", 60, True, True)
+    sim.set_diffusion_rate_particle("A", "background", 3e-12)
+    sim.set_transition_rate_particle("A", "background", "left", 1e-12)
+    sim.add_reaction(["A", "B"], ["B", "B"], 60.0)
+    sim.add_reaction(["A", "A"], ["B"], 400.0)
+    sim.add_reaction(["R"], ["R", "A"], 900.0)
+    a = _run_pair(oracle_mod, sim, 3, 25, threads)
+    assert a.counters.alias_writes > 0 and a.counters.failed_claims > 0 and a.counters.fired > 0
+    assert np.any(a.counts_array()[:, 1:] < 0)
