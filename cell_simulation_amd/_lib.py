@@ -130,6 +130,7 @@ SIGNATURES = {
     "cs_sim_read_occupancy": (C.c_int, [_SIM, C.c_void_p, C.c_size_t]),
     "cs_sim_render_view": (C.c_int, [_SIM, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_size_t]),
     "cs_sim_read_region_counts": (C.c_int, [_SIM, C.c_void_p, C.c_size_t]),
+    "cs_sim_read_region_counts_noalloc": (C.c_int, [_SIM, C.c_void_p, C.c_size_t]),
     "cs_sim_read_tables": (C.c_int, [_SIM, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_void_p,
                                      C.c_size_t, C.c_void_p, C.c_size_t]),
     "cs_sim_pop_stat": (C.c_int, [_SIM, C.POINTER(StatSampleC)]),

@@ -731,6 +731,20 @@ int cs_sim_read_region_counts(cs_sim *sim, int32_t *out, size_t n)
     });
 }
 
+int cs_sim_read_region_counts_noalloc(cs_sim *sim, int32_t *out, size_t n)
+{
+    return guard([&] {
+        need(out, "out");
+        cs::Model &m = model_of(sim);
+        const uint32_t n1 = m.num_species() + 1u, nreg = m.num_regions();
+        if (n != size_t(nreg) * n1)
+            throw cs::Error(CS_ERR_ARG, "region count buffer must hold nreg*(ns+1) entries");
+        if (!sim->device)
+            throw cs::Error(CS_ERR_STATE, "no device state: call cs_sim_prepare_device first");
+        sim->device->region_counts_noalloc(out);
+    });
+}
+
 int cs_sim_read_tables(const cs_sim *sim, float *diffusion, size_t n_diffusion, int32_t *stoich, size_t n_stoich,
                        int32_t *reactants, size_t n_reactants, float *rates, size_t n_rates)
 {

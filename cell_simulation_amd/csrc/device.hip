@@ -31,6 +31,14 @@
 #include "device.hpp"
 #include "ensemble.hpp"
 
+#include <mutex>
+
+// Bins of cs_sim_read_region_counts_noalloc's histogram: one fixed array per GPU, so that entry point allocates
+// nothing. Host code passes the address hipGetSymbolAddress gives for the current device (noalloc_bins), never
+// this name.
+constexpr uint32_t kNoAllocBins = 16384;
+static __device__ __align__(256) int32_t scratch[kNoAllocBins];
+
 namespace cs {
 
 namespace {
@@ -3033,6 +3041,28 @@ __global__ __launch_bounds__(kBlock) void region_count_kernel(const uint8_t *lat
 }
 
 
+__global__ __launch_bounds__(kBlock) void fill_i32_kernel(int32_t *p, uint32_t n, int32_t v)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n)
+        p[i] = v;
+}
+
+// Adds the count offsets of the skewed voxels (what their counts hold beyond their slots) to a region x species
+// histogram.
+__global__ __launch_bounds__(kBlock) void skew_region_kernel(SkewTab k, const uint8_t *reg, uint32_t ns, int32_t *hist)
+{
+    const uint32_t ne = k.ctl[0];
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < ne; i += gridDim.x * kBlock) {
+        const SkewEntry e = k.ent[i];
+        for (int q = 0; q < kSkewSp; ++q) {
+            const uint32_t sp = (e.sp >> (8 * q)) & 0xFFu;
+            if (sp != 0u && sp <= ns && e.dc[q] != 0)
+                atomicAdd(hist + uint32_t(reg[e.v]) * (ns + 1u) + sp, e.dc[q]);
+        }
+    }
+}
+
 // ---- batched ensembles (see ensemble.hpp) ------------------------------------------------------------
 struct EnsCell {
     uint64_t lat_off;          // byte offset of the cell's slots in each lattice buffer
@@ -3669,6 +3699,26 @@ static uint64_t count_reservoir_voxels(const Model &m)
     return n;
 }
 
+// Each device's copy of `scratch`, looked up once per device with that device current.
+static int32_t *noalloc_bins(int dev)
+{
+    static std::mutex mu;
+    static void *addr[256] = {};
+    if (dev < 0 || dev >= 256)
+        throw Error(CS_ERR_ARG, "HIP device index out of range");
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        int cur = -1;
+        hip_check(hipGetDevice(&cur), "hipGetDevice");
+        if (cur != dev)
+            throw Error(CS_ERR_STATE, "the fixed histogram is looked up with its device current");
+        void *p = nullptr;
+        hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        addr[dev] = p;
+    }
+    return static_cast<int32_t *>(addr[dev]);
+}
+
 Device::Device(const Model &m, int hip_device) : dev_(hip_device)
 {
     const LatticeParams &lp = m.params();
@@ -3690,6 +3740,7 @@ Device::Device(const Model &m, int hip_device) : dev_(hip_device)
     hipStream_t st;
     hip_check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
     stream_ = st;
+    (void)noalloc_bins(dev_);  // this device's fixed histogram, for region_counts_noalloc
 
     const uint32_t n1 = ns_ + 1u;
     const TableBlob blob = make_table_blob(m);
@@ -4281,6 +4332,30 @@ void Device::region_counts(int32_t *out)
         for (int q = 0; q < 4; ++q)
             if (e.sp[q] != 0u && e.sp[q] <= ns_)
                 out[size_t(reg_host_[e.v]) * (ns_ + 1u) + e.sp[q]] += e.dc[q];
+}
+
+void Device::region_counts_noalloc(int32_t *out)
+{
+    const uint32_t n1 = ns_ + 1u, nbins = nreg_ * n1;
+    if (nbins > kNoAllocBins)
+        throw Error(CS_ERR_ARG, "regions x (species + 1) exceeds the " + std::to_string(kNoAllocBins) +
+                                    " bins of the fixed histogram");
+    hip_check(hipSetDevice(dev_), "hipSetDevice");
+    hipStream_t st = static_cast<hipStream_t>(stream_);
+    int32_t *bins = noalloc_bins(dev_);
+    // the array keeps the previous call's bins: cleared by a kernel (no memset node for graph capture)
+    hipLaunchKernelGGL(fill_i32_kernel, dim3((nbins + kBlock - 1) / kBlock), dim3(kBlock), 0, st, bins, nbins, 0);
+    const uint32_t N = uint32_t(N_);
+    const size_t shmem = nbins * sizeof(int32_t);
+    switch (S_) {
+    case 4: hipLaunchKernelGGL(region_count_kernel<4>, dim3(1024), dim3(kBlock), shmem, st, lat_[cur_], reg_, N, ns_, nreg_, bins); break;
+    case 8: hipLaunchKernelGGL(region_count_kernel<8>, dim3(1024), dim3(kBlock), shmem, st, lat_[cur_], reg_, N, ns_, nreg_, bins); break;
+    default: hipLaunchKernelGGL(region_count_kernel<16>, dim3(1024), dim3(kBlock), shmem, st, lat_[cur_], reg_, N, ns_, nreg_, bins); break;
+    }
+    hipLaunchKernelGGL(skew_region_kernel, dim3(64), dim3(kBlock), 0, st, skew_->k, reg_, ns_, bins);
+    hip_check(hipGetLastError(), "region count launches");
+    hip_check(hipMemcpyAsync(out, bins, nbins * sizeof(int32_t), hipMemcpyDeviceToHost, st), "read bins");
+    hip_check(hipStreamSynchronize(st), "region counts");
 }
 
 void Device::read_counters(uint64_t *out)

@@ -44,6 +44,9 @@ public:
     void write_lattice(const uint8_t *in);
     // Species counts per region, int32[nreg][ns+1], reduced on the device.
     void region_counts(int32_t *out);
+    // The same counts with no allocation: the bins live in the library's fixed per-GPU array (calls that overlap
+    // on one GPU race); more than 16384 bins are refused.
+    void region_counts_noalloc(int32_t *out);
     void read_counters(uint64_t *out);
     void reset_counters();
 

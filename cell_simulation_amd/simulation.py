@@ -401,6 +401,14 @@ class Simulation:
         check(lib().cs_sim_read_region_counts(self._h, out.ctypes.data, out.size))
         return out.reshape(nreg, n1)
 
+    def region_counts_noalloc(self) -> np.ndarray:
+        """region_counts() through the entry point that allocates nothing (its bins are a fixed per-GPU array)."""
+        inf = self.info()
+        n1, nreg = inf["num_species"] + 1, inf["num_regions"]
+        out = np.empty(nreg * n1, dtype=np.int32)
+        check(lib().cs_sim_read_region_counts_noalloc(self._h, out.ctypes.data, out.size))
+        return out.reshape(nreg, n1)
+
     def tables(self) -> dict:
         inf = self.info()
         n1, nr1, nreg = inf["num_species"] + 1, inf["num_reactions"] + 1, inf["num_regions"]

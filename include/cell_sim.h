@@ -192,6 +192,12 @@ int cs_sim_read_species_counts(cs_sim *sim, int32_t *out, size_t n);
 /* Per-region species counts, int32[nreg][ns+1] (no reference counterpart: the reference keeps only
  * global counts). */
 int cs_sim_read_region_counts(cs_sim *sim, int32_t *out, size_t n);
+/* cs_sim_read_region_counts without allocating anything. No counterpart in the reference: for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). The
+ * histogram's bins live in one fixed device array of the library, 16384 int32 per GPU, held until the process
+ * ends; a model with more than 16384 regions x (species + 1) bins gets CS_ERR_ARG. Every call on one GPU uses
+ * that GPU's array, so calls that overlap on one GPU race. */
+int cs_sim_read_region_counts_noalloc(cs_sim *sim, int32_t *out, size_t n);
 
 /* Model tables in the reference's device layout: diffusion f32[nreg][nreg][ns+1] (src,dst,species),
  * stoichiometry i32[nr+1][ns+1], reactant indices i32[nr+1][3], rates f32[nr+1]. */
