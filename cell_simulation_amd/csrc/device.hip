@@ -959,7 +959,13 @@ struct ReplayBuf {
     uint32_t *ctl;      // [0] voxels noted this step, [2] F pool used, [kFixQueue..] the wave kernel's queues
     uint32_t *T;        // [tcap] noted voxels: by the step kernels (note_tight), then the skewed ones; sorted here
     uint32_t tcap;      // power of two
-    uint4 *clus;        // [tcap] per cluster: first noted voxel, count, cell, lowest plane
+    uint4 *xclus;       // [ccap] runs of noted voxels along x: first, count, cell, lowest plane
+    uint4 *clus;        // [ccap] the clusters replayed: first noted voxel, count, cell, lowest plane
+    uint32_t ccap;
+    uint32_t *mark_lo;  // bit per voxel, and
+    uint32_t *mark_hi;  // bit per 1024 voxels: a list longer than lds_sort_max is sorted through them (left clear)
+    uint32_t nvox;
+    uint32_t lds_sort_max;  // longest list sorted in LDS (kSortLds; lowered by a test hook)
     uint32_t *F;        // [fcap] the voxels each cluster replays, ascending
     uint8_t *FL;        // [fcap * S] their slots during the replay
     uint32_t *FO;       // [fcap] their occupancy words during the replay
@@ -1239,31 +1245,208 @@ __device__ void block_bitonic(uint32_t *arr, uint32_t n, uint32_t n2)
         }
 }
 
+// A skewed voxel whose replay can be left out: no particle, a walk that stays in its own (empty) slots, and no
+// particle of a neighbour hopping into it (so no claim meets its occupancy word). Its gathered output (empty) is
+// exact, its neighbours receive nothing from it, and its reaction runs with its offsets in skew_react.
+template <int S>
+__device__ bool skew_inert(const StepArgs &a, const SkewEntry &e)
+{
+    const uint32_t v = e.v - a.vox_base;
+    if (count_nonzero(load_slots<S>(a.src, v)) != 0u)
+        return false;
+    const uint32_t start = v * uint32_t(S), end = start + uint32_t(e.off);
+    if ((end > start ? end - start : 0u) > uint32_t(S))
+        return false;
+    const Tab tb = global_tab(a);
+    for (int k = 0; k < 6; ++k) {
+        const uint32_t u = neighbor_or_none(a, v, k);
+        if (u == 0xFFFFFFFFu)
+            continue;
+        const Slots<S> su = load_slots<S>(a.src, u);
+        const uint32_t occ = count_nonzero(su);
+        if (occ == 0u)
+            continue;
+        const uint32_t z = u % a.R2, y = (u / a.R2) % a.R1, x = u / a.R12;
+        const Hood h = make_hood(a, x, y, z, u, S);
+        const uint32_t back = uint32_t(k ^ 1);  // the direction from u to v
+        for (uint32_t j = 0; j < occ; ++j)
+            if (hop(a, tb, h, get_byte_dyn(su, j), j) == back)
+                return false;
+    }
+    return true;
+}
+
+// Ascending, duplicate-free list of the noted voxels from the bitmaps: T[0..n) marked, then read back in order.
+// Returns the list's length.
+__device__ uint32_t bitmap_sort(const ReplayBuf &b, uint32_t n)
+{
+    __shared__ uint32_t s_part[kReplayThreads + 1];
+    const uint32_t tid = threadIdx.x, nth = blockDim.x;
+    const uint32_t n_hi = (b.nvox + 1023u) / 1024u, hw = (n_hi + 31u) / 32u;
+    for (uint32_t i = tid; i < n; i += nth) {
+        const uint32_t v = b.T[i];
+        atomicOr(b.mark_lo + (v >> 5), 1u << (v & 31u));
+        atomicOr(b.mark_hi + (v >> 15), 1u << ((v >> 10) & 31u));
+    }
+    __syncthreads();
+    // voxels under each word of mark_hi, then its first position: contiguous words per thread, block prefix
+    const uint32_t per = (hw + nth - 1u) / nth, w0 = min(tid * per, hw), w1 = min(w0 + per, hw);
+    uint32_t mine = 0;
+    for (uint32_t w = w0; w < w1; ++w) {
+        uint32_t c = 0, hb = b.mark_hi[w];
+        while (hb) {
+            const uint32_t bit = uint32_t(__ffs(hb) - 1);
+            hb &= hb - 1u;
+            const uint32_t lo0 = (w * 32u + bit) * 32u;
+            for (uint32_t q = 0; q < 32u; ++q)
+                c += uint32_t(__popc(b.mark_lo[lo0 + q]));
+        }
+        mine += c;
+    }
+    s_part[tid] = mine;
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t acc = 0;
+        for (uint32_t t = 0; t < nth; ++t) {
+            const uint32_t c = s_part[t];
+            s_part[t] = acc;
+            acc += c;
+        }
+        s_part[nth] = acc;
+    }
+    __syncthreads();
+    uint32_t pos = s_part[tid];
+    for (uint32_t w = w0; w < w1; ++w) {
+        uint32_t hb = b.mark_hi[w];
+        while (hb) {
+            const uint32_t bit = uint32_t(__ffs(hb) - 1);
+            hb &= hb - 1u;
+            const uint32_t lo0 = (w * 32u + bit) * 32u;
+            for (uint32_t q = 0; q < 32u; ++q) {
+                uint32_t lb = b.mark_lo[lo0 + q];
+                if (!lb)
+                    continue;
+                b.mark_lo[lo0 + q] = 0u;
+                while (lb) {
+                    const uint32_t t = uint32_t(__ffs(lb) - 1);
+                    lb &= lb - 1u;
+                    b.T[pos++] = (lo0 + q) * 32u + t;
+                }
+            }
+        }
+        b.mark_hi[w] = 0u;
+    }
+    __syncthreads();
+    return s_part[nth];
+}
+
+// Splits the run of noted voxels T[r.x .. r.x + r.y) of one cell (ascending, all within kClusterGap planes of the
+// next) where no two voxels kClusterGap apart along y, then along z, belong to one piece; appends each piece as a
+// cluster, its voxels ascending. Runs longer than 64 stay whole (a crowded run is one cluster anyway).
+__device__ void split_run(const StepArgs &a, const ReplayBuf &b, uint4 r, uint32_t *n_clus)
+{
+    uint32_t *T = b.T + r.x;
+    const uint32_t m = r.y;
+    auto emit = [&](uint32_t first, uint32_t count, uint32_t xlo) {
+        const uint32_t c = atomicAdd(n_clus, 1u);
+        if (c < b.ccap)
+            b.clus[c] = make_uint4(r.x + first, count, r.z, xlo);
+    };
+    if (m > 64u) {
+        emit(0u, m, r.w);
+        return;
+    }
+    uint32_t key[64], idx[64];
+    // order by (y, v)
+    for (uint32_t i = 0; i < m; ++i) {
+        const uint32_t y = (T[i] / a.R2) % a.R1;
+        uint32_t q = i;
+        while (q > 0u && key[q - 1u] > y) {
+            key[q] = key[q - 1u];
+            idx[q] = idx[q - 1u];
+            --q;
+        }
+        key[q] = y;
+        idx[q] = T[i];
+    }
+    uint32_t out[64];
+    uint32_t n_out = 0;
+    uint32_t g0 = 0;
+    while (g0 < m) {
+        uint32_t g1 = g0 + 1u;
+        while (g1 < m && key[g1] - key[g1 - 1u] < kClusterGap)
+            ++g1;
+        // the y-group idx[g0..g1): order by (z, v), split along z
+        uint32_t zk[64], zv[64];
+        const uint32_t gm = g1 - g0;
+        for (uint32_t i = 0; i < gm; ++i) {
+            const uint32_t v = idx[g0 + i], z = v % a.R2;
+            uint32_t q = i;
+            while (q > 0u && (zk[q - 1u] > z || (zk[q - 1u] == z && zv[q - 1u] > v))) {
+                zk[q] = zk[q - 1u];
+                zv[q] = zv[q - 1u];
+                --q;
+            }
+            zk[q] = z;
+            zv[q] = v;
+        }
+        uint32_t h0 = 0;
+        while (h0 < gm) {
+            uint32_t h1 = h0 + 1u;
+            while (h1 < gm && zk[h1] - zk[h1 - 1u] < kClusterGap)
+                ++h1;
+            // a piece: its voxels ascending
+            const uint32_t first = n_out;
+            uint32_t xlo = 0xFFFFFFFFu;
+            for (uint32_t i = h0; i < h1; ++i) {
+                const uint32_t v = zv[i];
+                uint32_t q = n_out++;
+                while (q > first && out[q - 1u] > v) {
+                    out[q] = out[q - 1u];
+                    --q;
+                }
+                out[q] = v;
+                xlo = min(xlo, v / a.R12);
+            }
+            emit(first, h1 - h0, xlo);
+            h0 = h1;
+        }
+        g0 = g1;
+    }
+    for (uint32_t i = 0; i < m; ++i)
+        T[i] = out[i];
+}
+
 // The replay kernel's work for one step (see the section comment). `cells` maps a voxel index to the cell that
 // holds it: find(v) -> cell, args(cell) -> its StepArgs (vox_base, extents, tables, dst).
 template <int S, class Cells>
 __device__ void replay_step(const Cells &cells, const SkewTab &k, const ReplayBuf &b, unsigned long long *counters)
 {
-    __shared__ uint32_t s_n, s_ncl;
+    __shared__ uint32_t s_n, s_nx, s_ncl;
     __shared__ uint32_t s_sort[kSortLds];
     __shared__ unsigned long long s_tally[6];  // fired, rolled back, near, exceptions, failed claims, aliases
     const uint32_t tid = threadIdx.x, nth = blockDim.x;
     const uint32_t ne = k.ctl[0];
-    if (tid == 0)
+    if (tid == 0) {
         s_n = b.ctl[0];
+        s_ncl = 0u;
+    }
     if (tid < 6u)
         s_tally[tid] = 0ull;
     __syncthreads();
     ReplayTally tally;
-    // every voxel whose occupancy word is off its slot count: its walk and its claims differ from the gather
-    for (uint32_t i = tid; i < ne; i += nth)
-        if (k.ent[i].off != 0) {
+    // every voxel whose occupancy word is off its slot count (its walk and the claims into it differ from the
+    // gather), unless nothing reaches it this step
+    for (uint32_t i = tid; i < ne; i += nth) {
+        const SkewEntry e = k.ent[i];
+        if (e.off != 0 && !skew_inert<S>(cells.args(cells.find(e.v)), e)) {
             const uint32_t j = atomicAdd(&s_n, 1u);
             if (j < b.tcap)
-                b.T[j] = k.ent[i].v;
+                b.T[j] = e.v;
         }
+    }
     __syncthreads();
-    const uint32_t n = s_n;
+    uint32_t n = s_n;
     if (n > b.tcap) {
         if (tid == 0)
             tally.bad += n;  // more voxels than the list holds: none replayed
@@ -1272,22 +1455,24 @@ __device__ void replay_step(const Cells &cells, const SkewTab &k, const ReplayBu
         while (n2 < n)
             n2 <<= 1;
         uint32_t *arr = b.T;
-        if (n2 <= kSortLds) {
+        if (n2 <= b.lds_sort_max) {
             for (uint32_t i = tid; i < n; i += nth)
                 s_sort[i] = b.T[i];
             arr = s_sort;
+            block_bitonic(arr, n, n2);
+        } else {
+            n = bitmap_sort(b, n);  // ascending and duplicate-free
         }
-        block_bitonic(arr, n, n2);
         if (tid == 0) {
-            // clusters: runs of one cell whose consecutive noted voxels are fewer than kClusterGap planes apart
-            uint32_t m = 0, ncl = 0, prev = 0xFFFFFFFFu, cell = 0, cbase = 0, cend = 0, last_x = 0;
+            // runs of one cell whose consecutive noted voxels are fewer than kClusterGap planes apart
+            uint32_t m = 0, nx = 0, prev = 0xFFFFFFFFu, cell = 0, cbase = 0, cend = 0, last_x = 0;
             uint32_t R12 = 1;
             for (uint32_t i = 0; i < n; ++i) {
                 const uint32_t v = arr[i];
                 if (v == prev)
                     continue;
                 prev = v;
-                bool fresh = ncl == 0u;
+                bool fresh = nx == 0u;
                 if (fresh || v >= cend) {
                     cell = cells.find(v);
                     const StepArgs ca = cells.args(cell);
@@ -1299,17 +1484,28 @@ __device__ void replay_step(const Cells &cells, const SkewTab &k, const ReplayBu
                 const uint32_t local = v - cbase, x = local / R12;
                 if (!fresh && x - last_x >= kClusterGap)
                     fresh = true;
-                if (fresh)
-                    b.clus[ncl++] = make_uint4(m, 0u, cell, x);
+                if (fresh) {
+                    if (nx == b.ccap)
+                        break;  // more runs than clusters fit: cannot happen (runs are kClusterGap planes apart)
+                    b.xclus[nx++] = make_uint4(m, 0u, cell, x);
+                }
                 b.T[m++] = local;
-                b.clus[ncl - 1u].y += 1u;
+                b.xclus[nx - 1u].y += 1u;
                 last_x = x;
             }
-            s_ncl = ncl;
+            s_nx = nx;
             b.ctl[2] = 0u;
         }
         __syncthreads();
-        for (uint32_t c = tid; c < s_ncl; c += nth) {
+        for (uint32_t c = tid; c < s_nx; c += nth) {
+            const uint4 r = b.xclus[c];
+            split_run(cells.args(r.z), b, r, &s_ncl);
+        }
+        __syncthreads();
+        const uint32_t ncl = min(s_ncl, b.ccap);
+        if (tid == 0 && s_ncl > b.ccap)
+            tally.bad += s_ncl - b.ccap;
+        for (uint32_t c = tid; c < ncl; c += nth) {
             const uint4 cl = b.clus[c];
             if (!replay_cluster<S>(cells.args(cl.z), k, b, b.T + cl.x, cl.y, cl.w, tally))
                 tally.bad += cl.y;
@@ -3569,9 +3765,10 @@ struct SkewStore {
 
     // Sizes the table for every voxel of a state of up to 2^20 voxels (a voxel keeps its entry once it has been
     // skewed), 2^20 entries above that (at least twice the reservoir voxels), and the replay scratch: up to tcap
-    // noted voxels per step (every voxel twice, at most 65536), and
-    // F sets for `max_clusters` clusters (a cluster's F is disjoint from every other's, so all of them hold at
-    // most every voxel once, plus kEscMax voxels each for the voxels they may add).
+    // noted voxels per step (every voxel twice, at most 2^18), at most `max_clusters` clusters (any two are
+    // kClusterGap voxels apart along some axis, so no box of kClusterGap^3 voxels holds the first voxels of two),
+    // and their F sets (a cluster's F is disjoint from every other's, so all of them hold at most every voxel
+    // once, plus kEscMax voxels each for the voxels they may add).
     void alloc(uint64_t nvox_, uint32_t S, uint64_t n_static, uint32_t *fix_ctl, uint64_t max_clusters)
     {
         nvox = nvox_;
@@ -3590,10 +3787,21 @@ struct SkewStore {
         hip_check(hipMalloc(&k.ctl, 4 * sizeof(uint32_t)), "hipMalloc skew control");
         hip_check(hipMalloc(&k.live, std::max<size_t>(live0.size(), 1) * sizeof(uint32_t)), "hipMalloc skew bits");
         b.ctl = fix_ctl;
-        b.tcap = pow2_at_least(std::max<uint64_t>(std::min<uint64_t>(2 * nvox + 64, 65536), 1024));
-        b.fcap = uint32_t(std::min<uint64_t>(12ull * b.tcap, nvox) + (kEscMax + 8) * (max_clusters + 1));
+        b.tcap = pow2_at_least(std::max<uint64_t>(std::min<uint64_t>(2 * nvox + 64, uint64_t(1) << 18), 1024));
+        b.ccap = uint32_t(std::min<uint64_t>(max_clusters + 1, b.tcap));
+        b.fcap = uint32_t(std::min<uint64_t>(12ull * b.tcap, nvox) + (kEscMax + 8) * uint64_t(b.ccap));
+        b.nvox = uint32_t(nvox);
+        b.lds_sort_max = kSortLds;
+        if (const char *x = std::getenv("CS_REPLAY_LDS_SORT"))  // test hook: longer lists take the bitmap sort
+            b.lds_sort_max = std::min<uint32_t>(uint32_t(std::strtoul(x, nullptr, 10)), kSortLds);
+        const size_t lo_words = (nvox + 31u) / 32u, hi_words = ((nvox + 1023u) / 1024u + 31u) / 32u;
         hip_check(hipMalloc(&b.T, size_t(b.tcap) * sizeof(uint32_t)), "hipMalloc replay list");
-        hip_check(hipMalloc(&b.clus, size_t(max_clusters + 1) * sizeof(uint4)), "hipMalloc replay clusters");
+        hip_check(hipMalloc(&b.xclus, size_t(b.ccap) * sizeof(uint4)), "hipMalloc replay runs");
+        hip_check(hipMalloc(&b.clus, size_t(b.ccap) * sizeof(uint4)), "hipMalloc replay clusters");
+        hip_check(hipMalloc(&b.mark_lo, (lo_words + 32u) * sizeof(uint32_t)), "hipMalloc replay marks");
+        hip_check(hipMalloc(&b.mark_hi, hi_words * sizeof(uint32_t)), "hipMalloc replay marks");
+        hip_check(hipMemset(b.mark_lo, 0, (lo_words + 32u) * sizeof(uint32_t)), "clear replay marks");
+        hip_check(hipMemset(b.mark_hi, 0, hi_words * sizeof(uint32_t)), "clear replay marks");
         hip_check(hipMalloc(&b.F, size_t(b.fcap) * sizeof(uint32_t)), "hipMalloc replay F");
         hip_check(hipMalloc(&b.FL, size_t(b.fcap) * S), "hipMalloc replay slots");
         hip_check(hipMalloc(&b.FO, size_t(b.fcap) * sizeof(uint32_t)), "hipMalloc replay occupancy");
@@ -3658,7 +3866,8 @@ struct SkewStore {
 
     void release()
     {
-        void *bufs[] = {k.keys, k.idx, k.ent, k.ctl, k.live, b.T, b.clus, b.F, b.FL, b.FO, b.FW, b.FS, b.FD};
+        void *bufs[] = {k.keys, k.idx, k.ent, k.ctl, k.live, b.T, b.xclus, b.clus, b.mark_lo, b.mark_hi,
+                        b.F, b.FL, b.FO, b.FW, b.FS, b.FD};
         for (void *p : bufs)
             if (p)
                 (void)hipFree(p);
@@ -3666,6 +3875,15 @@ struct SkewStore {
         b = ReplayBuf{};
     }
 };
+
+// Clusters a lattice can hold at once: one per box of kClusterGap^3 voxels.
+static uint64_t cluster_bound(const uint32_t res[3])
+{
+    uint64_t n = 1;
+    for (int i = 0; i < 3; ++i)
+        n *= (res[i] + kClusterGap - 1u) / kClusterGap;
+    return n;
+}
 
 // The reservoir voxels of a model, as static entries with voxel indices offset by `base`: sites of a reservoir
 // species R (one unit of C[v][R] in no slot) and voxels v whose reservoirs[v*S] names a species K (cme.wgsl:80
@@ -3838,7 +4056,7 @@ Device::Device(const Model &m, int hip_device) : dev_(hip_device)
     hip_check(hipMemsetAsync(fix_ctl_, 0, kFixCtl * sizeof(uint32_t), st), "clear fix_ctl");
     // skewed voxels: the reservoir voxels to start with (their table and bits uploaded below)
     skew_ = new SkewStore();
-    skew_->alloc(N_, S_, count_reservoir_voxels(m), fix_ctl_, res_[0] / kClusterGap + 1u);
+    skew_->alloc(N_, S_, count_reservoir_voxels(m), fix_ctl_, cluster_bound(res_));
     add_reservoir_entries(*skew_, m, 0u);
     reg_host_ = reg8;
 
@@ -4525,7 +4743,7 @@ Ensemble::Ensemble(const std::vector<const Model *> &cells, const std::vector<ui
         uint64_t n_static = 0, max_clusters = 0;
         for (const Model *m : cells) {
             n_static += count_reservoir_voxels(*m);
-            max_clusters += m->params().res[0] / kClusterGap + 1u;
+            max_clusters += cluster_bound(m->params().res);
         }
         skew_ = new SkewStore();
         skew_->alloc(total_vox_, S_, n_static, fix_ctl_, max_clusters);

@@ -182,7 +182,7 @@ def test_region_counts_match_lattice(oracle_mod):
     assert np.array_equal(sim.species_counts()[1:], exp.sum(axis=0)[1:])
 
 
-@pytest.mark.parametrize("kernel", ["wave", "wave-overflow", "wave-summary", "tiled", "simple"])
+@pytest.mark.parametrize("kernel", ["wave", "wave-overflow", "wave-summary", "wave-bitmapsort", "tiled", "simple"])
 def test_dense_capacity_replay(oracle_mod, kernel):
     """S=4 at high density: claims into full voxels are rejected (rdme.wgsl:62, 73-81) or land in the next
     voxel's slot 0 (rdme.wgsl:30-40), after which the reference's occupancy word and counts leave the slots.
@@ -196,6 +196,8 @@ def test_dense_capacity_replay(oracle_mod, kernel):
             os.environ["CS_WAVE_EVENT_CAP"] = "2"  # most planes overflow and take the global form
         if kernel == "wave-summary":
             os.environ["CS_OCC_SUMMARY"] = "on"  # the replay's outputs must mark the occupancy summary
+        if kernel == "wave-bitmapsort":
+            os.environ["CS_REPLAY_LDS_SORT"] = "1"  # every noted list sorted through the replay's bitmaps
         try:
             sim = _walled_model(20, 4, dens, seed=31) if walled else _dense_model(20, 4, dens, seed=31)
             orc = oracle_mod.Oracle.from_simulation(sim)
@@ -204,6 +206,7 @@ def test_dense_capacity_replay(oracle_mod, kernel):
             os.environ.pop("CS_STEP_KERNEL", None)
             os.environ.pop("CS_WAVE_EVENT_CAP", None)
             os.environ.pop("CS_OCC_SUMMARY", None)
+            os.environ.pop("CS_REPLAY_LDS_SORT", None)
         skew = 0
         for t, it in enumerate(default_itime_schedule(31, nsteps)):
             sim.step(t, int(it), 0)
