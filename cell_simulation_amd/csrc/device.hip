@@ -104,6 +104,14 @@ __device__ __forceinline__ bool voxel_bit(const uint32_t *bits, uint32_t v)
     return (bits[v >> 5] >> (v & 31u)) & 1u;
 }
 
+__device__ __forceinline__ void wave_sync()
+{
+    // LDS accesses of one wave complete in order; this keeps the compiler from moving them across.
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // ---- slot words: S one-byte slots packed little-endian into S/4 u32 ----------------------------
 template <int S>
 struct Slots {
@@ -1182,6 +1190,347 @@ __device__ bool replay_cluster(const StepArgs &a, const SkewTab &k, const Replay
     return true;
 }
 
+// The same replay with the cluster's arrays in the LDS of one wave, for clusters of at most kWaveReplayT noted
+// voxels (F then holds at most 8 * kWaveReplayT + kEscMax voxels). The lanes load F's state, list the sources W
+// and compute their hops side by side, keeping each move with an end in F as an event; the events are sorted
+// into the canonical order (source ascending, walk position ascending) and one lane applies them; the lanes then
+// run the reactions and write F's outputs. The one-lane form replays a cluster whose events overflow the list.
+constexpr uint32_t kWaveReplayT = 14;
+constexpr uint32_t kWaveReplayF = 8u * kWaveReplayT + uint32_t(kEscMax);
+constexpr uint32_t kWaveReplayW = 7u * kWaveReplayF;
+constexpr uint32_t kWaveReplayEv = 1024;
+constexpr uint32_t kNoF = 0xFFu;          // an event end outside F
+constexpr uint32_t kPastLattice = 0xFEu;  // slot S of the lattice's last voxel: no voxel after it
+static_assert(kWaveReplayF < kPastLattice && kWaveReplayW < 1024u, "event fields");
+
+template <int S>
+struct WaveReplayLds {
+    uint64_t ev[kWaveReplayEv];           // events: (W index << 8 | walk position << 3 | direction) << 32 |
+                                          // species | F index of target << 8, of target + 1 << 16, of block << 24
+    uint32_t F[kWaveReplayF];
+    uint32_t FL[kWaveReplayF * (S / 4)];  // F's slots during the replay
+    uint32_t FO[kWaveReplayF];            // F's occupancy words during the replay
+    uint32_t FW[kWaveReplayF];            // F's occupancy words when the step began
+    uint32_t FS[kWaveReplayF];            // species bytes of F's count offsets
+    int32_t FD[kWaveReplayF * kSkewSp];   // the count offsets
+    uint32_t W[kWaveReplayW];             // the sources, ascending
+    uint8_t WF[kWaveReplayW];             // F index of each source (kNoF: none)
+    uint32_t T[kWaveReplayT];             // the noted voxels
+    uint32_t ext[kEscMax];                // voxels added to F, ascending
+    uint32_t n_ev, n_ext, esc;
+};
+
+// Ascending sort of the wave's arr[0..n) by bitonic merging (arr padded to n2, a power of two, with all ones).
+template <class K>
+__device__ void wave_bitonic(K *arr, uint32_t n, uint32_t n2)
+{
+    const uint32_t l = __lane_id();
+    for (uint32_t i = n + l; i < n2; i += 64u)
+        arr[i] = K(~K(0));
+    wave_sync();
+    for (uint32_t kk = 2; kk <= n2; kk <<= 1)
+        for (uint32_t jj = kk >> 1; jj > 0; jj >>= 1) {
+            for (uint32_t i = l; i < n2; i += 64u) {
+                const uint32_t m = i ^ jj;
+                if (m > i) {
+                    const K p = arr[i], q = arr[m];
+                    if ((i & kk) == 0u ? p > q : p < q) {
+                        arr[i] = q;
+                        arr[m] = p;
+                    }
+                }
+            }
+            wave_sync();
+        }
+}
+
+__device__ __forceinline__ uint32_t pow2_at_least(uint32_t n)
+{
+    uint32_t n2 = 1u;
+    while (n2 < n)
+        n2 <<= 1;
+    return n2;
+}
+
+// The voxels cand[0..n) (0xFFFFFFFF: none) sorted, without repeats, into out; returns their number (one wave;
+// cand is overwritten).
+__device__ uint32_t wave_sort_unique(uint32_t *cand, uint32_t n, uint32_t *out)
+{
+    const uint32_t l = __lane_id();
+    wave_bitonic<uint32_t>(cand, n, pow2_at_least(n));
+    uint32_t m = 0;
+    for (uint32_t b0 = 0; b0 < n; b0 += 64u) {
+        const uint32_t i = b0 + l;
+        const uint32_t v = i < n ? cand[i] : 0xFFFFFFFFu;
+        const bool keep = v != 0xFFFFFFFFu && (i == 0u || cand[i - 1u] != v);
+        const uint64_t mk = __ballot(keep);
+        if (keep)
+            out[m + __builtin_amdgcn_mbcnt_hi(uint32_t(mk >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mk), 0u))] = v;
+        m += uint32_t(__popcll(mk));
+    }
+    wave_sync();
+    return m;
+}
+
+// neighbor_or_none(a, v, k) for k = 0 .. nk-1 (the six neighbours, v, v + 1), the coordinates formed once.
+__device__ __forceinline__ void hood_list(const StepArgs &a, uint32_t v, uint32_t nk, uint32_t *out)
+{
+    const uint32_t z = v % a.R2, y = (v / a.R2) % a.R1, x = v / a.R12;
+    out[0] = x > 0u ? v - a.R12 : 0xFFFFFFFFu;
+    out[1] = x + 1u < a.R0 ? v + a.R12 : 0xFFFFFFFFu;
+    out[2] = y > 0u ? v - a.R2 : 0xFFFFFFFFu;
+    out[3] = y + 1u < a.R1 ? v + a.R2 : 0xFFFFFFFFu;
+    out[4] = z > 0u ? v - 1u : 0xFFFFFFFFu;
+    out[5] = z + 1u < a.R2 ? v + 1u : 0xFFFFFFFFu;
+    out[6] = v;
+    if (nk > 7u)
+        out[7] = v + 1u < a.N ? v + 1u : 0xFFFFFFFFu;
+}
+
+// Called by every lane of a wave with the same cluster. Returns 1 when replayed, 0 when the cluster cannot be
+// replayed exactly (as replay_cluster's false: its gathered output stands), -1 when the events overflow the list
+// (nothing written: the caller replays it in the one-lane form).
+template <int S>
+__device__ int replay_cluster_wave(const StepArgs &a, const SkewTab &k, WaveReplayLds<S> &L, const uint32_t *T,
+                                   uint32_t nt, uint32_t xlo_t, ReplayTally &tally)
+{
+    constexpr uint32_t NW = S / 4;
+    const uint32_t l = __lane_id();
+    const Tab tb = global_tab(a);
+    const uint32_t base = a.vox_base;
+    const int xlo = int(xlo_t) - kEscX, xhi = int(T[nt - 1u] / a.R12) + kEscX;
+    for (uint32_t i = l; i < nt; i += 64u)
+        L.T[i] = T[i];
+    if (l == 0u)
+        L.n_ext = 0u;
+    wave_sync();
+    long long failed = 0, alias = 0;  // lane 0
+    bool bad = false;
+    uint32_t nF = 0;
+    bool done = false;
+    for (int round = 0; round <= kEscMax && !done; ++round) {
+        // F: the noted voxels' neighbours, themselves and the voxels after them, and the voxels added
+        uint32_t *cand = reinterpret_cast<uint32_t *>(L.ev);
+        const uint32_t n_ext = L.n_ext;
+        for (uint32_t i = l; i < nt; i += 64u)
+            hood_list(a, L.T[i], 8u, cand + 8u * i);
+        for (uint32_t i = l; i < n_ext; i += 64u)
+            cand[8u * nt + i] = L.ext[i];
+        wave_sync();
+        nF = wave_sort_unique(cand, 8u * nt + n_ext, L.F);
+        if (l == 0u)
+            L.n_ev = 0u;
+        if (int(L.F[0] / a.R12) < xlo || int(L.F[nF - 1u] / a.R12) > xhi)
+            return 0;
+        for (uint32_t i = l; i < nF; i += 64u) {
+            const uint32_t v = L.F[i];
+            const Slots<S> sv = load_slots<S>(a.src, v);
+            for (uint32_t q = 0; q < NW; ++q)
+                L.FL[i * NW + q] = sv.w[q];
+            const int ei = skew_find(k, base + v);
+            const SkewEntry e = ei >= 0 ? k.ent[ei] : SkewEntry{0u, 0, 0u, 0u, {0, 0, 0, 0}};
+            L.FO[i] = L.FW[i] = count_nonzero(sv) + uint32_t(e.off);
+            L.FS[i] = e.sp;
+            for (int q = 0; q < kSkewSp; ++q)
+                L.FD[i * kSkewSp + q] = e.dc[q];
+        }
+        for (uint32_t i = l; i < nF; i += 64u)
+            hood_list(a, L.F[i], 7u, cand + 7u * i);
+        wave_sync();
+        const uint32_t nW = wave_sort_unique(cand, 7u * nF, L.W);
+        // the moves of the sources W = F and its six neighbours that have an end in F
+        for (uint32_t wi = l; wi < nW; wi += 64u) {
+            const uint32_t w = L.W[wi];
+            const int wF = find_sorted(L.F, nF, w);
+            L.WF[wi] = uint8_t(wF >= 0 ? uint32_t(wF) : kNoF);
+            const Slots<S> sw = load_slots<S>(a.src, w);
+            // the walk: occupancy word at the step's start, with the reference's u32 end index
+            const uint32_t occw = wF >= 0 ? L.FW[wF] : count_nonzero(sw);
+            const uint32_t start = w * uint32_t(S), end = start + occw;
+            uint32_t walk = end > start ? end - start : 0u;
+            if (walk == 0u)
+                continue;
+            if (walk > 2u * uint32_t(S)) {  // a word wrapped below zero at the first voxels: unrepresentable
+                bad = true;
+                walk = 2u * uint32_t(S);
+            }
+            const uint32_t z = w % a.R2, y = (w / a.R2) % a.R1, x = w / a.R12;
+            const Hood h = make_hood(a, x, y, z, w, S);
+            const Slots<S> sn = (walk > uint32_t(S) && w + 1u < a.N) ? load_slots<S>(a.src, w + 1u) : zero_slots<S>();
+            for (uint32_t s = 0; s < walk; ++s) {
+                const uint32_t blk = w + s / uint32_t(S), j = s % uint32_t(S);
+                const uint32_t sp = blk >= a.N ? 0u : (blk == w ? get_byte_dyn(sw, j) : get_byte_dyn(sn, j));
+                if (sp == 0u)
+                    continue;
+                const uint32_t d = hop(a, tb, h, sp, s);
+                if (d >= 6u)
+                    continue;
+                const uint32_t t = neighbor_or_none(a, w, int(d));
+                const int tF = find_sorted(L.F, nF, t);
+                if (tF < 0 && wF < 0)
+                    continue;  // neither end in F: the gather has it
+                const int bF = blk == w ? wF : find_sorted(L.F, nF, blk);
+                uint32_t t1F = kNoF;
+                if (tF >= 0) {
+                    if (t + 1u >= a.N) {
+                        t1F = kPastLattice;
+                    } else {
+                        const int f1 = find_sorted(L.F, nF, t + 1u);
+                        t1F = f1 >= 0 ? uint32_t(f1) : kNoF;
+                    }
+                }
+                const uint32_t idx = atomicAdd(&L.n_ev, 1u);
+                if (idx < kWaveReplayEv)
+                    L.ev[idx] = (uint64_t((wi << 8) | (s << 3) | d) << 32) |
+                                (sp | (tF >= 0 ? uint32_t(tF) : kNoF) << 8 | t1F << 16 |
+                                 (bF >= 0 ? uint32_t(bF) : kNoF) << 24);
+            }
+        }
+        wave_sync();
+        const uint32_t n_ev = L.n_ev;
+        if (n_ev > kWaveReplayEv)
+            return -1;
+        if (n_ev > 1u) {
+            uint32_t n2 = 1u;
+            while (n2 < n_ev)
+                n2 <<= 1;
+            wave_bitonic<uint64_t>(L.ev, n_ev, n2);
+        }
+        // the claims and departures in the canonical order (the scatter form of rdme.wgsl:45-88), on one lane
+        if (l == 0u) {
+            failed = alias = 0;
+            uint32_t esc = 0xFFFFFFFFu;
+            uint8_t *FLb = reinterpret_cast<uint8_t *>(L.FL);
+            for (uint32_t i = 0; i < n_ev; ++i) {
+                const uint64_t e = L.ev[i];
+                const uint32_t key = uint32_t(e >> 32), lo = uint32_t(e);
+                const uint32_t wi = key >> 8, s = (key >> 3) & 31u, d = key & 7u;
+                const uint32_t sp = lo & 0xFFu, tF = (lo >> 8) & 0xFFu, t1F = (lo >> 16) & 0xFFu, bF = lo >> 24;
+                const uint32_t wF = L.WF[wi];
+                if (tF != kNoF) {
+                    const uint32_t old = L.FO[tF];
+                    L.FO[tF] = old + 1u;
+                    bool ok = false;
+                    if (old <= uint32_t(S)) {  // rdme.wgsl:62: admits a claim into a voxel holding S
+                        for (uint32_t q = 0; q < NW && !ok; ++q) {
+                            const uint32_t zb = ~nonzero_bytes(L.FL[tF * NW + q]) & 0x80808080u;
+                            if (zb != 0u) {  // the first empty slot
+                                FLb[tF * uint32_t(S) + 4u * q + (uint32_t(__ffs(zb) - 1) >> 3)] = uint8_t(sp);
+                                ok = true;
+                            }
+                        }
+                        if (!ok && t1F != kPastLattice) {  // slot S: slot 0 of the next voxel
+                            if (t1F == kNoF) {
+                                esc = neighbor_or_none(a, L.W[wi], int(d)) + 1u;
+                                break;
+                            }
+                            if (FLb[t1F * uint32_t(S)] == 0u) {
+                                FLb[t1F * uint32_t(S)] = uint8_t(sp);
+                                ok = true;
+                                ++alias;
+                                // C[t][sp] counts it, the slots of t+1 hold it
+                                bad |= !dc_add(L.FS[tF], L.FD + tF * kSkewSp, sp, 1);
+                                bad |= !dc_add(L.FS[t1F], L.FD + t1F * kSkewSp, sp, -1);
+                            }
+                        }
+                    }
+                    if (!ok) {
+                        L.FO[tF] = old;
+                        ++failed;
+                        if (wF == kNoF) {  // a source outside F keeps a particle the gather let go
+                            esc = L.W[wi];
+                            break;
+                        }
+                        continue;
+                    }
+                }
+                if (wF == kNoF)
+                    continue;
+                // the source: O[w] -= 1, lattice[i_part] = 0, C[w][sp] -= 1 (rdme.wgsl:69-71)
+                L.FO[wF] -= 1u;
+                if (bF == kNoF) {  // a walk past slot S into a block outside F
+                    esc = L.W[wi] + s / uint32_t(S);
+                    break;
+                }
+                const uint32_t at = bF * uint32_t(S) + s % uint32_t(S);
+                const uint32_t cur = FLb[at];
+                FLb[at] = 0u;
+                if (!(bF == wF && cur == sp)) {
+                    bad |= !dc_add(L.FS[bF], L.FD + bF * kSkewSp, cur, 1);
+                    bad |= !dc_add(L.FS[wF], L.FD + wF * kSkewSp, sp, -1);
+                }
+            }
+            if (esc != 0xFFFFFFFFu && L.n_ext < uint32_t(kEscMax)) {
+                uint32_t q = L.n_ext++;  // keep ext sorted
+                while (q > 0u && L.ext[q - 1u] > esc) {
+                    L.ext[q] = L.ext[q - 1u];
+                    --q;
+                }
+                L.ext[q] = esc;
+            } else if (esc != 0xFFFFFFFFu) {
+                L.n_ext = uint32_t(kEscMax) + 1u;  // more voxels than F may add
+            }
+            L.esc = esc;
+        }
+        wave_sync();
+        if (L.esc == 0xFFFFFFFFu)
+            done = true;
+        else if (L.n_ext > uint32_t(kEscMax) || __ballot(bad) != 0ull)
+            return 0;
+    }
+    if (!done)
+        return 0;
+    // the counts the step kernel took for F's gathered outputs; reaction and compaction of F; outputs, offsets and
+    // live bits (one voxel per lane)
+    for (uint32_t i = l; i < nF; i += 64u) {
+        const uint32_t v = L.F[i];
+        VoxelFlags g;
+        (void)gather_voxel<S>(a, tb, v, g);
+        tally.fired -= g.fired;
+        tally.rolled -= g.rolled;
+        tally.near -= g.near;
+        int ei = skew_find(k, base + v);
+        const uint32_t kf = ei >= 0 ? k.ent[ei].kf : 0u;
+        XVox<S> vx;
+        for (uint32_t q = 0; q < NW; ++q)
+            vx.sl.w[q] = L.FL[i * NW + q];
+        vx.occ = L.FO[i];
+        vx.sp = L.FS[i];
+        for (int q = 0; q < kSkewSp; ++q)
+            vx.dc[q] = L.FD[i * kSkewSp + q];
+        bool fired, rolled, near, xbad = false;
+        exact_react<S>(a, vx, kf & 0xFFu, v / a.R12, (v / a.R2) % a.R1, v % a.R2, fired, rolled, near, xbad);
+        tally.fired += fired;
+        tally.rolled += rolled;
+        tally.near += near;
+        bad |= xbad;
+        store_slots<S>(a.dst, v, vx.sl);
+        const int32_t off = int32_t(vx.occ - count_nonzero(vx.sl));
+        if (count_nonzero(vx.sl) != 0u)
+            sum_mark(a.sum_dst, v);
+        if (ei < 0 && (off != 0 || vx.sp != 0u))
+            ei = skew_insert(k, base + v);
+        if (ei >= 0) {
+            SkewEntry &e = k.ent[ei];
+            e.off = off;
+            e.sp = vx.sp;
+            for (int q = 0; q < kSkewSp; ++q)
+                e.dc[q] = vx.dc[q];
+            e.kf = kf | kSkewDone;
+            set_voxel_bit(k.live, base + v, skew_live(e));
+        } else if (off != 0 || vx.sp != 0u) {
+            bad = true;  // table full
+        }
+    }
+    const bool any_bad = __ballot(bad) != 0ull;
+    if (l == 0u) {
+        tally.failed += failed;
+        tally.alias += alias;
+        tally.bad += any_bad;
+    }
+    return 1;
+}
+
 // The reaction of a live entry that no cluster replayed this step: the step kernel stored the voxel's
 // post-diffusion slots and left its reaction here.
 template <int S>
@@ -1423,7 +1772,11 @@ template <int S, class Cells>
 __device__ void replay_step(const Cells &cells, const SkewTab &k, const ReplayBuf &b, unsigned long long *counters)
 {
     __shared__ uint32_t s_n, s_nx, s_ncl;
-    __shared__ uint32_t s_sort[kSortLds];
+    // the sort's list, and afterwards one replay area per wave
+    constexpr size_t kWaves = kReplayThreads / 64u;
+    constexpr size_t kRaw = std::max(size_t(kSortLds) * sizeof(uint32_t), kWaves * sizeof(WaveReplayLds<S>));
+    __shared__ __align__(16) uint8_t s_raw[kRaw];
+    uint32_t *s_sort = reinterpret_cast<uint32_t *>(s_raw);
     __shared__ unsigned long long s_tally[6];  // fired, rolled back, near, exceptions, failed claims, aliases
     const uint32_t tid = threadIdx.x, nth = blockDim.x;
     const uint32_t ne = k.ctl[0];
@@ -1505,10 +1858,26 @@ __device__ void replay_step(const Cells &cells, const SkewTab &k, const ReplayBu
         const uint32_t ncl = min(s_ncl, b.ccap);
         if (tid == 0 && s_ncl > b.ccap)
             tally.bad += s_ncl - b.ccap;
+        // clusters of more than kWaveReplayT noted voxels: one lane each, the arrays in global memory
         for (uint32_t c = tid; c < ncl; c += nth) {
             const uint4 cl = b.clus[c];
-            if (!replay_cluster<S>(cells.args(cl.z), k, b, b.T + cl.x, cl.y, cl.w, tally))
+            if (cl.y > kWaveReplayT && !replay_cluster<S>(cells.args(cl.z), k, b, b.T + cl.x, cl.y, cl.w, tally))
                 tally.bad += cl.y;
+        }
+        // the others: one wave each, the arrays in its LDS
+        WaveReplayLds<S> &wl = reinterpret_cast<WaveReplayLds<S> *>(s_raw)[tid / 64u];
+        for (uint32_t c = tid / 64u; c < ncl; c += nth / 64u) {
+            const uint4 cl = b.clus[c];
+            if (cl.y > kWaveReplayT)
+                continue;
+            const StepArgs ca = cells.args(cl.z);
+            const int r = replay_cluster_wave<S>(ca, k, wl, b.T + cl.x, cl.y, cl.w, tally);
+            if (r < 0 && __lane_id() == 0u) {
+                if (!replay_cluster<S>(ca, k, b, b.T + cl.x, cl.y, cl.w, tally))
+                    tally.bad += cl.y;
+            } else if (r == 0 && __lane_id() == 0u) {
+                tally.bad += cl.y;
+            }
         }
         __syncthreads();
     }
@@ -2253,14 +2622,6 @@ struct WaveLds {
 __device__ __forceinline__ void vm_drain()
 {
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0), expcnt and lgkmcnt left alone
-}
-
-__device__ __forceinline__ void wave_sync()
-{
-    // LDS accesses of one wave complete in order; this keeps the compiler from moving them across.
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 __device__ __forceinline__ int ring3(int x) { return (x + 3) % 3; }
