@@ -103,14 +103,25 @@ def test_sphere_volume_matches_predicate():
 
 
 def test_cube_and_shell_regions():
-    lam = 32e-9
+    # simulation.rs:627-658: voxels [trunc(p0 / vs), trunc(pf / vs)) on each axis, vs = dims / res. With a
+    # power-of-two voxel edge every quotient below is exact in f32, so the box is fixed voxel for voxel:
+    # whole multiples start and end on that voxel, fractions truncate toward the lower one.
+    lam = 2.0 ** -25
     sim = Simulation(LatticeParams([20 * lam] * 3, [20] * 3, 1e-5, lam, 8))
-    sim.add_region(RegionType.Cube("box", [2 * lam, 3 * lam, 4 * lam], [10 * lam, 9 * lam, 8 * lam]), 1e-12)
+    sim.add_region(RegionType.Cube("box", [2 * lam, 3.5 * lam, 4.75 * lam], [10 * lam, 9.25 * lam, 8.5 * lam]),
+                   1e-12)
     reg = sim.regions()
     exp = np.zeros((20, 20, 20), bool)
     exp[2:10, 3:9, 4:8] = True
-    # truncation of p/voxel_size may land one voxel short on an exact multiple; accept either side
-    assert abs(int((reg == 1).sum()) - int(exp.sum())) <= 6 * 8 * 4 and np.all(reg[exp[:, :, :] & (reg == 1)] == 1)
+    assert np.array_equal(reg == 1, exp) and np.all(reg[~exp] == 0)
+    # a degenerate box (p0 == pf after truncation) holds no voxel; a later box overwrites an earlier one
+    sim.add_region(RegionType.Cube("flat", [5 * lam, 5 * lam, 5 * lam], [5.5 * lam, 12 * lam, 12 * lam]), 1e-12)
+    sim.add_region(RegionType.Cube("corner", [0.0, 0.0, 0.0], [3 * lam, 4 * lam, 5 * lam]), 1e-12)
+    reg = sim.regions()
+    assert int((reg == 2).sum()) == 0
+    corner = np.zeros((20, 20, 20), bool)
+    corner[0:3, 0:4, 0:5] = True
+    assert np.array_equal(reg == 3, corner) and np.array_equal(reg == 1, exp & ~corner)
     sim2 = Simulation(LatticeParams([30 * lam] * 3, [30] * 3, 1e-5, lam, 8))
     sim2.add_region(RegionType.SphericalShell("mem", "cyto", [15 * lam] * 3, 10 * lam, 12 * lam), 1e-12)
     r2 = sim2.regions()

@@ -4,7 +4,8 @@
   of xorshift32 and the u32 -> f32 map.
 * Tiny lattices whose outcome is derived by hand from res/shader/rdme.wgsl, cme.wgsl and
   final_stage.wgsl: forced hops (direction order), the capacity hazards (slot-S alias, rejected claim),
-  one reaction per voxel per step, the rollback at S slots, the zero-propensity firing at seed 0.
+  one reaction per voxel per step, the rollback at S slots, the zero-propensity firing at seed 0, the choice
+  between two channels by the cumulative propensity (exact powers of two, r2 from the pinned RNG).
 * Closed forms: hop probabilities, acceptance probability 1 - exp(-a*tau) (binomial 3-sigma).
 * Invariants on dense random lattices: conservation, occupancy == non-empty slots, counts == slots.
 """
@@ -164,6 +165,42 @@ def test_one_reaction_per_voxel_per_step(oracle_mod):
         seen.append(list(o.lattice))
     assert seen == [[1, 1, 2, 0], [1, 2, 2, 0], [2, 2, 2, 0], [2, 2, 2, 0], [2, 2, 2, 0]]
     assert list(o.stat[1:]) == [0, 3]
+
+
+def _pcg(state):
+    # random.wgsl:14-20 (pinned by tests/golden/rng_kat.json above)
+    state &= 0xFFFFFFFF
+    word = (((state >> ((state >> 28) + 4)) ^ state) * 277803737) & 0xFFFFFFFF
+    return (word >> 22) ^ word
+
+
+def _uf(state):
+    # random.wgsl:32-45 then 27-29: xorshift32, u32 -> f32 times 2^-32
+    s = state & 0xFFFFFFFF
+    s ^= (s << 13) & 0xFFFFFFFF
+    s ^= s >> 17
+    s ^= (s << 5) & 0xFFFFFFFF
+    return float(np.float32(s) * np.float32(2.0 ** -32))
+
+
+def test_two_channels_split_by_cumulative_propensity(oracle_mod):
+    # One voxel holding two A; A -> B at k and A -> C at 3k, k = 2^90 so that a1 = 2^91, a2 = 3 * 2^91 and
+    # a0 = 2^93 are exact and 1 - exp(-a0 * tau) = 1: every step fires. cme.wgsl:50-52, 68-73 pick the first i
+    # with r2 <= cum_i / a0, i.e. A -> B for r2 <= 1/4 and A -> C above, with r2 = UF(PCG(seed) + 1) and seed =
+    # itime + x*y + z = itime. The rewrite (cme.wgsl:85-114) then writes A, B or A, C.
+    seen = {"B": [], "C": []}
+    for itime in range(1, 400):
+        r2 = _uf(_pcg(itime) + 1)
+        o = _one_voxel(oracle_mod, [1, 1, 0, 0], [[0, -1, 1, 0], [0, -1, 0, 1]], [[1], [1]],
+                       [2.0 ** 90, 3 * 2.0 ** 90], ns=3)
+        o.step(itime)
+        assert o.counters.fired == 1
+        want = [1, 2, 0, 0] if r2 <= 0.25 else [1, 3, 0, 0]
+        assert list(o.lattice) == want, (itime, r2)
+        seen["B" if r2 <= 0.25 else "C"].append(r2)
+    # both channels taken, from either side of the split and far from it
+    assert len(seen["B"]) > 50 and len(seen["C"]) > 200
+    assert min(seen["C"]) < 0.3 and max(seen["B"]) > 0.2 and max(seen["C"]) > 0.9
 
 
 def test_rollback_when_rewrite_reaches_s(oracle_mod):
