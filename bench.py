@@ -104,7 +104,8 @@ def ensemble_leg(args, populated: bool = False) -> dict:
              "their stationary means)" if populated else "started as stated: empty lattices, genes off")
     return {
         "workload": f"{len(cells)} replicate cells, gene-expression network (well-mixed gene switching, lattice "
-                    f"mRNA/protein), edges log-uniform 32..96, rates x LogNormal(sigma 0.2), S=8, seed 99, {start}",
+                    f"mRNA/protein), edges log-uniform 32..96, rates x LogNormal(sigma 0.2), S=8, dt=8us (every replicate's "
+                    f"hop vector sums below 1), seed 99, {start}",
         "populated": populated, "particles": particles,
         "voxels_per_step": vox, "unit": "replicate-voxel-steps/s",
         "batched_value": batched_value, "batched_ms_per_step": ens_ms / args.ensemble_steps,

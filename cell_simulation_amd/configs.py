@@ -11,7 +11,8 @@
 
 * `ensemble_cells` -- replicate cells of the gene-expression network with edges drawn log-uniform in
   32..96 voxels, every rate constant (and the diffusion constants) perturbed per replicate by a
-  log-normal factor (sigma 0.2), S=8, seed 99; per-cell itime seeds for an Ensemble.
+  log-normal factor (sigma 0.2), S=8, dt = 8 us (see ENSEMBLE_TAU), seed 99; per-cell itime seeds for an
+  Ensemble.
 * `precision_stress` -- 512^3 lattice, S=8, 32 nm voxels, dt = 5 us; the large-cell regions; 12 lattice
   species (about 10M particles in the cytoplasm and nucleoid) with region-dependent D, 40 well-mixed species
   (10..2000 copies each); 120 reactions in total: 40 per-voxel lattice reactions with rate constants
@@ -170,8 +171,15 @@ def gene_expression(seed: int = 7, edge: int = 128, slots: int = 8, tau: float =
     return sim
 
 
+# Time step of the replicate cells: the gene-expression network at 8 us keeps every replicate's hop vector below 1
+# (protein: 6 * D * f * tau / lambda^2 = 0.47 f, with f the replicate's diffusion factor). At the single cell's
+# 20 us it is 1.17 f: the reference's cumulative vector then truncates the +z entry (rdme.wgsl:138-191), and every
+# protein drifts onto the -z membrane, where the packed voxels make each step a capacity replay.
+ENSEMBLE_TAU = 8e-6
+
+
 def ensemble_cells(n_cells: int = 512, seed: int = 99, edge_range=(32, 96), sigma: float = 0.2,
-                   rate_scale: float = 1.0, slots: int = 8, populated: bool = False):
+                   rate_scale: float = 1.0, slots: int = 8, populated: bool = False, tau: float = ENSEMBLE_TAU):
     """Replicate cells for a batched ensemble: returns (cells, itime_seeds). `populated`: see gene_expression."""
     rng = np.random.Generator(np.random.PCG64(seed))
     lo, hi = edge_range
@@ -179,7 +187,7 @@ def ensemble_cells(n_cells: int = 512, seed: int = 99, edge_range=(32, 96), sigm
     cells, seeds = [], []
     for i, e in enumerate(edges):
         factors = np.exp(rng.normal(0.0, sigma, size=7))
-        cells.append(gene_expression(seed=seed * 100003 + i, edge=int(e), slots=slots, rate_scale=rate_scale,
+        cells.append(gene_expression(seed=seed * 100003 + i, edge=int(e), slots=slots, tau=tau, rate_scale=rate_scale,
                                      rate_factors=factors, populated=populated))
         seeds.append(int(rng.integers(0, 2**32)))
     return cells, seeds

@@ -22,7 +22,7 @@ N_CELLS = 7
 
 
 def _cells():
-    return configs.ensemble_cells(N_CELLS, seed=11, edge_range=(16, 72), rate_scale=20000.0)
+    return configs.ensemble_cells(N_CELLS, seed=11, edge_range=(16, 72), rate_scale=20000.0, tau=20e-6)
 
 
 def _plain(cells, seeds, nsteps):
@@ -178,16 +178,17 @@ def test_dense_and_reservoir_cells_replayed_per_cell(oracle_mod):
     assert any(np.any(o.counts_array()[:, 1:] < 0) for o in orcs), "the case must take counts below zero"
 
 
-@pytest.mark.parametrize("populated", [False, True])
-def test_gene_expression_512_cells(oracle_mod, populated):
+@pytest.mark.parametrize("populated,tau", [(False, configs.ENSEMBLE_TAU), (True, configs.ENSEMBLE_TAU),
+                                           (True, 20e-6)])
+def test_gene_expression_512_cells(oracle_mod, populated, tau):
     """The 512-cell workload (edges 32..96, S=8, rates x LogNormal(0.2), seed 99) for two 100-step graph blocks:
     the three smallest cells and the two most crowded against the oracle's whole state, every cell's slots
     compacted, particles outside the cytoplasm and nucleoid only where a slot-S alias put them (a claim into a full
-    voxel lands in slot 0 of the next voxel, rdme.wgsl:30-40; a perturbed diffusion constant can take 6p past 1,
-    and the cut cumulative vector drives the proteins against one side of the cell), no exception. `populated`
-    starts each cell near its stationary state (see configs.gene_expression)."""
-    announce(f"512 gene-expression cells (populated={populated}), 200 steps in graph blocks of 100")
-    cells, seeds = configs.ensemble_cells(512, seed=99, populated=populated)
+    voxel lands in slot 0 of the next voxel, rdme.wgsl:30-40), no exception. `populated` starts each cell near its
+    stationary state (see configs.gene_expression). At 20 us the protein's 6p passes 1, the cut cumulative vector
+    drives the proteins against the -z side of the cell, and the packed voxels are replayed every step."""
+    announce(f"512 gene-expression cells (populated={populated}, dt={tau}), 200 steps in graph blocks of 100")
+    cells, seeds = configs.ensemble_cells(512, seed=99, populated=populated, tau=tau)
     sizes = [int(np.prod(c.info()["res"])) for c in cells]
     crowd = [int(np.count_nonzero(c.lattice())) / sizes[i] for i, c in enumerate(cells)]
     small = sorted(range(len(cells)), key=lambda i: sizes[i])[:3]
