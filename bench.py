@@ -219,6 +219,38 @@ def cpu_leg(args, N: int, S: int) -> dict:
     return out
 
 
+def kernel_variants(args, steps: int) -> dict:
+    """The 256^3 step on each step kernel the library can take, `steps` timed steps each (HIP events around the
+    kernel, after 5 warm-up steps): the wave-march kernel with its tables in LDS (the default), the same kernel with
+    only the hop tables in LDS and the reaction tables in global memory (the form it takes when the tables pass
+    24 KB, for instance with hundreds of species and reactions), with every table in global memory (when even the
+    hop tables pass 24 KB), the block-shared plane-march kernel and the one-lane-per-voxel kernel."""
+    from cell_simulation_amd import configs, default_itime_schedule
+
+    variants = [("wave_lds_tables", {}), ("wave_split_tables", {"CS_WAVE_GLOBAL_TABLES": "1"}),
+                ("wave_global_tables", {"CS_WAVE_GLOBAL_TABLES": "2"}),
+                ("tiled", {"CS_STEP_KERNEL": "tiled"}), ("one_lane_per_voxel", {"CS_STEP_KERNEL": "simple"})]
+    out = {}
+    for name, env in variants:
+        sim = configs.large_cell(seed=1234, edge=args.edge)
+        info = sim.info()
+        N, S = info["n_voxels"], info["max_particles_site"]
+        os.environ.update(env)
+        try:
+            sim.prepare_for_gpu(0)
+        finally:
+            for k in env:
+                os.environ.pop(k, None)
+        its = default_itime_schedule(1234, 5 + steps)
+        sim.run(0, its[:5])
+        ms = sim.run_timed(its[5:]) / steps
+        gbs = (2 * S + 1) * N / (ms * 1e-3) / 1e9
+        out[name] = {"ms_per_step": ms, "achieved_gbs": gbs, "frac": gbs / HBM_PEAK_GBS}
+        log(f"kernel {name}: {ms:.4f} ms/step")
+        sim.close()
+    return out
+
+
 def single_cell_configs(args) -> dict:
     """The 64^3 spherical cell for its 1000 steps and the 128^3 gene-expression cell for its 10000 steps (as
     stated, and started near its stationary state), plain one-launch-per-step path, wall time of the run."""
@@ -257,6 +289,8 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=10,
                     help="timed steps of the CPU baseline on the 256^3 workload, after one warm-up step (0 = skip)")
     ap.add_argument("--cpu-cfg1", type=int, default=1, help="also time the CPU baseline on the 64^3 cell (1/0)")
+    ap.add_argument("--kernel-variants", type=int, default=50, metavar="STEPS",
+                    help="time the 256^3 step on every step kernel for this many steps (0: skip)")
     ap.add_argument("--configs", type=int, default=1,
                     help="also time the 64^3 and 128^3 single-cell workloads on the GPU (1/0)")
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "large_cell_256_traffic.json"),
@@ -342,6 +376,7 @@ def main() -> int:
     sim.close()
     cpu = cpu_leg(args, N, S) if args.cpu_steps > 0 else None
     single = single_cell_configs(args) if args.configs else None
+    variants = kernel_variants(args, args.kernel_variants) if args.kernel_variants > 0 else None
 
     batched = None
     if args.ensemble_cells > 0:
@@ -375,6 +410,7 @@ def main() -> int:
                         "near_threshold": int(counters[2]), "fired": int(counters[3]),
                         "failed_claims": int(counters[8]), "slot_s_aliases": int(counters[9])},
         "configs": single,
+        "kernel_variants": variants,
         "batched": batched,
         "precision": precision,
     }

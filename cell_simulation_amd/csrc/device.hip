@@ -3379,21 +3379,39 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
 // first and the short ones fill in behind them. The queue is split per XCD: XCD k takes the groups of the k-th
 // band of block rows, whose neighbours' halo lines are then in its own L2; an XCD whose queue is empty helps
 // the others.
-template <int S>
+//
+// TM, where the model tables are read: kTabLds all staged in LDS; kTabSplit the hop probabilities and the propensity
+// bounds (every particle and voxel reads them) staged, the reaction tables (read only by the rare reactions) in global
+// memory through the caches; kTabGlobal all in global memory. The last two serve tables larger than the LDS a block
+// has beside its plane records.
+constexpr int kTabLds = 0, kTabSplit = 1, kTabGlobal = 2;
+
+template <int S, int TM>
 __global__ __launch_bounds__(kWaveBlock) __attribute__((amdgpu_waves_per_eu(3)))
 void rdme_step_wave(StepArgs a, WaveArgs w)
 {
     __shared__ WaveLds<S> L[kWavesPerBlock];
     extern __shared__ uint32_t tab_lds[];
-    for (uint32_t i = threadIdx.x; i < w.tab_words; i += kWaveBlock)
-        tab_lds[i] = w.tab[i];
+    const uint8_t *gbase = reinterpret_cast<const uint8_t *>(w.tab);
+    const uint8_t *lbase = reinterpret_cast<const uint8_t *>(tab_lds);
+    if constexpr (TM == kTabLds) {
+        for (uint32_t i = threadIdx.x; i < w.tab_words; i += kWaveBlock)
+            tab_lds[i] = w.tab[i];
+    } else if constexpr (TM == kTabSplit) {
+        // the hop probabilities (from offset 0) and then the S + 1 propensity bounds
+        const uint32_t np2 = w.off_rates / 4u, nb = uint32_t(S) + 1u;
+        for (uint32_t i = threadIdx.x; i < np2 + nb; i += kWaveBlock)
+            tab_lds[i] = i < np2 ? w.tab[i] : w.tab[w.off_bound / 4u + (i - np2)];
+    }
     __syncthreads();
-    const uint8_t *tbase = reinterpret_cast<const uint8_t *>(tab_lds);
-    const Tab tb{reinterpret_cast<const float *>(tbase + w.off_p2), reinterpret_cast<const float *>(tbase + w.off_rates),
-                 reinterpret_cast<const uint32_t *>(tbase + w.off_ridx),
-                 reinterpret_cast<const int8_t *>(tbase + w.off_stoich),
-                 reinterpret_cast<const uint64_t *>(tbase + w.off_rmask),
-                 reinterpret_cast<const float *>(tbase + w.off_bound)};
+    const uint8_t *hb = TM == kTabGlobal ? gbase : lbase;  // hop probabilities and bounds
+    const uint8_t *rb = TM == kTabLds ? lbase : gbase;     // reaction tables
+    const uint32_t off_bound = TM == kTabSplit ? w.off_rates : w.off_bound;
+    const Tab tb{reinterpret_cast<const float *>(hb + w.off_p2), reinterpret_cast<const float *>(rb + w.off_rates),
+                 reinterpret_cast<const uint32_t *>(rb + w.off_ridx),
+                 reinterpret_cast<const int8_t *>(rb + w.off_stoich),
+                 reinterpret_cast<const uint64_t *>(rb + w.off_rmask),
+                 reinterpret_cast<const float *>(hb + off_bound)};
     const uint32_t wv = threadIdx.x / 64u;
     // XCC_ID hardware register (id 20, bits 3:0): the XCD this block runs on
     uint32_t xcd = uint32_t(__builtin_amdgcn_s_getreg((31 << 11) | 20)) & 0xFu;
@@ -4386,22 +4404,35 @@ Device::Device(const Model &m, int hip_device) : dev_(hip_device)
     rmask_ = reinterpret_cast<uint64_t *>(tb + off_rmask_);
     stoich_ = reinterpret_cast<int8_t *>(tb + off_stoich_);
     bound_ = reinterpret_cast<float *>(tb + off_bound_);
-    if (blob_bytes > kTabLdsMax)
-        simple_ = true;  // tables too large to stage in LDS next to the plane records
     if (nreg_ >= 255u)
         tiled_ = true;   // the wave kernel's padded labels use 0xFF for "outside the lattice"
     if (res_[1] > 4096u || res_[2] > 4096u || res_[0] > 65535u)
         tiled_ = true;   // the wave kernel forms y * R2 with 24-bit multiplies, packs x into 16 bits
+    // tables too large to stage in LDS next to the plane records: the wave kernel stages the hop probabilities and
+    // bounds and reads the reaction tables from global memory, or reads everything there when even those are too
+    // large (the tiled kernel, which stages all of them, gives way to the one-lane-per-voxel kernel)
+    const size_t split_bytes = off_rates_ + (size_t(S_) + 1u) * sizeof(float);
+    const char *gt = std::getenv("CS_WAVE_GLOBAL_TABLES");  // test hook: "1" the split form if it fits, "2" global
+    if (blob_bytes > kTabLdsMax || gt != nullptr) {
+        if (tiled_ && blob_bytes > kTabLdsMax)
+            simple_ = true;
+        const bool all = gt != nullptr && std::string(gt) == "2";
+        tab_mode_ = !all && split_bytes <= kTabLdsMax ? kTabSplit : kTabGlobal;
+    }
     {
         // resident blocks of the persistent wave kernel: every CU filled to its occupancy
         int cus = 0, per_cu = 0;
         hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev_), "CU count");
-        const size_t shm = blob_bytes;
+        const size_t shm = wave_shm_bytes();
         hipError_t e = hipSuccess;
-        switch (S_) {
-        case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rdme_step_wave<4>, kWaveBlock, shm); break;
-        case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rdme_step_wave<8>, kWaveBlock, shm); break;
-        default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rdme_step_wave<16>, kWaveBlock, shm); break;
+        switch (S_ * 4 + tab_mode_) {
+#define CS_WAVE_OCC(s, tm) \
+    case s * 4 + tm: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rdme_step_wave<s, tm>, kWaveBlock, shm); break;
+        CS_WAVE_OCC(4, kTabLds) CS_WAVE_OCC(4, kTabSplit) CS_WAVE_OCC(4, kTabGlobal)
+        CS_WAVE_OCC(8, kTabLds) CS_WAVE_OCC(8, kTabSplit) CS_WAVE_OCC(8, kTabGlobal)
+        CS_WAVE_OCC(16, kTabLds) CS_WAVE_OCC(16, kTabSplit) CS_WAVE_OCC(16, kTabGlobal)
+#undef CS_WAVE_OCC
+        default: throw Error(CS_ERR_UNSUPPORTED, "slots per voxel must be 4, 8 or 16");
         }
         hip_check(e, "occupancy of the wave kernel");
         wave_blocks_ = uint32_t(std::max(cus, 1) * std::max(per_cu, 1));
@@ -4609,13 +4640,17 @@ void Device::step(uint32_t itime)
         w.items = static_cast<const uint2 *>(items_);
         w.sum_src = sum_src.bytes;
         const uint32_t blocks = std::min(n_items_, wave_blocks_);
-        const size_t shm = size_t(tab_words_) * 4u;
+        const size_t shm = wave_shm_bytes();
         // with a timing pair set, the dispatch itself stamps the events at the kernel's start and end
         hipEvent_t e0 = static_cast<hipEvent_t>(kernel_ev_[0]), e1 = static_cast<hipEvent_t>(kernel_ev_[1]);
-        switch (S_) {
-        case 4: hipExtLaunchKernelGGL(rdme_step_wave<4>, dim3(blocks), dim3(kWaveBlock), shm, st, e0, e1, 0u, a, w); break;
-        case 8: hipExtLaunchKernelGGL(rdme_step_wave<8>, dim3(blocks), dim3(kWaveBlock), shm, st, e0, e1, 0u, a, w); break;
-        case 16: hipExtLaunchKernelGGL(rdme_step_wave<16>, dim3(blocks), dim3(kWaveBlock), shm, st, e0, e1, 0u, a, w); break;
+        const dim3 g(blocks), b(kWaveBlock);
+        switch (S_ * 4 + tab_mode_) {
+#define CS_WAVE_LAUNCH(s, tm) \
+    case s * 4 + tm: hipExtLaunchKernelGGL(rdme_step_wave<s, tm>, g, b, shm, st, e0, e1, 0u, a, w); break;
+        CS_WAVE_LAUNCH(4, kTabLds) CS_WAVE_LAUNCH(4, kTabSplit) CS_WAVE_LAUNCH(4, kTabGlobal)
+        CS_WAVE_LAUNCH(8, kTabLds) CS_WAVE_LAUNCH(8, kTabSplit) CS_WAVE_LAUNCH(8, kTabGlobal)
+        CS_WAVE_LAUNCH(16, kTabLds) CS_WAVE_LAUNCH(16, kTabSplit) CS_WAVE_LAUNCH(16, kTabGlobal)
+#undef CS_WAVE_LAUNCH
         default: throw Error(CS_ERR_UNSUPPORTED, "slots per voxel must be 4, 8 or 16");
         }
     } else {
@@ -4689,6 +4724,15 @@ void Device::read_skew(std::vector<SkewVoxel> &out)
 {
     hip_check(hipSetDevice(dev_), "hipSetDevice");
     skew_->read(static_cast<hipStream_t>(stream_), out);
+}
+
+size_t Device::wave_shm_bytes() const
+{
+    switch (tab_mode_) {
+    case kTabLds: return size_t(tab_words_) * 4u;
+    case kTabSplit: return size_t(off_rates_) + (size_t(S_) + 1u) * sizeof(float);
+    default: return 0u;
+    }
 }
 
 float Device::run_timed(const uint32_t *itime, uint32_t nsteps)

@@ -81,6 +81,9 @@ private:
     bool has_zero_order_ = false;  // some reaction has no reactants
     bool simple_ = false;          // one-lane-per-voxel kernel without LDS staging
     bool tiled_ = false;           // block-shared plane-march kernel instead of the wave-march kernel
+    int tab_mode_ = 0;             // where the wave-march kernel reads the model tables: all in LDS (0), the hop
+                                   // tables in LDS and the reaction tables in global memory (1), all global (2)
+    size_t wave_shm_bytes() const; // its dynamic LDS for the tables
     uint32_t xc_ = 32;             // planes marched per tile: the longest x-range of a wave-kernel work item
     uint32_t wtiles_y_ = 0, wtiles_z_ = 0, pad_y_ = 0, pad_z_ = 0;  // wave tiles; padded label volume
     uint32_t ev_cap_ = 96;         // arrival events a wave-tile plane may hold

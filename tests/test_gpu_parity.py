@@ -182,7 +182,8 @@ def test_region_counts_match_lattice(oracle_mod):
     assert np.array_equal(sim.species_counts()[1:], exp.sum(axis=0)[1:])
 
 
-@pytest.mark.parametrize("kernel", ["wave", "wave-overflow", "wave-summary", "wave-bitmapsort", "tiled", "simple"])
+@pytest.mark.parametrize("kernel", ["wave", "wave-overflow", "wave-summary", "wave-bitmapsort", "wave-splittab",
+                                    "wave-globaltab", "tiled", "simple"])
 def test_dense_capacity_replay(oracle_mod, kernel):
     """S=4 at high density: claims into full voxels are rejected (rdme.wgsl:62, 73-81) or land in the next
     voxel's slot 0 (rdme.wgsl:30-40), after which the reference's occupancy word and counts leave the slots.
@@ -198,6 +199,10 @@ def test_dense_capacity_replay(oracle_mod, kernel):
             os.environ["CS_OCC_SUMMARY"] = "on"  # the replay's outputs must mark the occupancy summary
         if kernel == "wave-bitmapsort":
             os.environ["CS_REPLAY_LDS_SORT"] = "1"  # every noted list sorted through the replay's bitmaps
+        if kernel == "wave-splittab":
+            os.environ["CS_WAVE_GLOBAL_TABLES"] = "1"  # the reaction tables read from global memory
+        if kernel == "wave-globaltab":
+            os.environ["CS_WAVE_GLOBAL_TABLES"] = "2"  # every model table read from global memory
         try:
             sim = _walled_model(20, 4, dens, seed=31) if walled else _dense_model(20, 4, dens, seed=31)
             orc = oracle_mod.Oracle.from_simulation(sim)
@@ -207,6 +212,7 @@ def test_dense_capacity_replay(oracle_mod, kernel):
             os.environ.pop("CS_WAVE_EVENT_CAP", None)
             os.environ.pop("CS_OCC_SUMMARY", None)
             os.environ.pop("CS_REPLAY_LDS_SORT", None)
+            os.environ.pop("CS_WAVE_GLOBAL_TABLES", None)
         skew = 0
         for t, it in enumerate(default_itime_schedule(31, nsteps)):
             sim.step(t, int(it), 0)
@@ -218,6 +224,49 @@ def test_dense_capacity_replay(oracle_mod, kernel):
         if not walled:
             assert orc.counters.alias_writes > 0 and skew > 0, "the case must leave slot-S aliases"
         print(f"oracle {orc.counters.as_dict()}; tight {int(c[CS_CNT_TIGHT])}; skewed voxels up to {skew}")
+
+
+def _big_network(edge, seed, n_species=160, n_reactions=250):
+    """Two regions, n_species lattice species and n_reactions first- and second-order reactions: the tables
+    (stoichiometry alone (n_reactions + 1) x (n_species + 1) bytes) exceed what the wave kernel stages in LDS."""
+    rng = np.random.default_rng(seed)
+    lam = 32e-9
+    sim = Simulation(LatticeParams([edge * lam] * 3, [edge] * 3, 50e-6, lam, 8))
+    sim.set_seed(seed)
+    sim.add_region(RegionType.Sphere("core", [edge * lam / 2] * 3, edge * lam / 3), 1e-12)
+    sim.prepare_regions()
+    names = [f"P{i:03d}" for i in range(n_species)]
+    for i, nm in enumerate(names):
+        sim.add_particle_count(nm, "core" if i % 2 else "background", 40, True, False)
+        sim.set_diffusion_rate_particle(nm, "background", float(rng.uniform(1e-13, 4e-12)))
+        sim.set_diffusion_rate_particle(nm, "core", float(rng.uniform(1e-13, 4e-12)))
+        sim.set_transition_rate_particle(nm, "background", "core", 1e-12)
+        sim.set_transition_rate_particle(nm, "core", "background", 1e-12)
+    for j in range(n_reactions):
+        a = names[int(rng.integers(n_species))]
+        if j % 3:
+            sim.add_reaction([a], [names[int(rng.integers(n_species))]], float(rng.uniform(50.0, 2000.0)))
+        else:
+            b = names[int(rng.integers(n_species))]
+            sim.add_reaction([a, b], [names[int(rng.integers(n_species))]], float(rng.uniform(50.0, 2000.0)))
+    return sim
+
+
+def test_tables_past_lds_read_from_global_memory(oracle_mod):
+    """A 160-species, 250-reaction network: the tables (about 48 KB) do not fit the wave kernel's LDS beside the
+    plane records, so it stages the hop tables and reads the reaction tables from global memory. Against the
+    oracle's whole state."""
+    announce("160 species x 250 reactions (tables past LDS), 24^3 S=8, 60 steps")
+    sim = _big_network(24, seed=8)
+    orc = oracle_mod.Oracle.from_simulation(sim)
+    sim.prepare_for_gpu(0)
+    for t, it in enumerate(default_itime_schedule(8, 60)):
+        sim.step(t, int(it), 0)
+        orc.step(int(it))
+    check_state(sim, orc, "tables past LDS after 60 steps")
+    c = check_counters(sim, orc)
+    assert orc.counters.fired > 20, "the case must fire reactions"
+    print(f"oracle {orc.counters.as_dict()}; device counters {c.tolist()}")
 
 
 def test_dense_s16_full_heads(oracle_mod):
