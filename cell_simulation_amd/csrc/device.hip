@@ -264,7 +264,17 @@ struct Tab {
     const int8_t *stoich;   // [nr+1][ns+1]
     const uint64_t *rmask;  // [ns+1][nrw] reactions having the species as a reactant; row 0: zero-order
     const float *bound;     // [S+1] upper bound of the total propensity of a voxel holding c particles
+    const float *pair = nullptr;  // [ns+1][ns+1] upper bound for a voxel holding species a and b (0: no particle),
+                                  // or null; see pair_bounds
 };
+
+// The pair table follows the S + 1 propensity bounds in the table blob, 8-byte aligned; bound_words is the length
+// of both in 32-bit words (n1 = ns + 1, or 0 without a pair table).
+__host__ __device__ constexpr uint32_t pair_offset(uint32_t S) { return (S + 2u) & ~1u; }
+__host__ __device__ constexpr uint32_t bound_words(uint32_t S, uint32_t n1)
+{
+    return n1 != 0u ? pair_offset(S) + n1 * n1 : S + 1u;
+}
 
 __device__ __forceinline__ Tab global_tab(const StepArgs &a)
 {
@@ -366,13 +376,33 @@ __device__ __forceinline__ uint32_t wave_words(uint32_t cnt)
 // sequential sums bit for bit.
 // The reject test that opens react(): false when the voxel's draw lies above the upper bound of its total
 // propensity, i.e. nothing can fire there this step.
+// With the pair table, a voxel of at most four particles (species s0..s3 in slots 0..3, zeros past cnt) is bounded
+// by the sum of the pair entries over its pairs, which is at least its total propensity when no reaction has more
+// than two reactants (pair_bounds); more particles take bound[cnt].
 template <int S>
-__device__ __forceinline__ bool react_may_fire(const StepArgs &a, const Tab &tb, uint32_t cnt, uint32_t x,
-                                               uint32_t y, uint32_t z)
+__device__ __forceinline__ bool react_may_fire(const StepArgs &a, const Tab &tb, const Slots<S> &out, uint32_t cnt,
+                                               uint32_t x, uint32_t y, uint32_t z)
 {
     // x * y with x, y < 2^24: the low 32 bits of the 24-bit product are those of the 32-bit one
     const float r1 = uniform_float(pcg_hash(a.itime + __umul24(x, y) + z));
-    return !(r1 > tb.bound[cnt] * a.tau * 1.001f + 1e-6f);
+    float bnd;
+    if (tb.pair != nullptr && cnt <= 4u) {
+        const uint32_t n1 = a.ns + 1u, w0 = out.w[0];
+        const uint32_t s0 = w0 & 0xFFu, s1 = (w0 >> 8) & 0xFFu;
+        bnd = tb.pair[__umul24(s0, n1) + s1];
+        if (cnt >= 3u) {
+            const uint32_t s2 = (w0 >> 16) & 0xFFu;
+            bnd = bnd + tb.pair[__umul24(s0, n1) + s2] + tb.pair[__umul24(s1, n1) + s2];
+            if (cnt == 4u) {
+                const uint32_t s3 = w0 >> 24;
+                bnd = bnd + tb.pair[__umul24(s0, n1) + s3] + tb.pair[__umul24(s1, n1) + s3] +
+                      tb.pair[__umul24(s2, n1) + s3];
+            }
+        }
+    } else {
+        bnd = tb.bound[cnt];
+    }
+    return !(r1 > bnd * a.tau * 1.001f + 1e-6f);
 }
 
 template <int S>
@@ -2588,6 +2618,7 @@ struct WaveArgs {
     uint32_t ev_cap;        // events a plane may hold (<= kWEv); a plane with more takes the global form
     uint32_t tab_words;
     uint32_t off_p2, off_rates, off_ridx, off_stoich, off_rmask, off_bound;
+    uint32_t pair_n1;       // ns + 1 when the pair table follows the bounds (pair_offset), else 0
     const uint32_t *tab;
     const uint32_t *regp;   // padded labels [R0+4][PY][PZ] (0xFF outside the lattice), read as words
     uint32_t PY, PZ4;       // PY = kWY*wtiles_y + 4 rows; PZ4 = (kWZ*tiles_z + 4) / 4 words per row
@@ -3295,7 +3326,7 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
                     const uint64_t stay = first_nibbles(occ) & ~spread_nibbles(lv);
                     out = assemble<S>(own, occ, occ ? stay : 0ull, lo, min(nlo, 16u), hi, min(nhi, 16u), cnt);
                 }
-                const bool may = act && react_may_fire<S>(a, tb, cnt, uint32_t(Q), Y, Z);
+                const bool may = act && react_may_fire<S>(a, tb, out, cnt, uint32_t(Q), Y, Z);
                 // the result goes back to the list entry; the voxel's own lane stores whole rows below, so no
                 // output line is written in pieces at different times (a reaction still to run is stored
                 // by its batch instead, flagged here)
@@ -3398,8 +3429,8 @@ void rdme_step_wave(StepArgs a, WaveArgs w)
         for (uint32_t i = threadIdx.x; i < w.tab_words; i += kWaveBlock)
             tab_lds[i] = w.tab[i];
     } else if constexpr (TM == kTabSplit) {
-        // the hop probabilities (from offset 0) and then the S + 1 propensity bounds
-        const uint32_t np2 = w.off_rates / 4u, nb = uint32_t(S) + 1u;
+        // the hop probabilities (from offset 0) and then the propensity bounds with the pair table
+        const uint32_t np2 = w.off_rates / 4u, nb = bound_words(uint32_t(S), w.pair_n1);
         for (uint32_t i = threadIdx.x; i < np2 + nb; i += kWaveBlock)
             tab_lds[i] = i < np2 ? w.tab[i] : w.tab[w.off_bound / 4u + (i - np2)];
     }
@@ -3407,11 +3438,12 @@ void rdme_step_wave(StepArgs a, WaveArgs w)
     const uint8_t *hb = TM == kTabGlobal ? gbase : lbase;  // hop probabilities and bounds
     const uint8_t *rb = TM == kTabLds ? lbase : gbase;     // reaction tables
     const uint32_t off_bound = TM == kTabSplit ? w.off_rates : w.off_bound;
+    const float *bnd = reinterpret_cast<const float *>(hb + off_bound);
     const Tab tb{reinterpret_cast<const float *>(hb + w.off_p2), reinterpret_cast<const float *>(rb + w.off_rates),
                  reinterpret_cast<const uint32_t *>(rb + w.off_ridx),
                  reinterpret_cast<const int8_t *>(rb + w.off_stoich),
-                 reinterpret_cast<const uint64_t *>(rb + w.off_rmask),
-                 reinterpret_cast<const float *>(hb + off_bound)};
+                 reinterpret_cast<const uint64_t *>(rb + w.off_rmask), bnd,
+                 w.pair_n1 != 0u ? bnd + pair_offset(uint32_t(S)) : nullptr};
     const uint32_t wv = threadIdx.x / 64u;
     // XCC_ID hardware register (id 20, bits 3:0): the XCD this block runs on
     uint32_t xcd = uint32_t(__builtin_amdgcn_s_getreg((31 << 11) | 20)) & 0xFu;
@@ -3646,7 +3678,7 @@ struct EnsCell {
     uint32_t tiles_z, wtiles_y, btiles_y, chunks, first_item;  // wave tiles in z and y, groups in y, x-chunks
     uint32_t regp_off, PY, PZ4;  // the cell's padded labels: word offset, rows per plane, words per row
     uint32_t tab_off, tab_words;  // the cell's table blob, in 32-bit words of the tables buffer
-    uint32_t off_p2, off_rates, off_ridx, off_stoich, off_rmask, off_bound;
+    uint32_t off_p2, off_rates, off_ridx, off_stoich, off_rmask, off_bound, pair_n1;
     uint32_t ns, nr, nreg, nrw, has_zero_order, mod_zero;
     float tau;
     uint32_t itime_seed;       // itime(step) = low 32 bits of SplitMix64((itime_seed << 32) + step)
@@ -3738,6 +3770,7 @@ __device__ __forceinline__ WaveArgs cell_wave_args(const EnsArgs &e, const EnsCe
     w.off_stoich = c.off_stoich;
     w.off_rmask = c.off_rmask;
     w.off_bound = c.off_bound;
+    w.pair_n1 = c.pair_n1;
     w.tab = e.tab + c.tab_off;
     w.regp = e.regp + c.regp_off;
     w.PY = c.PY;
@@ -3823,7 +3856,9 @@ __global__ __launch_bounds__(kWaveBlock) void rdme_step_ensemble(EnsArgs e)
                          reinterpret_cast<const uint32_t *>(tbase + w.off_ridx),
                          reinterpret_cast<const int8_t *>(tbase + w.off_stoich),
                          reinterpret_cast<const uint64_t *>(tbase + w.off_rmask),
-                         reinterpret_cast<const float *>(tbase + w.off_bound)};
+                         reinterpret_cast<const float *>(tbase + w.off_bound),
+                         w.pair_n1 != 0u ? reinterpret_cast<const float *>(tbase + w.off_bound) + pair_offset(uint32_t(S))
+                                         : nullptr};
             if (busy)
                 wave_march<S>(L[wv], tb, a, w, tz, wy, xb, xe, tc);
             __syncthreads();  // the tables stay until every wave of the group is done
@@ -3927,6 +3962,55 @@ static std::vector<float> propensity_bounds(const Model &m, uint32_t S)
     return out;
 }
 
+// Pair table of the reaction reject test: entry [a][b] (a, b species, 0 for no particle) bounds from above the total
+// propensity sum_r k_r * C[r0] * C[r1] * C[r2] (cme.wgsl:42-44, C[0] = 1) of a voxel holding exactly the particles a
+// and b, rounded up. For models whose reactions have at most two reactants every term of that sum is a sum over
+// single particles and pairs of particles (k * n^2 of a homodimer row counts each particle once alone and each
+// pair twice), so for a voxel of c >= 2 particles the sum of the entries over its c(c-1)/2 pairs is at least its
+// total. Empty (no table) when some reaction has three reactants, a rate is negative or not finite, or the table
+// would pass 4 KB.
+static std::vector<float> pair_bounds(const Model &m)
+{
+    const uint32_t ns = m.num_species(), nr = m.num_reactions(), n1 = ns + 1u;
+    if (size_t(n1) * n1 * sizeof(float) > 4096u)
+        return {};
+    for (uint32_t r = 1; r <= nr; ++r) {
+        const double k = m.rates()[r];
+        if (!(k >= 0.0) || !std::isfinite(k))
+            return {};
+        int order = 0;
+        for (int j = 0; j < 3; ++j)
+            order += m.reactants()[size_t(r) * 3u + j] > 0 ? 1 : 0;
+        if (order > 2)
+            return {};
+    }
+    std::vector<float> out(size_t(n1) * n1);
+    for (uint32_t a = 0; a < n1; ++a)
+        for (uint32_t b = 0; b < n1; ++b) {
+            double cnt[256] = {0.0};
+            cnt[0] = 1.0;
+            if (a > 0u)
+                cnt[a] += 1.0;
+            if (b > 0u)
+                cnt[b] += 1.0;
+            double tot = 0.0;
+            for (uint32_t r = 1; r <= nr; ++r) {
+                double prod = m.rates()[r];
+                for (int j = 0; j < 3; ++j) {
+                    const int32_t sp = m.reactants()[size_t(r) * 3u + j];
+                    prod *= cnt[sp > 0 ? sp : 0];
+                }
+                tot += prod;
+            }
+            const double v = tot * (1.0 + 1e-4);
+            float f = float(v);
+            if (double(f) < v)
+                f = std::nextafter(f, INFINITY);
+            out[size_t(a) * n1 + b] = f;
+        }
+    return out;
+}
+
 // Host form of the occupancy summary (SumRef) of one compacted lattice: `out` (zeroed, tiles x R0 bytes) gets a 1
 // for every wave tile and plane holding a particle.
 static void summary_bytes(const uint8_t *lat, const uint32_t res[3], uint32_t S, uint32_t tiles_z, uint8_t *out)
@@ -3977,6 +4061,7 @@ struct TableBlob {
     std::vector<uint8_t> bytes;
     uint32_t off_p2 = 0, off_rates = 0, off_ridx = 0, off_rmask = 0, off_stoich = 0, off_bound = 0;
     uint32_t nrw = 0;
+    uint32_t pair_n1 = 0;  // ns + 1 when the pair table follows the bounds
     bool has_zero_order = false;
 };
 
@@ -4012,7 +4097,13 @@ static TableBlob make_table_blob(const Model &m)
     }
     for (uint32_t w = 0; w < b.nrw; ++w)
         b.has_zero_order = b.has_zero_order || rmask[w] != 0ull;
-    const std::vector<float> bound = propensity_bounds(m, m.slots());
+    std::vector<float> bound = propensity_bounds(m, m.slots());
+    const std::vector<float> pair = pair_bounds(m);
+    if (!pair.empty()) {
+        b.pair_n1 = n1;
+        bound.resize(pair_offset(m.slots()), 0.0f);
+        bound.insert(bound.end(), pair.begin(), pair.end());
+    }
     b.off_p2 = 0;
     b.off_rates = uint32_t(align8(b.off_p2 + p2.size() * sizeof(float)));
     b.off_ridx = uint32_t(align8(b.off_rates + m.rates().size() * sizeof(float)));
@@ -4342,6 +4433,7 @@ Device::Device(const Model &m, int hip_device) : dev_(hip_device)
     const uint32_t n1 = ns_ + 1u;
     const TableBlob blob = make_table_blob(m);
     nrw_ = blob.nrw;
+    pair_n1_ = blob.pair_n1;
     has_zero_order_ = blob.has_zero_order;
     std::vector<uint8_t> reg8(N_);
     for (uint64_t v = 0; v < N_; ++v)
@@ -4411,7 +4503,7 @@ Device::Device(const Model &m, int hip_device) : dev_(hip_device)
     // tables too large to stage in LDS next to the plane records: the wave kernel stages the hop probabilities and
     // bounds and reads the reaction tables from global memory, or reads everything there when even those are too
     // large (the tiled kernel, which stages all of them, gives way to the one-lane-per-voxel kernel)
-    const size_t split_bytes = off_rates_ + (size_t(S_) + 1u) * sizeof(float);
+    const size_t split_bytes = off_rates_ + size_t(bound_words(S_, blob.pair_n1)) * sizeof(float);
     const char *gt = std::getenv("CS_WAVE_GLOBAL_TABLES");  // test hook: "1" the split form if it fits, "2" global
     if (blob_bytes > kTabLdsMax || gt != nullptr) {
         if (tiled_ && blob_bytes > kTabLdsMax)
@@ -4631,6 +4723,7 @@ void Device::step(uint32_t itime)
         w.off_stoich = off_stoich_;
         w.off_rmask = off_rmask_;
         w.off_bound = off_bound_;
+        w.pair_n1 = pair_n1_;
         w.tab = static_cast<const uint32_t *>(tab_);
         w.regp = reinterpret_cast<const uint32_t *>(regp_);
         w.PY = pad_y_;
@@ -4730,7 +4823,7 @@ size_t Device::wave_shm_bytes() const
 {
     switch (tab_mode_) {
     case kTabLds: return size_t(tab_words_) * 4u;
-    case kTabSplit: return size_t(off_rates_) + (size_t(S_) + 1u) * sizeof(float);
+    case kTabSplit: return size_t(off_rates_) + size_t(bound_words(S_, pair_n1_)) * sizeof(float);
     default: return 0u;
     }
 }
@@ -5080,6 +5173,7 @@ Ensemble::Ensemble(const std::vector<const Model *> &cells, const std::vector<ui
         d.off_stoich = b.off_stoich;
         d.off_rmask = b.off_rmask;
         d.off_bound = b.off_bound;
+        d.pair_n1 = b.pair_n1;
         d.ns = m.num_species();
         d.nr = m.num_reactions();
         d.nreg = m.num_regions();

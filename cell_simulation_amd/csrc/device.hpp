@@ -79,6 +79,7 @@ private:
     uint32_t mod_zero_ = 0xFFFFFFFFu;
     int cur_ = 0;                  // index of the buffer holding the current lattice
     bool has_zero_order_ = false;  // some reaction has no reactants
+    uint32_t pair_n1_ = 0;         // ns + 1 when the table blob carries the pair table of the reaction reject test
     bool simple_ = false;          // one-lane-per-voxel kernel without LDS staging
     bool tiled_ = false;           // block-shared plane-march kernel instead of the wave-march kernel
     int tab_mode_ = 0;             // where the wave-march kernel reads the model tables: all in LDS (0), the hop
