@@ -4448,6 +4448,8 @@ Device::Device(const Model &m, int hip_device) : dev_(hip_device)
         if (v >= 0 && v <= kWEv)
             ev_cap_ = uint32_t(v);
     }
+    if (const char *x = std::getenv("CS_WAVE_PLAN"))  // tuning hook: "cost" (the cost-ordered planner) or "balanced"
+        balanced_plan_ = std::string(x) != "cost";
     if (const char *x = std::getenv("CS_WAVE_ITEMS")) {  // tuning hook: work items per resident block
         const long v = std::strtol(x, nullptr, 10);
         if (v >= 1 && v <= 256)
@@ -4455,13 +4457,17 @@ Device::Device(const Model &m, int hip_device) : dev_(hip_device)
     }
     if (const char *x = std::getenv("CS_WAVE_MIN_PLANES")) {  // tuning hook: shortest x-range of an item
         const long v = std::strtol(x, nullptr, 10);
-        if (v >= 1 && v <= 4096)
+        if (v >= 1 && v <= 4096) {
             xc_min_ = uint32_t(v);
+            min_planes_set_ = true;
+        }
     }
     if (const char *x = std::getenv("CS_TILE_PLANES")) {  // fixed chunk length (tests, tuning)
         const long v = std::strtol(x, nullptr, 10);
-        if (v >= 1 && v <= 4096)
+        if (v >= 1 && v <= 4096) {
             xc_ = xc_min_ = tiled_xc_ = uint32_t(v);
+            balanced_plan_ = false;  // fixed chunks: the cost-ordered planner cuts every column into them
+        }
     }
 
     hip_check(hipMalloc(&lat_[0], N_ * S_), "hipMalloc lattice");
@@ -4898,7 +4904,14 @@ void Device::write_lattice(const uint8_t *in)
     hip_check(hipStreamSynchronize(st), "write lattice");
 }
 
-// Work items of the wave kernel: each column of kWavesPerBlock wave tiles stacked in y is cut along x into
+// A work item of the wave kernel: planes [x0, x1) of column col (block row col / tiles_z, z tile col % tiles_z).
+struct WaveItem {
+    double cost;
+    uint32_t col, x0, x1;
+};
+
+// Work items of the wave kernel. By default plan_balanced_items (below); the cost-ordered plan (CS_WAVE_PLAN=cost,
+// and fixed chunk lengths): each column of kWavesPerBlock wave tiles stacked in y is cut along x into
 // ranges of about equal expected time, queued longest first. A tile's time is a fixed cost per plane plus a
 // cost per particle (about 70 particles' worth per tile and plane, measured), and every range also marches
 // the plane on each side of it; so ranges through dense regions are short (a long one would finish last)
@@ -4925,14 +4938,15 @@ void Device::plan_wave_items(const uint8_t *lat)
     double total = 0.0;
     for (uint64_t v : cnt)
         total += double(v) + plane_cost;
+    if (balanced_plan_) {
+        plan_balanced_items(cnt, plane_cost, total);
+        return;
+    }
     // about `items_per_block_` items per resident block, so the longest ones end with the rest
-    const double target = std::max(total / (double(items_per_block_) * std::max(wave_blocks_, 1u)), 2.0 * plane_cost);
+    const uint32_t per_block = items_per_block_ != 0u ? items_per_block_ : 8u;
+    const double target = std::max(total / (double(per_block) * std::max(wave_blocks_, 1u)), 2.0 * plane_cost);
     const uint32_t min_len = std::min<uint32_t>(xc_min_, R0), max_len = std::max(xc_, min_len);
-    struct Item {
-        double cost;
-        uint32_t col, x0, x1;
-    };
-    std::vector<Item> items;
+    std::vector<WaveItem> items;
     for (uint32_t col = 0; col < tz_n * by_n; ++col) {
         const uint64_t *c = &cnt[size_t(col) * R0];
         uint32_t x0 = 0;
@@ -4947,21 +4961,28 @@ void Device::plan_wave_items(const uint8_t *lat)
             x0 = x1;
         }
     }
-    std::stable_sort(items.begin(), items.end(), [](const Item &p, const Item &q) { return p.cost > q.cost; });
+    std::stable_sort(items.begin(), items.end(), [](const WaveItem &p, const WaveItem &q) { return p.cost > q.cost; });
     // Queue k (XCD k) takes entries k, k + 8, ...: the columns of the k-th band of block rows (all their z tiles and
     // x ranges, cost order within the band), so the z and x neighbours of an item, whose halo lines it reads, run on
     // the same XCD and share its L2 (256^3: 10% fewer bytes read and 4.5% less time than dealing the cost-ordered
     // items round the XCDs, which CS_WAVE_PLACEMENT=rank restores; bands along x measured slower than either).
     // Shorter queues are padded with empty entries; an XCD whose queue is done helps the others.
-    std::vector<std::vector<const Item *>> q(8);
+    std::vector<std::vector<const WaveItem *>> q(8);
     const char *placement = std::getenv("CS_WAVE_PLACEMENT");
     if (placement != nullptr && std::string(placement) == "rank") {
         for (size_t i = 0; i < items.size(); ++i)
             q[i % 8].push_back(&items[i]);
     } else {
-        for (const Item &it : items)
+        for (const WaveItem &it : items)
             q[std::min<uint32_t>(7u, (it.col / tz_n) * 8u / std::max(by_n, 1u))].push_back(&it);
     }
+    upload_items(q);
+}
+
+// Packs the per-XCD queues (entry e of the item array belongs to queue e % 8) and uploads them.
+void Device::upload_items(const std::vector<std::vector<const WaveItem *>> &q)
+{
+    const uint32_t tz_n = wtiles_z_;
     size_t qlen = 0;
     for (const auto &l : q)
         qlen = std::max(qlen, l.size());
@@ -4970,7 +4991,7 @@ void Device::plan_wave_items(const uint8_t *lat)
         for (uint32_t k = 0; k < 8u; ++k) {
             const size_t e = i * 8u + k;
             if (i < q[k].size()) {
-                const Item &it = *q[k][i];
+                const WaveItem &it = *q[k][i];
                 packed[2 * e] = (it.col % tz_n) | ((it.col / tz_n) << 16);
                 packed[2 * e + 1] = it.x0 | (it.x1 << 16);
             } else {
@@ -4992,6 +5013,70 @@ void Device::plan_wave_items(const uint8_t *lat)
     hip_check(hipMemcpyAsync(items_, packed.data(), packed.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st),
               "upload wave items");
     hip_check(hipStreamSynchronize(st), "upload wave items");
+}
+
+// Items of about equal cost, `items_per_block_` per resident block in all (default one each, so every block marches
+// one item and no block waits for a second round). Column c (a block row of tiles at one z tile) gets a number of
+// items in proportion to its cost (particles plus a fixed cost per tile plane, as above), largest remainders first,
+// at least one and at most R0 / xc_min_, and is cut into that many x-ranges of equal cost. Queues: the columns in
+// order (block rows, then z tiles) are dealt to the XCDs in runs of about equal item count, so the y and z neighbours
+// of an item, whose halo lines it reads, mostly run on the same XCD and share its L2; an XCD whose queue is done
+// helps the others.
+void Device::plan_balanced_items(const std::vector<uint64_t> &cnt, double plane_cost, double total)
+{
+    const uint32_t tz_n = wtiles_z_, by_n = (wtiles_y_ + kWavesPerBlock - 1) / kWavesPerBlock;
+    const uint32_t R0 = res_[0], ncol = tz_n * by_n;
+    const uint32_t min_len = std::max(1u, std::min<uint32_t>(min_planes_set_ ? xc_min_ : 6u, R0));
+    const uint32_t max_per_col = std::max(1u, R0 / min_len);
+    const double want = double(items_per_block_ != 0u ? items_per_block_ : 1u) * double(std::max(wave_blocks_, 1u));
+    std::vector<double> ccost(ncol, 0.0);
+    for (uint32_t col = 0; col < ncol; ++col)
+        for (uint32_t x = 0; x < R0; ++x)
+            ccost[col] += double(cnt[size_t(col) * R0 + x]) + plane_cost;
+    // items per column: floor of the share, then the largest remainders
+    std::vector<uint32_t> n(ncol);
+    std::vector<std::pair<double, uint32_t>> rem;
+    double assigned = 0.0;
+    for (uint32_t col = 0; col < ncol; ++col) {
+        const double share = ccost[col] / total * want;
+        n[col] = std::min(max_per_col, std::max(1u, uint32_t(share)));
+        assigned += n[col];
+        rem.push_back({share - double(n[col]), col});
+    }
+    std::stable_sort(rem.begin(), rem.end(), [](const auto &p, const auto &q) { return p.first > q.first; });
+    for (size_t i = 0; i < rem.size() && assigned + 1.0 <= want; ++i)
+        if (rem[i].first > 0.0 && n[rem[i].second] < max_per_col) {
+            ++n[rem[i].second];
+            assigned += 1.0;
+        }
+    std::vector<WaveItem> items;
+    for (uint32_t col = 0; col < ncol; ++col) {
+        const uint64_t *c = &cnt[size_t(col) * R0];
+        const double step = ccost[col] / double(n[col]);
+        double acc = 0.0;
+        uint32_t x0 = 0, x = 0;
+        for (uint32_t k = 1; k <= n[col]; ++k) {
+            if (k == n[col]) {
+                x = R0;
+            } else {
+                const double cut = step * double(k);
+                // at least min_len planes per range, and min_len left for each range after it
+                while (x < R0 && (x - x0 < min_len || acc + (double(c[x]) + plane_cost) * 0.5 <= cut) &&
+                       R0 - (x + 1u) >= (n[col] - k) * min_len) {
+                    acc += double(c[x]) + plane_cost;
+                    ++x;
+                }
+            }
+            if (x > x0)
+                items.push_back({0.0, col, x0, x});
+            x0 = x;
+        }
+    }
+    // deal to the XCD queues in column order, runs of about equal length
+    std::vector<std::vector<const WaveItem *>> q(8);
+    for (size_t i = 0; i < items.size(); ++i)
+        q[std::min<size_t>(7u, i * 8u / std::max<size_t>(items.size(), 1u))].push_back(&items[i]);
+    upload_items(q);
 }
 
 void Device::read_cme_counts(int32_t *out)

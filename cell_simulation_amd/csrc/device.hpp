@@ -26,6 +26,7 @@ struct SkewVoxel {
 };
 
 struct SkewStore;  // the device table of skewed voxels and the replay scratch (device.hip)
+struct WaveItem;   // a work item of the wave kernel (device.hip)
 
 class Device {
 public:
@@ -68,6 +69,8 @@ public:
 
 private:
     void plan_wave_items(const uint8_t *lat);
+    void plan_balanced_items(const std::vector<uint64_t> &cnt, double plane_cost, double total);
+    void upload_items(const std::vector<std::vector<const WaveItem *>> &q);
 
     int dev_ = 0;
     void *stream_ = nullptr;       // hipStream_t
@@ -93,7 +96,10 @@ private:
     uint32_t n_items_ = 0;
     uint32_t xc_min_ = 12;         // shortest x-range of a work item (its two side planes cost as much as
                                    // 1/6 of it; a longer one through a dense column would finish last)
-    uint32_t items_per_block_ = 8; // wave-kernel work items per resident block (planning target)
+    uint32_t items_per_block_ = 0; // wave-kernel work items per resident block (planning target; 0: the planner's
+                                   // default, 1 balanced, 8 cost-ordered)
+    bool balanced_plan_ = true;    // items of equal cost, one per resident block (else cost-ordered, shortest 12)
+    bool min_planes_set_ = false;  // xc_min_ given by the environment
     uint32_t tiled_xc_ = 16;       // planes marched per block by the block-tiled kernel
 
     uint8_t *lat_[2] = {nullptr, nullptr};
