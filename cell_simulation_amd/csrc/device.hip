@@ -3053,23 +3053,23 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
         // ---- prefetch plane X+1's slots and plane X+2's labels (clamped; unused past the chunk)
         {
             // planes past xe are never listed nor gathered; on the side plane xe only the interior rows are
-            // listed (a particle there reaches the tile with a hop along x or not at all)
+            // listed (a particle there reaches the tile with a hop along x or not at all), and a plane the
+            // summaries mark empty is not listed either. The loads are issued on every plane all the same, the
+            // ones not needed at one line (the lattice's first record, the label volume's first word), and their
+            // registers are then never read: with the same number of loads on every path, the wait for the
+            // previous plane's registers below leaves all of this plane's loads in flight (with some loads on some
+            // paths only, the compiler has to assume the fewest and waits for part of this prefetch too).
             const uint32_t sb = slot_base(X + 1), lb = lab_base(X + 2);
-            if (X + 1 <= xe && (!summ || pact(X + 1))) {
-                const int nb = X + 1 == xe ? kWY : kWBatches;
+            const bool go = X + 1 <= xe && (!summ || pact(X + 1));
+            const int nb = !go ? 0 : (X + 1 == xe ? kWY : kWBatches);
 #pragma unroll
-                for (int k = 0; k < kWBatches; ++k)
-                    fill.rec[k] = k < nb ? load_head<S>(a.src, sb + off[k]) : HVec{};
-            } else {
+            for (int k = 0; k < kWBatches; ++k)
+                fill.rec[k] = load_head<S>(a.src, k < nb ? sb + off[k] : 0u);
+            // labels up to plane xe + 1 (the hops of plane xe look at them)
+            const bool lgo = X + 1 <= xe;
 #pragma unroll
-                for (int k = 0; k < kWBatches; ++k)
-                    fill.rec[k] = HVec{};  // no particle in any record of that plane
-            }
-            if (X + 1 <= xe) {  // labels up to plane xe + 1 (the hops of plane xe look at them)
-#pragma unroll
-                for (int i = 0; i < kWLabLane; ++i)
-                    fill.lab[i] = w.regp[lb + loff[i]];
-            }
+            for (int i = 0; i < kWLabLane; ++i)
+                fill.lab[i] = w.regp[lgo ? lb + loff[i] : 0u];
         }
         store_lab(X + 1, use.lab);
         // plane X+1's event list starts empty (it was consumed as plane X-2's); plane X's leave masks too
