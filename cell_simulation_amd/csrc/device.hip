@@ -96,12 +96,20 @@ struct StepArgs {
     uint32_t mod_zero;
     float tau;
     const uint32_t *live;      // bit per voxel (index vox_base + v): a skewed voxel, whose reaction the replay runs
+                               // (null in the wave kernels while the skewed-voxel table is empty: no bit is set)
+    const uint32_t *skew_n;    // entries of the skewed-voxel table (no entry is added during a step kernel)
     SumRef sum_dst;            // summary of dst, marked by the passes after the step kernel
 };
 
 __device__ __forceinline__ bool voxel_bit(const uint32_t *bits, uint32_t v)
 {
     return (bits[v >> 5] >> (v & 31u)) & 1u;
+}
+
+// A skewed voxel, whose reaction the replay kernel runs; false for every voxel when a.live is null.
+__device__ __forceinline__ bool live_voxel(const StepArgs &a, uint32_t v)
+{
+    return a.live != nullptr && voxel_bit(a.live, v);
 }
 
 __device__ __forceinline__ void wave_sync()
@@ -416,7 +424,7 @@ __device__ __forceinline__ void react(const StepArgs &a, const Tab &tb, Slots<S>
     // above the bound's reject line is above the exact one below as well: nothing fires, nothing is near.
     if (r1 > tb.bound[cnt] * a.tau * 1.001f + 1e-6f)
         return;
-    if (voxel_bit(a.live, a.vox_base + z + y * a.R2 + x * a.R12))
+    if (live_voxel(a, a.vox_base + z + y * a.R2 + x * a.R12))
         return;  // a skewed voxel: its reaction runs in the replay kernel after the step
     const uint32_t words = wave_words<S>(cnt);  // slot words holding particles in any lane still here
 
@@ -2886,7 +2894,7 @@ __device__ __forceinline__ void count_idle_fires(const StepArgs &a, uint32_t tz,
             const uint32_t zfire = 0u - a.itime - uint32_t(Q) * uint32_t(y0 + r) - uint32_t(z0);
             bool f = Q < xe && zfire < uint32_t(kWZ) && ((im >> (zfire & 63u)) & 1ull) != 0ull;
             if (f)  // a skewed voxel's reaction runs in the replay kernel
-                f = !voxel_bit(a.live, a.vox_base + z0 + zfire + uint32_t(y0 + r) * a.R2 + uint32_t(Q) * a.R12);
+                f = !live_voxel(a, a.vox_base + z0 + zfire + uint32_t(y0 + r) * a.R2 + uint32_t(Q) * a.R12);
             nf += uint32_t(__popcll(__ballot(f)));
         }
     }
@@ -3050,6 +3058,9 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
     };
 
     auto plane = [&](int X, WaveStaged<S> &use, WaveStaged<S> &fill) __attribute__((always_inline)) {
+        // The per-lane record bits are taken as new on every plane: otherwise the compiler hoists one lane mask per
+        // record batch out of the march into scalar registers, which then spill and are read back on every plane.
+        asm volatile("" : "+v"(okm), "+v"(inl));
         // ---- prefetch plane X+1's slots and plane X+2's labels (clamped; unused past the chunk)
         {
             // planes past xe are never listed nor gathered; on the side plane xe only the interior rows are
@@ -3213,7 +3224,7 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
                 // an empty voxel's zero-propensity draw fires at seed 0 (itime + x*y + z == 0)
                 const uint32_t zfire = 0u - a.itime - uint32_t(Q) * uint32_t(y0 + r) - uint32_t(z0);
                 if (l == 0 && zfire < uint32_t(kWZ) &&
-                    !voxel_bit(a.live, a.vox_base + z0 + zfire + uint32_t(y0 + r) * a.R2 + uint32_t(Q) * a.R12))
+                    !live_voxel(a, a.vox_base + z0 + zfire + uint32_t(y0 + r) * a.R2 + uint32_t(Q) * a.R12))
                     n_fired += uint32_t((inm & ~m) >> zfire) & 1u;
                 const bool need = (m >> l) & 1ull;
                 const bool occ = (occ_prev >> r) & 1u;
@@ -3419,8 +3430,11 @@ constexpr int kTabLds = 0, kTabSplit = 1, kTabGlobal = 2;
 
 template <int S, int TM>
 __global__ __launch_bounds__(kWaveBlock) __attribute__((amdgpu_waves_per_eu(3)))
-void rdme_step_wave(StepArgs a, WaveArgs w)
+void rdme_step_wave(StepArgs a0, WaveArgs w)
 {
+    StepArgs a = a0;
+    if (*a0.skew_n == 0u)
+        a.live = nullptr;  // no skewed voxel: the live-bit loads (and their waits) are skipped
     __shared__ WaveLds<S> L[kWavesPerBlock];
     extern __shared__ uint32_t tab_lds[];
     const uint8_t *gbase = reinterpret_cast<const uint8_t *>(w.tab);
@@ -3701,6 +3715,7 @@ struct EnsArgs {
     uint32_t *fix_ctl, *fix_list;
     uint32_t max_tight;
     const uint32_t *live;         // skewed voxels (bit per voxel of every cell)
+    const uint32_t *skew_n;       // entries of the skewed-voxel table
     uint8_t *sum_src, *sum_dst;   // occupancy summaries of src and dst (every cell's, at EnsCell::sum_off)
     const uint32_t *item_cell;    // [total_items] the cell of each work item
 };
@@ -3749,6 +3764,7 @@ __device__ __forceinline__ StepArgs cell_step_args(const EnsArgs &e, const EnsCe
     a.mod_zero = c.mod_zero;
     a.tau = c.tau;
     a.live = e.live;
+    a.skew_n = e.skew_n;
     a.sum_dst = cell_sum(e.sum_dst, c);
     return a;
 }
@@ -3830,7 +3846,9 @@ __global__ __launch_bounds__(kWaveBlock) void rdme_step_ensemble(EnsArgs e)
         for (uint32_t item = first; item < last; ++item) {
             const uint32_t lo = __builtin_amdgcn_readfirstlane(e.item_cell[item]);
             const EnsCell &c = e.cells[lo];
-            const StepArgs a = cell_step_args(e, c, e.itime[lo]);
+            StepArgs a = cell_step_args(e, c, e.itime[lo]);
+            if (*e.skew_n == 0u)
+                a.live = nullptr;  // no skewed voxel: the live-bit loads are skipped
             const WaveArgs w = cell_wave_args(e, c);
             const uint32_t local = item - c.first_item;
             const uint32_t tz = local % c.tiles_z, by = (local / c.tiles_z) % c.btiles_y;
@@ -4678,6 +4696,7 @@ void Device::step(uint32_t itime)
     a.mod_zero = mod_zero_;
     a.tau = tau_;
     a.live = skew_->k.live;
+    a.skew_n = skew_->k.ctl;
     a.sum_dst = SumRef{summ_ ? sum_[cur_ ^ 1] : nullptr, res_[0], wtiles_z_, res_[1], res_[2]};
     const SumRef sum_src{summ_ ? sum_[cur_] : nullptr, res_[0], wtiles_z_, res_[1], res_[2]};
     hipStream_t st = static_cast<hipStream_t>(stream_);
@@ -4807,6 +4826,7 @@ void Device::render_view(uint32_t view, uint32_t axis, uint32_t slice, float *ou
     a.N = uint32_t(N_);
     a.vox_base = 0u;
     a.live = skew_->k.live;
+    a.skew_n = skew_->k.ctl;
     const SkewTab k = skew_->k;
     const uint32_t blocks = uint32_t((N_ + kBlock - 1) / kBlock);
     switch (S_) {
@@ -5500,6 +5520,7 @@ void Ensemble::enqueue_step()
     e.fix_list = skew_->b.T;
     e.max_tight = skew_->b.tcap;
     e.live = skew_->k.live;
+    e.skew_n = skew_->k.ctl;
     const SkewTab k = skew_->k;
     const ReplayBuf rb = skew_->b;
     const uint32_t pro_blocks = (n_cells() + 63u) / 64u;
