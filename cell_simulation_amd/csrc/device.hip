@@ -2610,6 +2610,7 @@ constexpr int kWLY = kWY + 4;                     // label rows y0-2 .. y0+kWY+1
 constexpr int kWLZ = kWZ + 4;                     // label columns z0-2 .. z0+kWZ+1 (68 bytes, 17 words)
 constexpr int kWLabWords = kWLY * kWLZ / 4;       // 136
 constexpr int kWBatches = kWY + 3;                // records per lane: interior rows, rows -1 and kWY, z halo
+static_assert(kWBatches < 8 && kWY < 8, "wave_offset takes per-lane counts below 8");
 constexpr int kWHop = kWY * kWZ + 2 * kWZ + 2 * kWY;  // records a plane can list (392)
 constexpr int kWEv = 96;                          // arrival events per destination plane
 constexpr int kWReact = 80;                       // deferred reactions a wave holds (runs of up to 64 once 16 wait)
@@ -2664,6 +2665,18 @@ __device__ __forceinline__ void vm_drain()
 }
 
 __device__ __forceinline__ int ring3(int x) { return (x + 3) % 3; }
+
+// Exclusive prefix sum over the wave of per-lane counts c < 8 (three bit planes, one ballot each); `total` gets
+// the wave's sum.
+__device__ __forceinline__ uint32_t wave_offset(uint32_t c, uint32_t &total)
+{
+    const uint64_t b0 = __ballot(c & 1u), b1 = __ballot(c & 2u), b2 = __ballot(c & 4u);
+    auto below = [](uint64_t m) {
+        return __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
+    };
+    total = uint32_t(__popcll(b0)) + 2u * uint32_t(__popcll(b1)) + 4u * uint32_t(__popcll(b2));
+    return below(b0) + 2u * below(b1) + 4u * below(b2);
+}
 
 template <int S>
 __device__ __forceinline__ bool vec_nonzero(const typename SlotVec<S>::T &v)
@@ -3099,31 +3112,32 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
         // ---- T0: list the occupied records of plane X
         uint32_t n = 0u;
         const bool hop_plane = X >= 0 && X < R0 && X <= xe && (!summ || pact(X));
-        // on the planes beside the x-range only the interior rows matter: a particle there reaches the
-        // tile with a hop along x or not at all
-        const int batches = (X == xb - 1 || X == xe) ? kWY : kWBatches;
         if (hop_plane) {
+            // on the planes beside the x-range only the interior rows matter: a particle there reaches the
+            // tile with a hop along x or not at all
+            const uint32_t bmask = (X == xb - 1 || X == xe) ? (1u << kWY) - 1u : (1u << kWBatches) - 1u;
+            // this lane's occupied records as bits; each lane writes its records to consecutive list entries,
+            // placed by one exclusive prefix sum of the per-lane counts (list order is free: see T1)
+            uint32_t om = 0u;
+#pragma unroll
+            for (int k = 0; k < kWBatches; ++k)
+                om |= uint32_t(head_nonzero<S>(use.rec[k])) << k;
+            om &= okm & bmask;
+            uint32_t pos = wave_offset(__popc(om), n);
 #pragma unroll
             for (int k = 0; k < kWBatches; ++k) {
-                if (k >= batches)
-                    break;
-                const bool occ = ((okm >> k) & 1u) && head_nonzero<S>(use.rec[k]);
-                const uint64_t m = __ballot(occ);
-                const uint32_t rank =
-                    __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
-                if (occ) {
+                if ((om >> k) & 1u) {
                     uint32_t rec;
                     if (k < kWY) rec = uint32_t((k + 1) * kWRZ + l + 1);
                     else if (k == kWY) rec = uint32_t(l + 1);
                     else if (k == kWY + 1) rec = uint32_t((kWY + 1) * kWRZ + l + 1);
                     else rec = zrec;
-                    L.hop_slots[n + rank] = use.rec[k];
-                    L.hop_rec[n + rank] = uint16_t(rec | (head_full<S>(use.rec[k]) ? 0x8000u : 0u));
+                    L.hop_slots[pos] = use.rec[k];
+                    L.hop_rec[pos] = uint16_t(rec | (head_full<S>(use.rec[k]) ? 0x8000u : 0u));
+                    ++pos;
                 }
-                n += uint32_t(__popcll(m));
-                if (k < kWY)
-                    occ_cur |= uint32_t(occ) << k;
             }
+            occ_cur |= om & ((1u << kWY) - 1u);
         }
         wave_sync();
 
@@ -3216,28 +3230,34 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
             // the voxels to gather, as wave-wide row masks: occupied, or a particle arrives (everything when the
             // event list overflowed or empty voxels can react)
             const uint64_t every = (over || w.has_zero_order != 0u) ? ~0ull : 0ull;
-            uint32_t ns = 0u, needm = 0u, sidx[kWY];
+            uint32_t ns = 0u, needm = every != 0ull ? (1u << kWY) - 1u : occ_prev, sidx[kWY];
+#pragma unroll
+            for (int r = 0; r < kWY; ++r)
+                needm |= uint32_t(L.arrive[rq][r] >> l & 1ull) << r;
+            needm &= inl;
 #pragma unroll
             for (int r = 0; r < kWY; ++r) {
-                const uint64_t inm = __ballot((inl >> r) & 1u);
-                const uint64_t m = inm & (__ballot((occ_prev >> r) & 1u) | L.arrive[rq][r] | every);
                 // an empty voxel's zero-propensity draw fires at seed 0 (itime + x*y + z == 0)
                 const uint32_t zfire = 0u - a.itime - uint32_t(Q) * uint32_t(y0 + r) - uint32_t(z0);
-                if (l == 0 && zfire < uint32_t(kWZ) &&
-                    !live_voxel(a, a.vox_base + z0 + zfire + uint32_t(y0 + r) * a.R2 + uint32_t(Q) * a.R12))
-                    n_fired += uint32_t((inm & ~m) >> zfire) & 1u;
-                const bool need = (m >> l) & 1ull;
-                const bool occ = (occ_prev >> r) & 1u;
-                const uint32_t rank =
-                    __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
-                sidx[r] = ns + rank;
-                needm |= uint32_t(need) << r;
-                if (need) {
-                    const bool full = occ && head_full<S>(prev[r]);
-                    L.slow[ns + rank] = uint16_t(uint32_t(r * kWZ + l) | (uint32_t(full) << 12));
-                    L.hop_slots[ns + rank] = occ ? prev[r] : HVec{};  // the hop list is done with
+                if (zfire < uint32_t(kWZ)) {
+                    const uint64_t inm = __ballot((inl >> r) & 1u), m = __ballot((needm >> r) & 1u);
+                    if (l == 0 &&
+                        !live_voxel(a, a.vox_base + z0 + zfire + uint32_t(y0 + r) * a.R2 + uint32_t(Q) * a.R12))
+                        n_fired += uint32_t((inm & ~m) >> zfire) & 1u;
                 }
-                ns += uint32_t(__popcll(m));
+            }
+            // the voxels to gather, this lane's consecutively (one exclusive prefix sum of the counts)
+            uint32_t pos = wave_offset(__popc(needm), ns);
+#pragma unroll
+            for (int r = 0; r < kWY; ++r) {
+                sidx[r] = pos;
+                if ((needm >> r) & 1u) {
+                    const bool occ = (occ_prev >> r) & 1u;
+                    const bool full = occ && head_full<S>(prev[r]);
+                    L.slow[pos] = uint16_t(uint32_t(r * kWZ + l) | (uint32_t(full) << 12));
+                    L.hop_slots[pos] = occ ? prev[r] : HVec{};  // the hop list is done with
+                    ++pos;
+                }
             }
             wave_sync();
             const uint16_t *lvq = reinterpret_cast<const uint16_t *>(L.leave[Q & 1]);
