@@ -5066,9 +5066,22 @@ void Device::plan_balanced_items(const std::vector<uint64_t> &cnt, double plane_
 {
     const uint32_t tz_n = wtiles_z_, by_n = (wtiles_y_ + kWavesPerBlock - 1) / kWavesPerBlock;
     const uint32_t R0 = res_[0], ncol = tz_n * by_n;
-    const uint32_t min_len = std::max(1u, std::min<uint32_t>(min_planes_set_ ? xc_min_ : 6u, R0));
+    // at least 6 planes per item; 12 on a lattice of fewer than one particle per thousand voxels, whose items are
+    // mostly skipped through the occupancy summaries and cost their fixed part (an empty 128^3 cell: 43 us per step
+    // with 6-plane items, 26 with 12; a populated 64^3 cell: 47 and 72)
+    uint64_t particles = 0;
+    for (uint64_t v : cnt)
+        particles += v;
+    const uint32_t min_default = particles * 1000u < N_ ? 12u : 6u;
+    const uint32_t min_len = std::max(1u, std::min<uint32_t>(min_planes_set_ ? xc_min_ : min_default, R0));
     const uint32_t max_per_col = std::max(1u, R0 / min_len);
-    const double want = double(items_per_block_ != 0u ? items_per_block_ : 1u) * double(std::max(wave_blocks_, 1u));
+    // items per resident block: one while an item holds up to about 24 planes per tile column, more on larger
+    // lattices (an item of hundreds of planes would leave the cost model's error as a tail)
+    const double blocks = double(std::max(wave_blocks_, 1u));
+    const uint32_t per_block =
+        items_per_block_ != 0u ? items_per_block_
+                               : std::max(1u, uint32_t(std::lround(double(ncol) * double(R0) / (blocks * 24.0))));
+    const double want = double(per_block) * blocks;
     std::vector<double> ccost(ncol, 0.0);
     for (uint32_t col = 0; col < ncol; ++col)
         for (uint32_t x = 0; x < R0; ++x)
