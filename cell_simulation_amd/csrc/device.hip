@@ -182,6 +182,21 @@ __device__ __forceinline__ uint32_t count_nonzero(const Slots<S> &s)
 
 // number of slots holding species sp (sp >= 1); only the first `words` slot words are examined (a
 // wave-uniform bound: the slots are compacted, so words past every lane's occupancy hold no particle)
+// count_nonzero of compacted slots: at S = 16 the upper eight hold particles only when the lower eight are full,
+// so they are counted only then (no lane of a pass holds a full head, as a rule: eight fewer vector operations)
+template <int S>
+__device__ __forceinline__ uint32_t count_compacted(const Slots<S> &s)
+{
+    if constexpr (S == 16) {
+        uint32_t n = uint32_t(__builtin_popcount(nonzero_bytes(s.w[0])) + __builtin_popcount(nonzero_bytes(s.w[1])));
+        if (n == 8u)
+            n += uint32_t(__builtin_popcount(nonzero_bytes(s.w[2])) + __builtin_popcount(nonzero_bytes(s.w[3])));
+        return n;
+    } else {
+        return count_nonzero(s);
+    }
+}
+
 template <int S>
 __device__ __forceinline__ uint32_t count_species(const Slots<S> &s, uint32_t sp, uint32_t words = S / 4)
 {
@@ -3181,7 +3196,7 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
                 const uint32_t xmr = __shfl(xm, rho + 1);
                 const uint32_t xmy = Y == 0u ? xm_zero : xmr;
                 h.seed0 = a.itime + xmy + Z + (Z + __umul24(Y, a.R2) + uint32_t(X) * a.R12) * uint32_t(S);
-                const uint32_t occ = act ? count_nonzero(sl) : 0u;
+                const uint32_t occ = act ? count_compacted(sl) : 0u;
                 const bool interior = rho >= 0 && rho < kWY && zeta >= 0 && zeta < kWZ;
                 uint32_t lv = 0u;
                 for (uint32_t j = 0;; ++j) {
@@ -3277,7 +3292,7 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
                         vm_drain();
                     }
                 }
-                const uint32_t occ = count_nonzero(own);
+                const uint32_t occ = count_compacted(own);
                 const uint32_t lv = act ? lvq[me] : 0u;
                 // Most voxels end with at most four particles: then the arrivals and the assembled slots fit
                 // one 32-bit word, and the wave takes that path when every lane fits it.
