@@ -1027,6 +1027,8 @@ struct ReplayBuf {
     uint32_t *mark_hi;  // bit per 1024 voxels: a list longer than lds_sort_max is sorted through them (left clear)
     uint32_t nvox;
     uint32_t lds_sort_max;  // longest list sorted in LDS (kSortLds; lowered by a test hook)
+    uint32_t wave_max;      // clusters of up to this many noted voxels are replayed by one wave (kWaveReplayT),
+    uint32_t block_max;     // up to this many by the whole block (kBlockReplayT), larger ones by one lane
     uint32_t *F;        // [fcap] the voxels each cluster replays, ascending
     uint8_t *FL;        // [fcap * S] their slots during the replay
     uint32_t *FO;       // [fcap] their occupancy words during the replay
@@ -1584,6 +1586,350 @@ __device__ int replay_cluster_wave(const StepArgs &a, const SkewTab &k, WaveRepl
     return 1;
 }
 
+// Ascending sort of arr[0..n) in place by bitonic merging, the whole block taking part (arr padded to n2, a
+// power of two, with 0xFFFFFFFF).
+template <class K>
+__device__ void block_bitonic(K *arr, uint32_t n, uint32_t n2)
+{
+    for (uint32_t i = n + threadIdx.x; i < n2; i += blockDim.x)
+        arr[i] = K(~K(0));
+    __syncthreads();
+    for (uint32_t kk = 2; kk <= n2; kk <<= 1)
+        for (uint32_t jj = kk >> 1; jj > 0; jj >>= 1) {
+            for (uint32_t i = threadIdx.x; i < n2; i += blockDim.x) {
+                const uint32_t l = i ^ jj;
+                if (l > i) {
+                    const K p = arr[i], q = arr[l];
+                    const bool up = (i & kk) == 0u;
+                    if (up ? p > q : p < q) {
+                        arr[i] = q;
+                        arr[l] = p;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+}
+
+// The same replay by the whole block, for clusters of more than kWaveReplayT and at most kBlockReplayT noted
+// voxels whose F holds at most kBlockReplayF voxels (W then at most seven times that) and whose sources make at
+// most kBlockReplayEv events, the arrays in the block's LDS: the threads load F's state, sort F and W, compute the
+// hops of W and keep the moves with an end in F as events, sort the events into the canonical order, and thread 0
+// applies them; the threads then run the reactions and write F's outputs. F indices take 16 bits here: an event
+// carries the source's W index, the walk position, the direction, the species and the target's F index, and the
+// applying thread finds the voxel after the target and a walk's block past slot S in F by binary search (both
+// rare). Returns as replay_cluster_wave (-1 also when F or the events pass their capacity).
+constexpr uint32_t kBlockReplayT = 512;
+constexpr uint32_t kBlockReplayF = 1024;
+constexpr uint32_t kBlockReplayW = 7u * kBlockReplayF;
+constexpr uint32_t kBlockReplayEv = 4096;
+constexpr uint32_t kBNoF = 0xFFFFu;
+static_assert(kBlockReplayW < (1u << 24) && kBlockReplayF < kBNoF, "block replay event fields");
+static_assert(kBlockReplayEv * sizeof(uint64_t) >= 8192u * sizeof(uint32_t) && 8192u >= kBlockReplayW &&
+                  8192u >= 8u * kBlockReplayT + kEscMax,
+              "the candidate lists (padded to powers of two) fit the event list");
+
+template <int S>
+struct BlockReplayLds {
+    uint64_t ev[kBlockReplayEv];            // events: (W index << 8 | walk position << 3 | direction) << 32 |
+                                            // species | F index of target << 8; first the candidate lists of F, W
+    uint32_t F[kBlockReplayF];
+    uint32_t FL[kBlockReplayF * (S / 4)];   // F's slots during the replay
+    uint32_t FO[kBlockReplayF];             // F's occupancy words during the replay
+    uint32_t FW[kBlockReplayF];             // F's occupancy words when the step began
+    uint32_t FS[kBlockReplayF];             // species bytes of F's count offsets
+    int32_t FD[kBlockReplayF * kSkewSp];    // the count offsets
+    uint32_t W[kBlockReplayW];              // the sources, ascending
+    uint16_t WF[kBlockReplayW];             // F index of each source (kBNoF: none)
+    uint32_t T[kBlockReplayT];              // the noted voxels
+    uint32_t ext[kEscMax];                  // voxels added to F, ascending
+    uint32_t scan[kReplayThreads / 64u];    // per-wave sums of block_offset
+    uint32_t n_ev, n_ext, esc, n_out;
+};
+
+// Exclusive prefix sum of c over the block (threads in order); total gets the block's sum. Every thread calls it.
+__device__ uint32_t block_offset(uint32_t c, uint32_t *wave_sums, uint32_t &total)
+{
+    const uint32_t l = __lane_id(), wv = threadIdx.x / 64u, nw = blockDim.x / 64u;
+    uint32_t x = c;
+    for (uint32_t d = 1; d < 64u; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d);
+        if (l >= d)
+            x += y;
+    }
+    if (l == 63u)
+        wave_sums[wv] = x;
+    __syncthreads();
+    uint32_t off = x - c;
+    total = 0u;
+    for (uint32_t w = 0; w < nw; ++w) {
+        const uint32_t t = wave_sums[w];
+        off += w < wv ? t : 0u;
+        total += t;
+    }
+    __syncthreads();
+    return off;
+}
+
+// Sorts cand[0, n) (room for pow2_at_least(n) entries), drops repeats and 0xFFFFFFFF, writes the rest to out in
+// ascending order (the first cap of them); returns their number. Every thread calls it.
+__device__ uint32_t block_sort_unique(uint32_t *cand, uint32_t n, uint32_t *out, uint32_t cap, uint32_t *wave_sums)
+{
+    block_bitonic<uint32_t>(cand, n, pow2_at_least(n));
+    const uint32_t nth = blockDim.x, per = (n + nth - 1u) / nth;
+    const uint32_t b0 = min(threadIdx.x * per, n), e0 = min(b0 + per, n);
+    uint32_t c = 0;
+    for (uint32_t i = b0; i < e0; ++i)
+        c += cand[i] != 0xFFFFFFFFu && (i == 0u || cand[i - 1u] != cand[i]);
+    uint32_t total;
+    uint32_t o = block_offset(c, wave_sums, total);
+    for (uint32_t i = b0; i < e0; ++i)
+        if (cand[i] != 0xFFFFFFFFu && (i == 0u || cand[i - 1u] != cand[i])) {
+            if (o < cap)
+                out[o] = cand[i];
+            ++o;
+        }
+    __syncthreads();
+    return total;
+}
+
+template <int S>
+__device__ int replay_cluster_block(const StepArgs &a, const SkewTab &k, BlockReplayLds<S> &L, const uint32_t *T,
+                                    uint32_t nt, uint32_t xlo_t, ReplayTally &tally)
+{
+    constexpr uint32_t NW = S / 4;
+    const uint32_t tid = threadIdx.x, nth = blockDim.x;
+    const Tab tb = global_tab(a);
+    const uint32_t base = a.vox_base;
+    const int xlo = int(xlo_t) - kEscX, xhi = int(T[nt - 1u] / a.R12) + kEscX;
+    for (uint32_t i = tid; i < nt; i += nth)
+        L.T[i] = T[i];
+    if (tid == 0u)
+        L.n_ext = 0u;
+    __syncthreads();
+    long long failed = 0, alias = 0;  // thread 0
+    bool bad = false;
+    uint32_t nF = 0;
+    bool done = false;
+    for (int round = 0; round <= kEscMax && !done; ++round) {
+        // F: the noted voxels' neighbours, themselves and the voxels after them, and the voxels added
+        uint32_t *cand = reinterpret_cast<uint32_t *>(L.ev);
+        const uint32_t n_ext = L.n_ext;
+        for (uint32_t i = tid; i < nt; i += nth)
+            hood_list(a, L.T[i], 8u, cand + 8u * i);
+        for (uint32_t i = tid; i < n_ext; i += nth)
+            cand[8u * nt + i] = L.ext[i];
+        __syncthreads();
+        nF = block_sort_unique(cand, 8u * nt + n_ext, L.F, kBlockReplayF, L.scan);
+        if (nF > kBlockReplayF)
+            return -1;  // uniform
+        if (tid == 0u)
+            L.n_ev = 0u;
+        if (int(L.F[0] / a.R12) < xlo || int(L.F[nF - 1u] / a.R12) > xhi)
+            return 0;  // uniform: every thread reads the same F
+        for (uint32_t i = tid; i < nF; i += nth) {
+            const uint32_t v = L.F[i];
+            const Slots<S> sv = load_slots<S>(a.src, v);
+            for (uint32_t q = 0; q < NW; ++q)
+                L.FL[i * NW + q] = sv.w[q];
+            const int ei = skew_find(k, base + v);
+            const SkewEntry e = ei >= 0 ? k.ent[ei] : SkewEntry{0u, 0, 0u, 0u, {0, 0, 0, 0}};
+            L.FO[i] = L.FW[i] = count_nonzero(sv) + uint32_t(e.off);
+            L.FS[i] = e.sp;
+            for (int q = 0; q < kSkewSp; ++q)
+                L.FD[i * kSkewSp + q] = e.dc[q];
+        }
+        for (uint32_t i = tid; i < nF; i += nth)
+            hood_list(a, L.F[i], 7u, cand + 7u * i);
+        __syncthreads();
+        const uint32_t nW = block_sort_unique(cand, 7u * nF, L.W, kBlockReplayW, L.scan);  // nW <= 7 nF
+        // the moves of the sources W = F and its six neighbours that have an end in F
+        for (uint32_t wi = tid; wi < nW; wi += nth) {
+            const uint32_t w = L.W[wi];
+            const int wF = find_sorted(L.F, nF, w);
+            L.WF[wi] = uint16_t(wF >= 0 ? uint32_t(wF) : kBNoF);
+            const Slots<S> sw = load_slots<S>(a.src, w);
+            // the walk: occupancy word at the step's start, with the reference's u32 end index
+            const uint32_t occw = wF >= 0 ? L.FW[wF] : count_nonzero(sw);
+            const uint32_t start = w * uint32_t(S), end = start + occw;
+            uint32_t walk = end > start ? end - start : 0u;
+            if (walk == 0u)
+                continue;
+            if (walk > 2u * uint32_t(S)) {  // a word wrapped below zero at the first voxels: unrepresentable
+                bad = true;
+                walk = 2u * uint32_t(S);
+            }
+            const uint32_t z = w % a.R2, y = (w / a.R2) % a.R1, x = w / a.R12;
+            const Hood h = make_hood(a, x, y, z, w, S);
+            const Slots<S> sn = (walk > uint32_t(S) && w + 1u < a.N) ? load_slots<S>(a.src, w + 1u) : zero_slots<S>();
+            for (uint32_t s = 0; s < walk; ++s) {
+                const uint32_t blk = w + s / uint32_t(S), j = s % uint32_t(S);
+                const uint32_t sp = blk >= a.N ? 0u : (blk == w ? get_byte_dyn(sw, j) : get_byte_dyn(sn, j));
+                if (sp == 0u)
+                    continue;
+                const uint32_t d = hop(a, tb, h, sp, s);
+                if (d >= 6u)
+                    continue;
+                const int tF = find_sorted(L.F, nF, neighbor_or_none(a, w, int(d)));
+                if (tF < 0 && wF < 0)
+                    continue;  // neither end in F: the gather has it
+                const uint32_t idx = atomicAdd(&L.n_ev, 1u);
+                if (idx < kBlockReplayEv)
+                    L.ev[idx] = (uint64_t((wi << 8) | (s << 3) | d) << 32) |
+                                (sp | (tF >= 0 ? uint32_t(tF) : kBNoF) << 8);
+            }
+        }
+        __syncthreads();
+        const uint32_t n_ev = L.n_ev;
+        if (n_ev > kBlockReplayEv)
+            return -1;  // uniform
+        if (n_ev > 1u)
+            block_bitonic<uint64_t>(L.ev, n_ev, pow2_at_least(n_ev));
+        __syncthreads();
+        // the claims and departures in the canonical order (the scatter form of rdme.wgsl:45-88), on one thread
+        if (tid == 0u) {
+            failed = alias = 0;
+            uint32_t esc = 0xFFFFFFFFu;
+            uint8_t *FLb = reinterpret_cast<uint8_t *>(L.FL);
+            for (uint32_t i = 0; i < n_ev; ++i) {
+                const uint64_t e = L.ev[i];
+                const uint32_t key = uint32_t(e >> 32), lo = uint32_t(e);
+                const uint32_t wi = key >> 8, s = (key >> 3) & 31u, d = key & 7u;
+                const uint32_t sp = lo & 0xFFu, tF = lo >> 8;
+                const uint32_t wF = L.WF[wi];
+                if (tF != kBNoF) {
+                    const uint32_t old = L.FO[tF];
+                    L.FO[tF] = old + 1u;
+                    bool ok = false;
+                    if (old <= uint32_t(S)) {  // rdme.wgsl:62: admits a claim into a voxel holding S
+                        for (uint32_t q = 0; q < NW && !ok; ++q) {
+                            const uint32_t zb = ~nonzero_bytes(L.FL[tF * NW + q]) & 0x80808080u;
+                            if (zb != 0u) {  // the first empty slot
+                                FLb[tF * uint32_t(S) + 4u * q + (uint32_t(__ffs(zb) - 1) >> 3)] = uint8_t(sp);
+                                ok = true;
+                            }
+                        }
+                        const uint32_t t = L.F[tF];
+                        if (!ok && t + 1u < a.N) {  // slot S: slot 0 of the next voxel
+                            const int f1 = find_sorted(L.F, nF, t + 1u);
+                            if (f1 < 0) {
+                                esc = t + 1u;
+                                break;
+                            }
+                            const uint32_t t1F = uint32_t(f1);
+                            if (FLb[t1F * uint32_t(S)] == 0u) {
+                                FLb[t1F * uint32_t(S)] = uint8_t(sp);
+                                ok = true;
+                                ++alias;
+                                // C[t][sp] counts it, the slots of t+1 hold it
+                                bad |= !dc_add(L.FS[tF], L.FD + tF * kSkewSp, sp, 1);
+                                bad |= !dc_add(L.FS[t1F], L.FD + t1F * kSkewSp, sp, -1);
+                            }
+                        }
+                    }
+                    if (!ok) {
+                        L.FO[tF] = old;
+                        ++failed;
+                        if (wF == kBNoF) {  // a source outside F keeps a particle the gather let go
+                            esc = L.W[wi];
+                            break;
+                        }
+                        continue;
+                    }
+                }
+                if (wF == kBNoF)
+                    continue;
+                // the source: O[w] -= 1, lattice[i_part] = 0, C[w][sp] -= 1 (rdme.wgsl:69-71)
+                L.FO[wF] -= 1u;
+                uint32_t bF = wF;
+                if (s >= uint32_t(S)) {  // a walk past slot S: the particle sits in the next voxel's block
+                    const int f = find_sorted(L.F, nF, L.W[wi] + 1u);
+                    if (f < 0) {  // a block outside F
+                        esc = L.W[wi] + 1u;
+                        break;
+                    }
+                    bF = uint32_t(f);
+                }
+                const uint32_t at = bF * uint32_t(S) + s % uint32_t(S);
+                const uint32_t cur = FLb[at];
+                FLb[at] = 0u;
+                if (!(bF == wF && cur == sp)) {
+                    bad |= !dc_add(L.FS[bF], L.FD + bF * kSkewSp, cur, 1);
+                    bad |= !dc_add(L.FS[wF], L.FD + wF * kSkewSp, sp, -1);
+                }
+            }
+            if (esc != 0xFFFFFFFFu && L.n_ext < uint32_t(kEscMax)) {
+                uint32_t q = L.n_ext++;  // keep ext sorted
+                while (q > 0u && L.ext[q - 1u] > esc) {
+                    L.ext[q] = L.ext[q - 1u];
+                    --q;
+                }
+                L.ext[q] = esc;
+            } else if (esc != 0xFFFFFFFFu) {
+                L.n_ext = uint32_t(kEscMax) + 1u;  // more voxels than F may add
+            }
+            L.esc = esc;
+        }
+        __syncthreads();
+        const bool any_bad = __syncthreads_or(bad ? 1 : 0) != 0;
+        if (L.esc == 0xFFFFFFFFu)
+            done = true;
+        else if (L.n_ext > uint32_t(kEscMax) || any_bad)
+            return 0;
+    }
+    if (!done)
+        return 0;
+    // the counts the step kernel took for F's gathered outputs; reaction and compaction of F; outputs, offsets and
+    // live bits (one voxel per thread)
+    for (uint32_t i = tid; i < nF; i += nth) {
+        const uint32_t v = L.F[i];
+        VoxelFlags g;
+        (void)gather_voxel<S>(a, tb, v, g);
+        tally.fired -= g.fired;
+        tally.rolled -= g.rolled;
+        tally.near -= g.near;
+        int ei = skew_find(k, base + v);
+        const uint32_t kf = ei >= 0 ? k.ent[ei].kf : 0u;
+        XVox<S> vx;
+        for (uint32_t q = 0; q < NW; ++q)
+            vx.sl.w[q] = L.FL[i * NW + q];
+        vx.occ = L.FO[i];
+        vx.sp = L.FS[i];
+        for (int q = 0; q < kSkewSp; ++q)
+            vx.dc[q] = L.FD[i * kSkewSp + q];
+        bool fired, rolled, near, xbad = false;
+        exact_react<S>(a, vx, kf & 0xFFu, v / a.R12, (v / a.R2) % a.R1, v % a.R2, fired, rolled, near, xbad);
+        tally.fired += fired;
+        tally.rolled += rolled;
+        tally.near += near;
+        bad |= xbad;
+        store_slots<S>(a.dst, v, vx.sl);
+        const int32_t off = int32_t(vx.occ - count_nonzero(vx.sl));
+        if (count_nonzero(vx.sl) != 0u)
+            sum_mark(a.sum_dst, v);
+        if (ei < 0 && (off != 0 || vx.sp != 0u))
+            ei = skew_insert(k, base + v);
+        if (ei >= 0) {
+            SkewEntry &e = k.ent[ei];
+            e.off = off;
+            e.sp = vx.sp;
+            for (int q = 0; q < kSkewSp; ++q)
+                e.dc[q] = vx.dc[q];
+            e.kf = kf | kSkewDone;
+            set_voxel_bit(k.live, base + v, skew_live(e));
+        } else if (off != 0 || vx.sp != 0u) {
+            bad = true;  // table full
+        }
+    }
+    const bool any_bad = __syncthreads_or(bad ? 1 : 0) != 0;
+    if (tid == 0u) {
+        tally.failed += failed;
+        tally.alias += alias;
+        tally.bad += any_bad;
+    }
+    return 1;
+}
+
 // The reaction of a live entry that no cluster replayed this step: the step kernel stored the voxel's
 // post-diffusion slots and left its reaction here.
 template <int S>
@@ -1623,29 +1969,6 @@ __device__ void skew_react(const StepArgs &a, const SkewTab &k, uint32_t ei, Rep
         set_voxel_bit(k.live, e.v, false);
 }
 
-// Ascending sort of arr[0..n) in place by bitonic merging, the whole block taking part (arr padded to n2, a
-// power of two, with 0xFFFFFFFF).
-__device__ void block_bitonic(uint32_t *arr, uint32_t n, uint32_t n2)
-{
-    for (uint32_t i = n + threadIdx.x; i < n2; i += blockDim.x)
-        arr[i] = 0xFFFFFFFFu;
-    __syncthreads();
-    for (uint32_t kk = 2; kk <= n2; kk <<= 1)
-        for (uint32_t jj = kk >> 1; jj > 0; jj >>= 1) {
-            for (uint32_t i = threadIdx.x; i < n2; i += blockDim.x) {
-                const uint32_t l = i ^ jj;
-                if (l > i) {
-                    const uint32_t p = arr[i], q = arr[l];
-                    const bool up = (i & kk) == 0u;
-                    if (up ? p > q : p < q) {
-                        arr[i] = q;
-                        arr[l] = p;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-}
 
 // A skewed voxel whose replay can be left out: no particle, a walk that stays in its own (empty) slots, and no
 // particle of a neighbour hopping into it (so no claim meets its occupancy word). Its gathered output (empty) is
@@ -1827,7 +2150,8 @@ __device__ void replay_step(const Cells &cells, const SkewTab &k, const ReplayBu
     __shared__ uint32_t s_n, s_nx, s_ncl;
     // the sort's list, and afterwards one replay area per wave
     constexpr size_t kWaves = kReplayThreads / 64u;
-    constexpr size_t kRaw = std::max(size_t(kSortLds) * sizeof(uint32_t), kWaves * sizeof(WaveReplayLds<S>));
+    constexpr size_t kRaw = std::max(std::max(size_t(kSortLds) * sizeof(uint32_t), kWaves * sizeof(WaveReplayLds<S>)),
+                                     sizeof(BlockReplayLds<S>));
     __shared__ __align__(16) uint8_t s_raw[kRaw];
     uint32_t *s_sort = reinterpret_cast<uint32_t *>(s_raw);
     __shared__ unsigned long long s_tally[6];  // fired, rolled back, near, exceptions, failed claims, aliases
@@ -1911,17 +2235,33 @@ __device__ void replay_step(const Cells &cells, const SkewTab &k, const ReplayBu
         const uint32_t ncl = min(s_ncl, b.ccap);
         if (tid == 0 && s_ncl > b.ccap)
             tally.bad += s_ncl - b.ccap;
-        // clusters of more than kWaveReplayT noted voxels: one lane each, the arrays in global memory
+        // clusters of more than block_max noted voxels: one lane each, the arrays in global memory
         for (uint32_t c = tid; c < ncl; c += nth) {
             const uint4 cl = b.clus[c];
-            if (cl.y > kWaveReplayT && !replay_cluster<S>(cells.args(cl.z), k, b, b.T + cl.x, cl.y, cl.w, tally))
+            if (cl.y > b.block_max && !replay_cluster<S>(cells.args(cl.z), k, b, b.T + cl.x, cl.y, cl.w, tally))
                 tally.bad += cl.y;
+        }
+        // more than wave_max: the whole block, one cluster after another, the arrays in its LDS
+        BlockReplayLds<S> &bl = *reinterpret_cast<BlockReplayLds<S> *>(s_raw);
+        for (uint32_t c = 0; c < ncl; ++c) {
+            const uint4 cl = b.clus[c];
+            if (cl.y <= b.wave_max || cl.y > b.block_max)
+                continue;
+            const StepArgs ca = cells.args(cl.z);
+            const int r = replay_cluster_block<S>(ca, k, bl, b.T + cl.x, cl.y, cl.w, tally);
+            if (r < 0 && tid == 0u) {
+                if (!replay_cluster<S>(ca, k, b, b.T + cl.x, cl.y, cl.w, tally))
+                    tally.bad += cl.y;
+            } else if (r == 0 && tid == 0u) {
+                tally.bad += cl.y;
+            }
+            __syncthreads();
         }
         // the others: one wave each, the arrays in its LDS
         WaveReplayLds<S> &wl = reinterpret_cast<WaveReplayLds<S> *>(s_raw)[tid / 64u];
         for (uint32_t c = tid / 64u; c < ncl; c += nth / 64u) {
             const uint4 cl = b.clus[c];
-            if (cl.y > kWaveReplayT)
+            if (cl.y > b.wave_max)
                 continue;
             const StepArgs ca = cells.args(cl.z);
             const int r = replay_cluster_wave<S>(ca, k, wl, b.T + cl.x, cl.y, cl.w, tally);
@@ -4317,6 +4657,15 @@ struct SkewStore {
         b.lds_sort_max = kSortLds;
         if (const char *x = std::getenv("CS_REPLAY_LDS_SORT"))  // test hook: longer lists take the bitmap sort
             b.lds_sort_max = std::min<uint32_t>(uint32_t(std::strtoul(x, nullptr, 10)), kSortLds);
+        b.wave_max = kWaveReplayT;
+        b.block_max = kBlockReplayT;
+        if (const char *x = std::getenv("CS_REPLAY_FORMS")) {  // test hook: "block" (no wave form), "lane" (one lane)
+            const std::string f(x);
+            if (f == "block" || f == "lane")
+                b.wave_max = 0u;
+            if (f == "lane")
+                b.block_max = 0u;
+        }
         const size_t lo_words = (nvox + 31u) / 32u, hi_words = ((nvox + 1023u) / 1024u + 31u) / 32u;
         hip_check(hipMalloc(&b.T, size_t(b.tcap) * sizeof(uint32_t)), "hipMalloc replay list");
         hip_check(hipMalloc(&b.xclus, size_t(b.ccap) * sizeof(uint4)), "hipMalloc replay runs");
@@ -5081,13 +5430,13 @@ void Device::plan_balanced_items(const std::vector<uint64_t> &cnt, double plane_
 {
     const uint32_t tz_n = wtiles_z_, by_n = (wtiles_y_ + kWavesPerBlock - 1) / kWavesPerBlock;
     const uint32_t R0 = res_[0], ncol = tz_n * by_n;
-    // at least 6 planes per item; 12 on a lattice of fewer than one particle per thousand voxels, whose items are
-    // mostly skipped through the occupancy summaries and cost their fixed part (an empty 128^3 cell: 43 us per step
-    // with 6-plane items, 26 with 12; a populated 64^3 cell: 47 and 72)
+    // at least 6 planes per item (shorter marches end sooner where a few blocks hold all the work: a populated 64^3
+    // cell takes 47 us per step with 6-plane items, 72 with 12); 12 on an empty lattice, whose items the occupancy
+    // summaries skip whole, so that only their fixed cost counts (an empty 128^3 cell: 43 us with 6, 26 with 12)
     uint64_t particles = 0;
     for (uint64_t v : cnt)
         particles += v;
-    const uint32_t min_default = particles * 1000u < N_ ? 12u : 6u;
+    const uint32_t min_default = particles == 0u ? 12u : 6u;
     const uint32_t min_len = std::max(1u, std::min<uint32_t>(min_planes_set_ ? xc_min_ : min_default, R0));
     const uint32_t max_per_col = std::max(1u, R0 / min_len);
     // items per resident block: one while an item holds up to about 24 planes per tile column, more on larger

@@ -183,7 +183,7 @@ def test_region_counts_match_lattice(oracle_mod):
 
 
 @pytest.mark.parametrize("kernel", ["wave", "wave-overflow", "wave-summary", "wave-bitmapsort", "wave-splittab",
-                                    "wave-globaltab", "tiled", "simple"])
+                                    "wave-globaltab", "wave-replayblock", "wave-replaylane", "tiled", "simple"])
 def test_dense_capacity_replay(oracle_mod, kernel):
     """S=4 at high density: claims into full voxels are rejected (rdme.wgsl:62, 73-81) or land in the next
     voxel's slot 0 (rdme.wgsl:30-40), after which the reference's occupancy word and counts leave the slots.
@@ -203,6 +203,10 @@ def test_dense_capacity_replay(oracle_mod, kernel):
             os.environ["CS_WAVE_GLOBAL_TABLES"] = "1"  # the reaction tables read from global memory
         if kernel == "wave-globaltab":
             os.environ["CS_WAVE_GLOBAL_TABLES"] = "2"  # every model table read from global memory
+        if kernel == "wave-replayblock":
+            os.environ["CS_REPLAY_FORMS"] = "block"  # every cluster of up to 64 noted voxels replayed by the block
+        if kernel == "wave-replaylane":
+            os.environ["CS_REPLAY_FORMS"] = "lane"  # every cluster replayed by one lane, arrays in global memory
         try:
             sim = _walled_model(20, 4, dens, seed=31) if walled else _dense_model(20, 4, dens, seed=31)
             orc = oracle_mod.Oracle.from_simulation(sim)
@@ -213,6 +217,7 @@ def test_dense_capacity_replay(oracle_mod, kernel):
             os.environ.pop("CS_OCC_SUMMARY", None)
             os.environ.pop("CS_REPLAY_LDS_SORT", None)
             os.environ.pop("CS_WAVE_GLOBAL_TABLES", None)
+            os.environ.pop("CS_REPLAY_FORMS", None)
         skew = 0
         for t, it in enumerate(default_itime_schedule(31, nsteps)):
             sim.step(t, int(it), 0)
