@@ -3072,6 +3072,29 @@ __device__ __forceinline__ void full_records(const uint8_t *src, uint32_t v, uin
     }
 }
 
+// Raw buffer stores (stride 0): a lane whose byte offset is at or past the resource's size stores nothing. Word 3
+// of the resource is the gfx9 default (32-bit data format, no swizzle); kBufferDrop lies past any plane of a
+// lattice the wave kernels take (R1, R2 <= 4096, S <= 16: below 2^28 bytes).
+constexpr int kBufferRsrcWord3 = 0x00020000;
+constexpr uint32_t kBufferDrop = 0x80000000u;
+typedef uint32_t u32x2_t __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+
+template <int S>
+__device__ __forceinline__ void buffer_store_vec(const typename SlotVec<S>::T &v, __amdgpu_buffer_rsrc_t rs,
+                                                 uint32_t off)
+{
+    if constexpr (S == 16) {
+        const u32x4_t d = {v.x, v.y, v.z, v.w};
+        __builtin_amdgcn_raw_buffer_store_b128(d, rs, int(off), 0, 0);
+    } else if constexpr (S == 8) {
+        const u32x2_t d = {v.x, v.y};
+        __builtin_amdgcn_raw_buffer_store_b64(d, rs, int(off), 0, 0);
+    } else {
+        __builtin_amdgcn_raw_buffer_store_b32(v, rs, int(off), 0, 0);
+    }
+}
+
 template <int S>
 __device__ __forceinline__ typename HeadVec<S>::T load_head(const uint8_t *base, uint32_t v)
 {
@@ -3290,7 +3313,6 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
     // record for lanes < 2*kWY (rows 0..kWY-1 at column -1, then at column kWZ)
     uint32_t off[kWBatches];
     uint32_t okm = 0u, inl = 0u;  // record inside the lattice; interior voxel inside the lattice
-    int zrow = 0, zcol = -1;
 #pragma unroll
     for (int k = 0; k < kWBatches; ++k) {
         int row = k, col = l;
@@ -3299,8 +3321,6 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
         if (k == kWY + 2) {
             row = (l < kWY) ? l : l - kWY;
             col = (l < kWY) ? -1 : kWZ;
-            zrow = row;
-            zcol = col;
         }
         const int Y = y0 + row, Z = z0 + col;
         const bool ok = (k < kWY + 2 || l < 2 * kWY) && Y >= 0 && Y < R1 && Z >= 0 && Z < R2;
@@ -3309,7 +3329,6 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
         if (k < kWY)
             inl |= uint32_t(ok) << k;
     }
-    const uint32_t zrec = uint32_t((zrow + 1) * kWRZ + (zcol + 1));
     // label words of this lane: word i of the plane's (kWY+4) x 17-word grid is row i / 17, word i % 17
     uint32_t loff[kWLabLane];
 #pragma unroll
@@ -3427,8 +3446,11 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
 
     auto plane = [&](int X, WaveStaged<S> &use, WaveStaged<S> &fill) __attribute__((always_inline)) {
         // The per-lane record bits are taken as new on every plane: otherwise the compiler hoists one lane mask per
-        // record batch out of the march into scalar registers, which then spill and are read back on every plane.
+        // record batch out of the march into scalar registers, which then spill and are read back on every plane;
         asm volatile("" : "+v"(okm), "+v"(inl));
+        // the lane id, likewise: values derived from it are recomputed in the plane rather than kept (and spilled)
+        int l = int(__lane_id());
+        asm volatile("" : "+v"(l));
         // ---- prefetch plane X+1's slots and plane X+2's labels (clamped; unused past the chunk)
         {
             // planes past xe are never listed nor gathered; on the side plane xe only the interior rows are
@@ -3437,18 +3459,20 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
             // ones not needed at one line (the lattice's first record, the label volume's first word), and their
             // registers are then never read: with the same number of loads on every path, the wait for the
             // previous plane's registers below leaves all of this plane's loads in flight (with some loads on some
-            // paths only, the compiler has to assume the fewest and waits for part of this prefetch too).
+            // paths only, the compiler has to assume the fewest and waits for part of this prefetch too). The
+            // unneeded loads of one batch read different words of one line, so that no two of them are the same
+            // load (which the compiler would issue once and copy, waiting for it).
             const uint32_t sb = slot_base(X + 1), lb = lab_base(X + 2);
             const bool go = X + 1 <= xe && (!summ || pact(X + 1));
             const int nb = !go ? 0 : (X + 1 == xe ? kWY : kWBatches);
 #pragma unroll
             for (int k = 0; k < kWBatches; ++k)
-                fill.rec[k] = load_head<S>(a.src, k < nb ? sb + off[k] : 0u);
+                fill.rec[k] = load_head<S>(a.src, k < nb ? sb + off[k] : (off[k] & 7u));
             // labels up to plane xe + 1 (the hops of plane xe look at them)
             const bool lgo = X + 1 <= xe;
 #pragma unroll
             for (int i = 0; i < kWLabLane; ++i)
-                fill.lab[i] = w.regp[lgo ? lb + loff[i] : 0u];
+                fill.lab[i] = w.regp[lgo ? lb + loff[i] : (loff[i] & 31u)];
         }
         store_lab(X + 1, use.lab);
         // plane X+1's event list starts empty (it was consumed as plane X-2's); plane X's leave masks too
@@ -3486,7 +3510,7 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
                     if (k < kWY) rec = uint32_t((k + 1) * kWRZ + l + 1);
                     else if (k == kWY) rec = uint32_t(l + 1);
                     else if (k == kWY + 1) rec = uint32_t((kWY + 1) * kWRZ + l + 1);
-                    else rec = zrec;
+                    else rec = uint32_t((l < kWY ? l + 1 : l - kWY + 1) * kWRZ + (l < kWY ? 0 : kWZ + 1));
                     L.hop_slots[pos] = use.rec[k];
                     L.hop_rec[pos] = uint16_t(rec | (head_full<S>(use.rec[k]) ? 0x8000u : 0u));
                     ++pos;
@@ -3578,14 +3602,19 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
 
         // ---- T2: gather plane Q = X-1
         const int Q = X - 1;
-        if (Q >= xb && Q < xe) {
+        const bool t2 = Q >= xb && Q < xe;
+        uint32_t ns = 0u, needm = 0u, sidx[kWY];  // voxels to gather: count, this lane's rows, list positions
+#pragma unroll
+        for (int r = 0; r < kWY; ++r)
+            sidx[r] = 0u;
+        if (t2) {
             const int rq = ring3(Q);
             // a plane whose event list overflowed gathers every interior voxel's arrivals from global memory
             const bool over = L.ev_n[rq] > w.ev_cap;
             // the voxels to gather, as wave-wide row masks: occupied, or a particle arrives (everything when the
             // event list overflowed or empty voxels can react)
             const uint64_t every = (over || w.has_zero_order != 0u) ? ~0ull : 0ull;
-            uint32_t ns = 0u, needm = every != 0ull ? (1u << kWY) - 1u : occ_prev, sidx[kWY];
+            needm = every != 0ull ? (1u << kWY) - 1u : occ_prev;
 #pragma unroll
             for (int r = 0; r < kWY; ++r)
                 needm |= uint32_t(L.arrive[rq][r] >> l & 1ull) << r;
@@ -3742,23 +3771,29 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
                 }
             }
             wave_sync();
+        }
+        // ---- the output rows of plane Q, one store per row and lane on every plane (side planes included): a
+        // lane with nothing to store gets an offset past the plane, which the buffer store drops. With the same
+        // number of stores on every path, the wait for the next plane's prefetch leaves them in flight; with
+        // stores on some paths only, that wait had to cover this plane's stores too, a write round trip per plane.
+        {
             // an empty output over an empty destination needs no store
             const bool old_q = summ && pold(Q);
+            const bool st = t2 && (!summ || ns != 0u || old_q);
+            const uint32_t Qc = uint32_t(min(max(Q, 0), R0 - 1));
+            const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+                a.dst + size_t(Qc * a.R12) * uint32_t(S), 0, int(a.R12 * uint32_t(S)), kBufferRsrcWord3);
             bool nz = false;
-            if (!summ || ns != 0u || old_q) {
 #pragma unroll
-                for (int r = 0; r < kWY; ++r) {
-                    if ((inl >> r) & 1u) {
-                        const bool need = (needm >> r) & 1u;
-                        const Vec v = to_vec<S>(from_head<S>(need ? L.hop_slots[sidx[r]] : HVec{}));
-                        const bool deferred = need && L.slow[sidx[r]] != 0u;
-                        if (!deferred)
-                            *reinterpret_cast<Vec *>(a.dst + size_t(off[r] + uint32_t(Q) * a.R12) * uint32_t(S)) = v;
-                        nz = nz || deferred || vec_nonzero<S>(v);
-                    }
-                }
+            for (int r = 0; r < kWY; ++r) {
+                const bool need = (needm >> r) & 1u;  // implies a lattice voxel of a gathered plane
+                const Vec v = to_vec<S>(from_head<S>(need ? L.hop_slots[sidx[r]] : HVec{}));
+                const bool deferred = need && L.slow[sidx[r]] != 0u;
+                const bool put = st && ((inl >> r) & 1u) && !deferred;
+                buffer_store_vec<S>(v, rs, put ? off[r] * uint32_t(S) : kBufferDrop);
+                nz = nz || deferred || (put && vec_nonzero<S>(v));
             }
-            if (summ) {
+            if (summ && t2) {
                 const bool new_q = __ballot(nz) != 0ull;
                 if (new_q != old_q && l == 0)
                     own_sum[Q] = uint8_t(new_q);
