@@ -3072,9 +3072,10 @@ __device__ __forceinline__ void full_records(const uint8_t *src, uint32_t v, uin
     }
 }
 
-// Raw buffer stores (stride 0): a lane whose byte offset is at or past the resource's size stores nothing. Word 3
-// of the resource is the gfx9 default (32-bit data format, no swizzle); kBufferDrop lies past any plane of a
-// lattice the wave kernels take (R1, R2 <= 4096, S <= 16: below 2^28 bytes).
+// Raw buffer stores (stride 0, no scalar offset): a lane whose byte offset is at or past the resource's size stores
+// nothing. Word 3 of the resource is the gfx9 default (32-bit data format, no swizzle); kBufferDrop lies past any
+// plane of a lattice the wave kernels take (R1, R2 <= 4096, S <= 16: below 2^28 bytes). The plane's offset goes
+// into the base: passed as the scalar offset instead, one store of the gene-expression parity case was lost.
 constexpr int kBufferRsrcWord3 = 0x00020000;
 constexpr uint32_t kBufferDrop = 0x80000000u;
 typedef uint32_t u32x2_t __attribute__((ext_vector_type(2)));
@@ -3780,6 +3781,14 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
             // an empty output over an empty destination needs no store
             const bool old_q = summ && pold(Q);
             const bool st = t2 && (!summ || ns != 0u || old_q);
+            HVec h[kWY];
+            uint32_t sl[kWY];
+#pragma unroll
+            for (int r = 0; r < kWY; ++r) {  // every list read first (in bounds on every lane), then one wait
+                const uint32_t i = ((needm >> r) & 1u) ? sidx[r] : 0u;
+                h[r] = L.hop_slots[i];
+                sl[r] = L.slow[i];
+            }
             const uint32_t Qc = uint32_t(min(max(Q, 0), R0 - 1));
             const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
                 a.dst + size_t(Qc * a.R12) * uint32_t(S), 0, int(a.R12 * uint32_t(S)), kBufferRsrcWord3);
@@ -3787,8 +3796,8 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
 #pragma unroll
             for (int r = 0; r < kWY; ++r) {
                 const bool need = (needm >> r) & 1u;  // implies a lattice voxel of a gathered plane
-                const Vec v = to_vec<S>(from_head<S>(need ? L.hop_slots[sidx[r]] : HVec{}));
-                const bool deferred = need && L.slow[sidx[r]] != 0u;
+                const Vec v = to_vec<S>(from_head<S>(need ? h[r] : HVec{}));
+                const bool deferred = need && sl[r] != 0u;
                 const bool put = st && ((inl >> r) & 1u) && !deferred;
                 buffer_store_vec<S>(v, rs, put ? off[r] * uint32_t(S) : kBufferDrop);
                 nz = nz || deferred || (put && vec_nonzero<S>(v));
