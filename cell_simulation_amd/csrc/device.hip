@@ -2995,16 +2995,17 @@ static_assert(kWY == int(kSumTY) && kWZ == int(kSumTZ), "the occupancy summary c
 
 template <int S>
 struct WaveLds {
-    typename HeadVec<S>::T hop_slots[kWHop];  // listed records of the plane being hopped (their heads)
-    uint16_t hop_rec[kWHop];                   // their record ids: (row + 1) * kWRZ + (column + 1); bit 15: full head
+    typename HeadVec<S>::T hop_slots[kWHop];  // listed records of the plane being hopped (their heads); then the
+                                               // own slots of the voxels of plane X-1 to gather
+    uint16_t hop_rec[kWHop];                   // their record ids: (row + 1) * kWRZ + (column + 1); bit 15: full head;
+                                               // then the gathered voxels' leave masks
     uint32_t ev[3][kWEv];                      // arrival events per destination plane (X % 3): direction |
                                                // species << 3 | next event to the same voxel << 11
     uint32_t head[3][kWY * kWZ / 2];           // latest event into each interior voxel (kWEvEnd: none), 16 bits each
     uint32_t ev_n[4];                          // events appended (may pass the cap: the plane overflowed)
-    uint64_t arrive[3][kWY];                   // bit z of row r: an event targets voxel (r, z)
     uint32_t leave[2][kWY * kWZ / 2];          // 16-bit leave mask per interior voxel (X % 2)
     uint32_t lab[3][kWLabWords];               // label planes (X % 3)
-    uint16_t slow[kWY * kWZ];                  // voxels of plane X-1 to gather
+    uint16_t slow[kWY * kWZ];                  // voxels of plane X-1 to gather: row * kWZ + column | chain head << 8
     typename SlotVec<S>::T rb_slots[kWReact];  // gathered voxels whose reaction draw passed the bound test,
     uint32_t rb_v[kWReact];                    // their voxel index,
     uint32_t rb_xc[kWReact];                   // x | particle count << 24
@@ -3218,12 +3219,14 @@ __device__ __forceinline__ uint32_t assemble4(uint32_t own, uint32_t occ, uint32
     return o;
 }
 
-// Event-chain heads, 16 bits per voxel in 32-bit words: exchange (returns the previous head) and read.
+// Event-chain heads, 16 bits per voxel in 32-bit words: exchange (returns the previous head) and read. The first
+// compare-and-swap takes both halves of the word as still empty (kWEvEnd), true for most arrivals, so that no read
+// of the word goes before it; when the guess is wrong the swap returns the word and is retried with it.
 __device__ __forceinline__ uint32_t head_exchange(uint32_t *heads, uint32_t v, uint32_t idx)
 {
     uint32_t *word = heads + (v >> 1);
     const uint32_t sh = (v & 1u) * 16u;
-    uint32_t old = *word, assumed;
+    uint32_t old = kWEvEnd | (kWEvEnd << 16), assumed;
     do {
         assumed = old;
         old = atomicCAS(word, assumed, (assumed & ~(0xFFFFu << sh)) | (idx << sh));
@@ -3403,8 +3406,6 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
         st0.lab[i] = w.regp[lab_base(xb) + loff[i]];
     if (l < 3)
         L.ev_n[l] = 0u;
-    if (l < 3 * kWY)
-        L.arrive[l / kWY][l % kWY] = 0ull;
 #pragma unroll
     for (int q = 0; q < 3 * kWY / 2; ++q)
         L.head[q / (kWY / 2)][(q % (kWY / 2)) * 64 + l] = kWEvEnd | (kWEvEnd << 16);
@@ -3479,8 +3480,6 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
         // plane X+1's event list starts empty (it was consumed as plane X-2's); plane X's leave masks too
         if (l == 0)
             L.ev_n[ring3(X + 1)] = 0u;
-        if (l < kWY)
-            L.arrive[ring3(X + 1)][l] = 0ull;
 #pragma unroll
         for (int q = 0; q < kWY / 2; ++q)
             L.head[ring3(X + 1)][q * 64 + l] = kWEvEnd | (kWEvEnd << 16);
@@ -3591,8 +3590,6 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
                             const uint32_t prev = head_exchange(L.head[rg3], dest, idx);
                             L.ev[rg3][idx] = d | (sp << 3) | (prev << 11);
                         }
-                        atomicOr(reinterpret_cast<unsigned long long *>(&L.arrive[rg3][r2]),
-                                 (unsigned long long)(1ull << z2));
                     }
                 }
                 if (act && interior && lv != 0u)
@@ -3610,15 +3607,24 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
             sidx[r] = 0u;
         if (t2) {
             const int rq = ring3(Q);
+            // this lane's voxels: their chain heads (kWEvEnd: no arrival) and leave masks, read together with the
+            // event count
+            const uint16_t *hq = reinterpret_cast<const uint16_t *>(L.head[rq]);
+            const uint16_t *lq = reinterpret_cast<const uint16_t *>(L.leave[Q & 1]);
+            uint32_t hd[kWY], lvr[kWY];
+#pragma unroll
+            for (int r = 0; r < kWY; ++r) {
+                hd[r] = hq[r * kWZ + l];
+                lvr[r] = lq[r * kWZ + l];
+            }
             // a plane whose event list overflowed gathers every interior voxel's arrivals from global memory
             const bool over = L.ev_n[rq] > w.ev_cap;
-            // the voxels to gather, as wave-wide row masks: occupied, or a particle arrives (everything when the
-            // event list overflowed or empty voxels can react)
-            const uint64_t every = (over || w.has_zero_order != 0u) ? ~0ull : 0ull;
-            needm = every != 0ull ? (1u << kWY) - 1u : occ_prev;
+            // the voxels to gather: occupied, or a particle arrives (everything when the event list overflowed,
+            // whose chains miss events, or when empty voxels can react)
+            needm = (over || w.has_zero_order != 0u) ? (1u << kWY) - 1u : occ_prev;
 #pragma unroll
             for (int r = 0; r < kWY; ++r)
-                needm |= uint32_t(L.arrive[rq][r] >> l & 1ull) << r;
+                needm |= uint32_t(hd[r] != kWEvEnd) << r;
             needm &= inl;
 #pragma unroll
             for (int r = 0; r < kWY; ++r) {
@@ -3638,32 +3644,34 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
                 sidx[r] = pos;
                 if ((needm >> r) & 1u) {
                     const bool occ = (occ_prev >> r) & 1u;
-                    const bool full = occ && head_full<S>(prev[r]);
-                    L.slow[pos] = uint16_t(uint32_t(r * kWZ + l) | (uint32_t(full) << 12));
-                    L.hop_slots[pos] = occ ? prev[r] : HVec{};  // the hop list is done with
+                    // the entry: voxel | chain head << 8, the leave mask, the own slots (the hop list and its
+                    // record ids are done with)
+                    L.slow[pos] = uint16_t(uint32_t(r * kWZ + l) | (hd[r] << 8));
+                    L.hop_rec[pos] = uint16_t(lvr[r]);
+                    L.hop_slots[pos] = occ ? prev[r] : HVec{};
                     ++pos;
                 }
             }
             wave_sync();
-            const uint16_t *lvq = reinterpret_cast<const uint16_t *>(L.leave[Q & 1]);
             for (uint32_t base = 0; base < ns; base += 64u) {
                 const uint32_t i = base + uint32_t(l);
                 const bool act = i < ns;
-                const uint32_t code = act ? L.slow[i] : 0u;
-                const uint32_t me = code & 0xFFFu;
+                const uint32_t code = act ? L.slow[i] : (kWEvEnd << 8);
+                const uint32_t me = code & 0xFFu, e0 = code >> 8;
+                const uint32_t lv = act ? L.hop_rec[i] : 0u;
                 const int r = int(me) / kWZ, z = int(me) % kWZ;
                 const uint32_t Y = uint32_t(y0 + r), Z = uint32_t(z0 + z);
                 const uint32_t V = Z + __umul24(Y, a.R2) + uint32_t(Q) * a.R12;
-                Slots<S> own = from_head<S>(act ? L.hop_slots[i] : HVec{});
+                const HVec hv = act ? L.hop_slots[i] : HVec{};
+                Slots<S> own = from_head<S>(hv);
                 if constexpr (S > kHeadSlots<S>) {
-                    const uint64_t fm = __ballot(act && (code & 0x1000u) != 0u);  // full heads (rare)
+                    const uint64_t fm = __ballot(act && head_full<S>(hv));  // full heads (rare)
                     if (fm != 0ull) {
                         full_records<S>(a.src, V, fm, own);
                         vm_drain();
                     }
                 }
                 const uint32_t occ = count_compacted(own);
-                const uint32_t lv = act ? lvq[me] : 0u;
                 // Most voxels end with at most four particles: then the arrivals and the assembled slots fit
                 // one 32-bit word, and the wave takes that path when every lane fits it.
                 uint32_t cnt = 0u;
@@ -3675,7 +3683,7 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
                         // arrivals per rank, one nibble each: exact while the voxel gets at most 15 per rank, and
                         // a lane with more has more than four arrivals, so it leaves this path anyway
                         uint32_t cn = 0u;
-                        uint32_t e = act ? head_read(L.head[rq], me) : kWEvEnd;
+                        uint32_t e = e0;
                         while (__ballot(e != kWEvEnd) != 0ull) {
                             if (e != kWEvEnd) {
                                 const uint32_t c = L.ev[rq][e];
@@ -3711,7 +3719,7 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
                         // in that order = nibble d of 0x231405).
                         BV arr{0ull, 0ull};
                         uint64_t cnts = 0ull;  // arrivals per rank, one byte each
-                        uint32_t e = act ? head_read(L.head[rq], me) : kWEvEnd;
+                        uint32_t e = e0;
                         while (__ballot(e != kWEvEnd) != 0ull) {
                             if (e != kWEvEnd) {
                                 const uint32_t c = L.ev[rq][e];
