@@ -3399,11 +3399,22 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
         store_lab(xb - 1, r);
     }
 #pragma unroll
-    for (int k = 0; k < kWBatches; ++k)  // the side plane xb - 1: interior rows only
-        st0.rec[k] = k < kWY ? load_head<S>(a.src, slot_base(xb - 1) + off[k]) : HVec{};
+    for (int k = 0; k < kWBatches; ++k)  // the side plane xb - 1: interior rows only (the other loads at one line)
+        st0.rec[k] = load_head<S>(a.src, k < kWY ? slot_base(xb - 1) + off[k] : (off[k] & 7u));
 #pragma unroll
     for (int i = 0; i < kWLabLane; ++i)
         st0.lab[i] = w.regp[lab_base(xb) + loff[i]];
+    {
+        // four stores that store nothing: with them, the memory operations in flight when the march enters its
+        // plane loop are those of every later plane (a prefetch, then the row stores), and the first plane's wait
+        // for its registers is the same as every other's, which leaves the previous plane's stores in flight (with
+        // fewer operations on this path the compiler's wait at the top of the loop covered some of those stores)
+        const __amdgpu_buffer_rsrc_t rs =
+            __builtin_amdgcn_make_buffer_rsrc(a.dst, 0, int(a.R12 * uint32_t(S)), kBufferRsrcWord3);
+#pragma unroll
+        for (int r = 0; r < kWY; ++r)  // at different offsets, which the compiler does not merge into one store
+            buffer_store_vec<S>(Vec{}, rs, kBufferDrop + uint32_t(r) * 1024u);
+    }
     if (l < 3)
         L.ev_n[l] = 0u;
 #pragma unroll
