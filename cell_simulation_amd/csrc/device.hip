@@ -3073,6 +3073,16 @@ __device__ __forceinline__ void full_records(const uint8_t *src, uint32_t v, uin
     }
 }
 
+// A pointer that is the same on every lane, in scalar registers: a buffer resource must be, and the ensemble
+// kernel's cell pointers come from a per-lane load of the cell's descriptor (without this the compiler wraps every
+// buffer store in a loop over the distinct values).
+__device__ __forceinline__ uint8_t *uniform_ptr(uint8_t *p)
+{
+    const uint64_t v = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(v)), hi = __builtin_amdgcn_readfirstlane(uint32_t(v >> 32));
+    return reinterpret_cast<uint8_t *>((uint64_t(hi) << 32) | lo);
+}
+
 // Raw buffer stores (stride 0, no scalar offset): a lane whose byte offset is at or past the resource's size stores
 // nothing. Word 3 of the resource is the gfx9 default (32-bit data format, no swizzle); kBufferDrop lies past any
 // plane of a lattice the wave kernels take (R1, R2 <= 4096, S <= 16: below 2^28 bytes). The plane's offset goes
@@ -3410,7 +3420,7 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
         // for its registers is the same as every other's, which leaves the previous plane's stores in flight (with
         // fewer operations on this path the compiler's wait at the top of the loop covered some of those stores)
         const __amdgpu_buffer_rsrc_t rs =
-            __builtin_amdgcn_make_buffer_rsrc(a.dst, 0, int(a.R12 * uint32_t(S)), kBufferRsrcWord3);
+            __builtin_amdgcn_make_buffer_rsrc(uniform_ptr(a.dst), 0, int(a.R12 * uint32_t(S)), kBufferRsrcWord3);
 #pragma unroll
         for (int r = 0; r < kWY; ++r)  // at different offsets, which the compiler does not merge into one store
             buffer_store_vec<S>(Vec{}, rs, kBufferDrop + uint32_t(r) * 1024u);
@@ -3810,7 +3820,7 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
             }
             const uint32_t Qc = uint32_t(min(max(Q, 0), R0 - 1));
             const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-                a.dst + size_t(Qc * a.R12) * uint32_t(S), 0, int(a.R12 * uint32_t(S)), kBufferRsrcWord3);
+                uniform_ptr(a.dst) + size_t(Qc * a.R12) * uint32_t(S), 0, int(a.R12 * uint32_t(S)), kBufferRsrcWord3);
             bool nz = false;
 #pragma unroll
             for (int r = 0; r < kWY; ++r) {
