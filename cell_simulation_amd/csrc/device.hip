@@ -4266,7 +4266,7 @@ __global__ __launch_bounds__(64) void ens_prologue(EnsArgs e, const CmeArgs *cme
 // kWavesPerBlock wave tiles stacked in y; the block stages the cell's tables in LDS and each wave marches
 // its tile.
 template <int S>
-__global__ __launch_bounds__(kWaveBlock) void rdme_step_ensemble(EnsArgs e)
+__global__ __launch_bounds__(kWaveBlock) __attribute__((amdgpu_waves_per_eu(3))) void rdme_step_ensemble(EnsArgs e)
 {
     __shared__ WaveLds<S> L[kWavesPerBlock];
     __shared__ uint32_t next;
