@@ -3813,10 +3813,17 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
             HVec h[kWY];
             uint32_t sl[kWY];
 #pragma unroll
-            for (int r = 0; r < kWY; ++r) {  // every list read first (in bounds on every lane), then one wait
-                const uint32_t i = ((needm >> r) & 1u) ? sidx[r] : 0u;
-                h[r] = L.hop_slots[i];
-                sl[r] = L.slow[i];
+            for (int r = 0; r < kWY; ++r) {
+                h[r] = HVec{};
+                sl[r] = 0u;
+            }
+            if (ns != 0u) {  // (a plane with nothing gathered stores zeros, if anything)
+#pragma unroll
+                for (int r = 0; r < kWY; ++r) {  // every list read first (in bounds on every lane), then one wait
+                    const uint32_t i = ((needm >> r) & 1u) ? sidx[r] : 0u;
+                    h[r] = L.hop_slots[i];
+                    sl[r] = L.slow[i];
+                }
             }
             const uint32_t Qc = uint32_t(min(max(Q, 0), R0 - 1));
             const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
