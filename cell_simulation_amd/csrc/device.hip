@@ -1644,6 +1644,7 @@ struct BlockReplayLds {
     uint32_t T[kBlockReplayT];              // the noted voxels
     uint32_t ext[kEscMax];                  // voxels added to F, ascending
     uint32_t scan[kReplayThreads / 64u];    // per-wave sums of block_offset
+    uint32_t box[6];                        // bounding box of block_hood_set: x, y, z minima and maxima
     uint32_t n_ev, n_ext, esc, n_out;
 };
 
@@ -1693,6 +1694,110 @@ __device__ uint32_t block_sort_unique(uint32_t *cand, uint32_t n, uint32_t *out,
     return total;
 }
 
+// The voxels hood_list(a, src[i], nk) for i < n, and extra[0, n_extra), sorted and without repeats (0xFFFFFFFF
+// entries dropped), written to out (the first cap of them); returns their number. They are marked in a bitmap over
+// their bounding box, whose row-major order is the lattice's, in `bits` (nwords words of scratch), and read back in
+// order; a box with more voxels than the bitmap has bits takes the sort instead (the candidates in `bits`, which
+// then holds n * nk + n_extra of them). A sort of the candidates costs a bitonic network over the next power of
+// two (91 stages with a block barrier each for 4096 candidates); the bitmap, a few passes. Every thread calls it.
+__device__ uint32_t block_hood_set(const StepArgs &a, const uint32_t *src, uint32_t n, uint32_t nk,
+                                   const uint32_t *extra, uint32_t n_extra, uint32_t *out, uint32_t cap,
+                                   uint32_t *bits, uint32_t nwords, uint32_t *box, uint32_t *wave_sums)
+{
+    const uint32_t tid = threadIdx.x, nth = blockDim.x;
+    if (tid < 6u)
+        box[tid] = (tid & 1u) ? 0u : 0xFFFFFFFFu;
+    __syncthreads();
+    uint32_t lo[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, hi[3] = {0u, 0u, 0u};
+    auto take = [&](uint32_t v) {
+        if (v == 0xFFFFFFFFu)
+            return;
+        const uint32_t c[3] = {v / a.R12, (v / a.R2) % a.R1, v % a.R2};
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = min(lo[d], c[d]);
+            hi[d] = max(hi[d], c[d]);
+        }
+    };
+    uint32_t h[8];
+    for (uint32_t i = tid; i < n; i += nth) {
+        hood_list(a, src[i], nk, h);
+        for (uint32_t q = 0; q < nk; ++q)
+            take(h[q]);
+    }
+    for (uint32_t i = tid; i < n_extra; i += nth)
+        take(extra[i]);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        for (int o = 32; o >= 1; o >>= 1) {
+            lo[d] = min(lo[d], uint32_t(__shfl_xor(int(lo[d]), o)));
+            hi[d] = max(hi[d], uint32_t(__shfl_xor(int(hi[d]), o)));
+        }
+    }
+    if (__lane_id() == 0u) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            atomicMin(&box[2 * d], lo[d]);
+            atomicMax(&box[2 * d + 1], hi[d]);
+        }
+    }
+    __syncthreads();
+    const uint32_t x0 = box[0], y0 = box[2], z0 = box[4];
+    if (x0 > box[1]) {  // no voxel at all
+        __syncthreads();
+        return 0u;
+    }
+    const uint32_t BY = box[3] - y0 + 1u, BZ = box[5] - z0 + 1u;
+    const uint64_t vol = uint64_t(box[1] - x0 + 1u) * BY * BZ;
+    __syncthreads();  // every thread has read the box
+    if (vol > 32ull * nwords) {
+        for (uint32_t i = tid; i < n; i += nth)
+            hood_list(a, src[i], nk, bits + nk * i);
+        for (uint32_t i = tid; i < n_extra; i += nth)
+            bits[nk * n + i] = extra[i];
+        __syncthreads();
+        return block_sort_unique(bits, nk * n + n_extra, out, cap, wave_sums);
+    }
+    const uint32_t nw = uint32_t((vol + 31u) / 32u);
+    for (uint32_t i = tid; i < nw; i += nth)
+        bits[i] = 0u;
+    __syncthreads();
+    auto mark = [&](uint32_t v) {
+        if (v == 0xFFFFFFFFu)
+            return;
+        const uint32_t idx = ((v / a.R12 - x0) * BY + ((v / a.R2) % a.R1 - y0)) * BZ + (v % a.R2 - z0);
+        atomicOr(&bits[idx >> 5], 1u << (idx & 31u));
+    };
+    for (uint32_t i = tid; i < n; i += nth) {
+        hood_list(a, src[i], nk, h);
+        for (uint32_t q = 0; q < nk; ++q)
+            mark(h[q]);
+    }
+    for (uint32_t i = tid; i < n_extra; i += nth)
+        mark(extra[i]);
+    __syncthreads();
+    // each thread reads back a run of words, placed by one prefix sum of their bit counts
+    const uint32_t per = (nw + nth - 1u) / nth, w0 = min(tid * per, nw), w1 = min(w0 + per, nw);
+    uint32_t c = 0u;
+    for (uint32_t w = w0; w < w1; ++w)
+        c += uint32_t(__popc(bits[w]));
+    uint32_t total;
+    uint32_t o = block_offset(c, wave_sums, total);
+    for (uint32_t w = w0; w < w1; ++w) {
+        uint32_t m = bits[w];
+        while (m != 0u) {
+            const uint32_t idx = w * 32u + uint32_t(__ffs(m) - 1);
+            m &= m - 1u;
+            const uint32_t bz = idx % BZ, t = idx / BZ, by = t % BY, bx = t / BY;
+            if (o < cap)
+                out[o] = ((x0 + bx) * a.R1 + (y0 + by)) * a.R2 + (z0 + bz);
+            ++o;
+        }
+    }
+    __syncthreads();
+    return total;
+}
+
 template <int S>
 __device__ int replay_cluster_block(const StepArgs &a, const SkewTab &k, BlockReplayLds<S> &L, const uint32_t *T,
                                     uint32_t nt, uint32_t xlo_t, ReplayTally &tally)
@@ -1715,12 +1820,8 @@ __device__ int replay_cluster_block(const StepArgs &a, const SkewTab &k, BlockRe
         // F: the noted voxels' neighbours, themselves and the voxels after them, and the voxels added
         uint32_t *cand = reinterpret_cast<uint32_t *>(L.ev);
         const uint32_t n_ext = L.n_ext;
-        for (uint32_t i = tid; i < nt; i += nth)
-            hood_list(a, L.T[i], 8u, cand + 8u * i);
-        for (uint32_t i = tid; i < n_ext; i += nth)
-            cand[8u * nt + i] = L.ext[i];
-        __syncthreads();
-        nF = block_sort_unique(cand, 8u * nt + n_ext, L.F, kBlockReplayF, L.scan);
+        nF = block_hood_set(a, L.T, nt, 8u, L.ext, n_ext, L.F, kBlockReplayF, cand, 2u * kBlockReplayEv, L.box,
+                            L.scan);
         if (nF > kBlockReplayF)
             return -1;  // uniform
         if (tid == 0u)
@@ -1739,10 +1840,9 @@ __device__ int replay_cluster_block(const StepArgs &a, const SkewTab &k, BlockRe
             for (int q = 0; q < kSkewSp; ++q)
                 L.FD[i * kSkewSp + q] = e.dc[q];
         }
-        for (uint32_t i = tid; i < nF; i += nth)
-            hood_list(a, L.F[i], 7u, cand + 7u * i);
         __syncthreads();
-        const uint32_t nW = block_sort_unique(cand, 7u * nF, L.W, kBlockReplayW, L.scan);  // nW <= 7 nF
+        const uint32_t nW =
+            block_hood_set(a, L.F, nF, 7u, nullptr, 0u, L.W, kBlockReplayW, cand, 2u * kBlockReplayEv, L.box, L.scan);
         // the moves of the sources W = F and its six neighbours that have an end in F
         for (uint32_t wi = tid; wi < nW; wi += nth) {
             const uint32_t w = L.W[wi];
