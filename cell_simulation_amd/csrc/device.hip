@@ -1624,7 +1624,8 @@ constexpr uint32_t kBlockReplayF = 1024;
 constexpr uint32_t kBlockReplayW = 7u * kBlockReplayF;
 constexpr uint32_t kBlockReplayEv = 4096;
 constexpr uint32_t kBNoF = 0xFFFFu;
-static_assert(kBlockReplayW < (1u << 24) && kBlockReplayF < kBNoF, "block replay event fields");
+constexpr uint32_t kBEvNoF = 0xFFFu;  // the same in a 12-bit event field
+static_assert(kBlockReplayW < (1u << 24) && kBlockReplayF < kBEvNoF, "block replay event fields");
 static_assert(kBlockReplayEv * sizeof(uint64_t) >= 8192u * sizeof(uint32_t) && 8192u >= kBlockReplayW &&
                   8192u >= 8u * kBlockReplayT + kEscMax,
               "the candidate lists (padded to powers of two) fit the event list");
@@ -1632,7 +1633,8 @@ static_assert(kBlockReplayEv * sizeof(uint64_t) >= 8192u * sizeof(uint32_t) && 8
 template <int S>
 struct BlockReplayLds {
     uint64_t ev[kBlockReplayEv];            // events: (W index << 8 | walk position << 3 | direction) << 32 |
-                                            // species | F index of target << 8; first the candidate lists of F, W
+                                            // species | F index of the target << 8 | of the source << 20
+                                            // (kBEvNoF: none); first the scratch of block_hood_set
     uint32_t F[kBlockReplayF];
     uint32_t FL[kBlockReplayF * (S / 4)];   // F's slots during the replay
     uint32_t FO[kBlockReplayF];             // F's occupancy words during the replay
@@ -1640,7 +1642,6 @@ struct BlockReplayLds {
     uint32_t FS[kBlockReplayF];             // species bytes of F's count offsets
     int32_t FD[kBlockReplayF * kSkewSp];    // the count offsets
     uint32_t W[kBlockReplayW];              // the sources, ascending
-    uint16_t WF[kBlockReplayW];             // F index of each source (kBNoF: none)
     uint32_t T[kBlockReplayT];              // the noted voxels
     uint32_t ext[kEscMax];                  // voxels added to F, ascending
     uint32_t scan[kReplayThreads / 64u];    // per-wave sums of block_offset
@@ -1847,7 +1848,6 @@ __device__ int replay_cluster_block(const StepArgs &a, const SkewTab &k, BlockRe
         for (uint32_t wi = tid; wi < nW; wi += nth) {
             const uint32_t w = L.W[wi];
             const int wF = find_sorted(L.F, nF, w);
-            L.WF[wi] = uint16_t(wF >= 0 ? uint32_t(wF) : kBNoF);
             const Slots<S> sw = load_slots<S>(a.src, w);
             // the walk: occupancy word at the step's start, with the reference's u32 end index
             const uint32_t occw = wF >= 0 ? L.FW[wF] : count_nonzero(sw);
@@ -1874,9 +1874,10 @@ __device__ int replay_cluster_block(const StepArgs &a, const SkewTab &k, BlockRe
                 if (tF < 0 && wF < 0)
                     continue;  // neither end in F: the gather has it
                 const uint32_t idx = atomicAdd(&L.n_ev, 1u);
-                if (idx < kBlockReplayEv)
+                if (idx < kBlockReplayEv)  // species | F index of the target << 8 | of the source << 20
                     L.ev[idx] = (uint64_t((wi << 8) | (s << 3) | d) << 32) |
-                                (sp | (tF >= 0 ? uint32_t(tF) : kBNoF) << 8);
+                                (sp | (tF >= 0 ? uint32_t(tF) : kBEvNoF) << 8 |
+                                 (wF >= 0 ? uint32_t(wF) : kBEvNoF) << 20);
             }
         }
         __syncthreads();
@@ -1891,12 +1892,17 @@ __device__ int replay_cluster_block(const StepArgs &a, const SkewTab &k, BlockRe
             failed = alias = 0;
             uint32_t esc = 0xFFFFFFFFu;
             uint8_t *FLb = reinterpret_cast<uint8_t *>(L.FL);
+            // each event carries both F indices, and the next event is read while this one runs: the state reads
+            // of an event are then its only round trips to LDS
+            uint64_t e_next = n_ev != 0u ? L.ev[0] : 0ull;
             for (uint32_t i = 0; i < n_ev; ++i) {
-                const uint64_t e = L.ev[i];
+                const uint64_t e = e_next;
+                if (i + 1u < n_ev)
+                    e_next = L.ev[i + 1u];
                 const uint32_t key = uint32_t(e >> 32), lo = uint32_t(e);
                 const uint32_t wi = key >> 8, s = (key >> 3) & 31u, d = key & 7u;
-                const uint32_t sp = lo & 0xFFu, tF = lo >> 8;
-                const uint32_t wF = L.WF[wi];
+                const uint32_t sp = lo & 0xFFu, tF0 = (lo >> 8) & 0xFFFu, wF0 = lo >> 20;
+                const uint32_t tF = tF0 == kBEvNoF ? kBNoF : tF0, wF = wF0 == kBEvNoF ? kBNoF : wF0;
                 if (tF != kBNoF) {
                     const uint32_t old = L.FO[tF];
                     L.FO[tF] = old + 1u;
