@@ -3194,6 +3194,10 @@ __device__ __forceinline__ uint8_t *uniform_ptr(uint8_t *p)
 // plane of a lattice the wave kernels take (R1, R2 <= 4096, S <= 16: below 2^28 bytes). The plane's offset goes
 // into the base: passed as the scalar offset instead, one store of the gene-expression parity case was lost.
 constexpr int kBufferRsrcWord3 = 0x00020000;
+// The row stores are non-temporal (cache policy bit 1, `nt`): no step reads an output line again before the lattice
+// has streamed through L2, and lines kept there would evict the halo lines that neighbouring tiles read shortly after
+// their owners (256^3: 3% fewer bytes read, 192.5 -> 189.9 us).
+constexpr int kStoreNT = 2;
 constexpr uint32_t kBufferDrop = 0x80000000u;
 typedef uint32_t u32x2_t __attribute__((ext_vector_type(2)));
 typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
@@ -3204,12 +3208,12 @@ __device__ __forceinline__ void buffer_store_vec(const typename SlotVec<S>::T &v
 {
     if constexpr (S == 16) {
         const u32x4_t d = {v.x, v.y, v.z, v.w};
-        __builtin_amdgcn_raw_buffer_store_b128(d, rs, int(off), 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b128(d, rs, int(off), 0, kStoreNT);
     } else if constexpr (S == 8) {
         const u32x2_t d = {v.x, v.y};
-        __builtin_amdgcn_raw_buffer_store_b64(d, rs, int(off), 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b64(d, rs, int(off), 0, kStoreNT);
     } else {
-        __builtin_amdgcn_raw_buffer_store_b32(v, rs, int(off), 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b32(v, rs, int(off), 0, kStoreNT);
     }
 }
 
