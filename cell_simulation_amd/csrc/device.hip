@@ -3185,7 +3185,8 @@ __device__ __forceinline__ void full_records(const uint8_t *src, uint32_t v, uin
 __device__ __forceinline__ uint8_t *uniform_ptr(uint8_t *p)
 {
     const uint64_t v = reinterpret_cast<uint64_t>(p);
-    const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(v)), hi = __builtin_amdgcn_readfirstlane(uint32_t(v >> 32));
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(v));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(uint32_t(v >> 32));
     return reinterpret_cast<uint8_t *>((uint64_t(hi) << 32) | lo);
 }
 
@@ -3194,11 +3195,11 @@ __device__ __forceinline__ uint8_t *uniform_ptr(uint8_t *p)
 // plane of a lattice the wave kernels take (R1, R2 <= 4096, S <= 16: below 2^28 bytes). The plane's offset goes
 // into the base: passed as the scalar offset instead, one store of the gene-expression parity case was lost.
 constexpr int kBufferRsrcWord3 = 0x00020000;
+constexpr uint32_t kBufferDrop = 0x80000000u;
 // The row stores are non-temporal (cache policy bit 1, `nt`): no step reads an output line again before the lattice
 // has streamed through L2, and lines kept there would evict the halo lines that neighbouring tiles read shortly after
 // their owners (256^3: 3% fewer bytes read, 192.5 -> 189.9 us).
 constexpr int kStoreNT = 2;
-constexpr uint32_t kBufferDrop = 0x80000000u;
 typedef uint32_t u32x2_t __attribute__((ext_vector_type(2)));
 typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
 
