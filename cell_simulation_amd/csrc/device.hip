@@ -31,6 +31,7 @@
 #include "device.hpp"
 #include "ensemble.hpp"
 
+#include <atomic>
 #include <mutex>
 
 // Bins of cs_sim_read_region_counts_noalloc's histogram: one fixed array per GPU, so that entry point allocates
@@ -4981,6 +4982,15 @@ static uint64_t count_reservoir_voxels(const Model &m)
 }
 
 // Each device's copy of `scratch`, looked up once per device with that device current.
+// Simulations alive on each HIP device. The work planner cuts a small lattice into shorter items only when its
+// simulation has the device to itself: beside others (an application stepping many cells, each on its own stream)
+// the idle blocks are theirs, and the shorter items' extra side planes only add work.
+static std::atomic<int> &live_sims(int dev)
+{
+    static std::atomic<int> n[256];
+    return n[dev & 255];
+}
+
 static int32_t *noalloc_bins(int dev)
 {
     static std::mutex mu;
@@ -5210,10 +5220,14 @@ Device::Device(const Model &m, int hip_device) : dev_(hip_device)
     hip_check(hipMemsetAsync(counters_, 0, kNumCounters * sizeof(unsigned long long), st), "clear counters");
     hip_check(hipStreamSynchronize(st), "upload");
     cur_ = 0;
+    live_sims(dev_)++;
+    counted_ = true;
 }
 
 Device::~Device()
 {
+    if (counted_)
+        live_sims(dev_)--;
     hipSetDevice(dev_);
     if (stream_)
         hipStreamSynchronize(static_cast<hipStream_t>(stream_));
@@ -5628,10 +5642,12 @@ void Device::plan_balanced_items(const std::vector<uint64_t> &cnt, double plane_
     for (uint64_t v : cnt)
         particles += v;
     uint32_t min_default = particles == 0u ? 12u : 6u;
-    // a small lattice, whose 6-plane items would occupy under a quarter of the resident blocks, takes 2-plane items
-    // instead: its step is then the time of a few plane passes on mostly idle CUs, and shorter marches end sooner
-    // (a 64^3 cell: 45 -> 29 us per step; a 128^3 cell keeps 6, 0.376 against 0.406 ms with 2)
-    if (particles != 0u && uint64_t(ncol) * (R0 / 6u) * 4u < uint64_t(std::max(wave_blocks_, 1u)))
+    // a small lattice alone on its device, whose 6-plane items would occupy under a quarter of the resident blocks,
+    // takes 2-plane items instead: its step is then the time of a few plane passes on mostly idle CUs, and shorter
+    // marches end sooner (a 64^3 cell: 45 -> 29 us per step; a 128^3 cell keeps 6, 0.376 against 0.406 ms with 2;
+    // 512 cells of 32^3..96^3 stepped side by side, each on its own stream: 6.7 -> 8.4 ms per step with 2)
+    const bool alone = live_sims(dev_).load() == (counted_ ? 1 : 0);
+    if (particles != 0u && alone && uint64_t(ncol) * (R0 / 6u) * 4u < uint64_t(std::max(wave_blocks_, 1u)))
         min_default = 2u;
     const uint32_t min_len = std::max(1u, std::min<uint32_t>(min_planes_set_ ? xc_min_ : min_default, R0));
     const uint32_t max_per_col = std::max(1u, R0 / min_len);

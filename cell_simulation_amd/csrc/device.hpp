@@ -99,6 +99,7 @@ private:
     uint32_t items_per_block_ = 0; // wave-kernel work items per resident block (planning target; 0: the planner's
                                    // default, 1 balanced, 8 cost-ordered)
     bool balanced_plan_ = true;    // items of equal cost, one per resident block (else cost-ordered, shortest 12)
+    bool counted_ = false;         // counted among the simulations alive on dev_
     bool min_planes_set_ = false;  // xc_min_ given by the environment
     uint32_t tiled_xc_ = 16;       // planes marched per block by the block-tiled kernel
 
