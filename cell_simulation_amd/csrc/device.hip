@@ -4284,7 +4284,7 @@ struct EnsArgs {
     const uint32_t *live;         // skewed voxels (bit per voxel of every cell)
     const uint32_t *skew_n;       // entries of the skewed-voxel table
     uint8_t *sum_src, *sum_dst;   // occupancy summaries of src and dst (every cell's, at EnsCell::sum_off)
-    const uint32_t *item_cell;    // [total_items] the cell of each work item
+    const uint32_t *item_cell;    // [2 * total_items] the cell of each queue entry and its item within the cell
 };
 
 __device__ __forceinline__ SumRef cell_sum(uint8_t *bytes, const EnsCell &c)
@@ -4411,13 +4411,13 @@ __global__ __launch_bounds__(kWaveBlock) __attribute__((amdgpu_waves_per_eu(3)))
         rr += gridDim.x;
         const uint32_t last = min(first + per, e.total_items);
         for (uint32_t item = first; item < last; ++item) {
-            const uint32_t lo = __builtin_amdgcn_readfirstlane(e.item_cell[item]);
+            const uint32_t lo = __builtin_amdgcn_readfirstlane(e.item_cell[2u * item]);
+            const uint32_t local = __builtin_amdgcn_readfirstlane(e.item_cell[2u * item + 1u]);
             const EnsCell &c = e.cells[lo];
             StepArgs a = cell_step_args(e, c, e.itime[lo]);
             if (*e.skew_n == 0u)
                 a.live = nullptr;  // no skewed voxel: the live-bit loads are skipped
             const WaveArgs w = cell_wave_args(e, c);
-            const uint32_t local = item - c.first_item;
             const uint32_t tz = local % c.tiles_z, by = (local / c.tiles_z) % c.btiles_y;
             const uint32_t chunk = local / (c.tiles_z * c.btiles_y);
             const uint32_t wy = by * kWavesPerBlock + wv;
@@ -6070,10 +6070,29 @@ void Ensemble::build_descriptors()
     if (items >= (uint64_t(1) << 31))
         throw Error(CS_ERR_UNSUPPORTED, "too many work items");
     total_items_ = uint32_t(items);
-    std::vector<uint32_t> item_cell(std::max<uint64_t>(items, 1), 0u);
+    // the queue holds the full-length items first and the shorter last chunks of the cells behind them (each
+    // group in cell order), so that the items still running when the queue empties are the short ones
+    std::vector<uint32_t> order(items);
     for (uint32_t c = 0; c < n; ++c)
-        std::fill(item_cell.begin() + d[c].first_item,
-                  item_cell.begin() + d[c].first_item + size_t(d[c].tiles_z) * d[c].btiles_y * d[c].chunks, c);
+        for (uint32_t i = 0; i < d[c].tiles_z * d[c].btiles_y * d[c].chunks; ++i)
+            order[d[c].first_item + i] = c;
+    auto item_len = [&](uint32_t c, uint32_t local) {
+        const uint32_t chunk = local / (d[c].tiles_z * d[c].btiles_y);
+        return std::min(tiling_.xc, d[c].R0 - chunk * tiling_.xc);
+    };
+    std::vector<uint32_t> item_cell(2 * std::max<uint64_t>(items, 1), 0u);
+    {
+        std::vector<uint32_t> idx(items);
+        for (uint32_t i = 0; i < uint32_t(items); ++i)
+            idx[i] = i;
+        std::stable_sort(idx.begin(), idx.end(), [&](uint32_t p, uint32_t q) {
+            return item_len(order[p], p - d[order[p]].first_item) > item_len(order[q], q - d[order[q]].first_item);
+        });
+        for (uint32_t i = 0; i < uint32_t(items); ++i) {
+            item_cell[2 * i] = order[idx[i]];
+            item_cell[2 * i + 1] = idx[i] - d[order[idx[i]]].first_item;
+        }
+    }
     hipStream_t st = static_cast<hipStream_t>(stream_);
     if (item_cell_)
         hip_check(hipFree(item_cell_), "hipFree item cells");

@@ -88,7 +88,7 @@ private:
     uint8_t *lat_[2] = {nullptr, nullptr};
     uint8_t *sum_[2] = {nullptr, nullptr};   // occupancy summaries of lat_[0], lat_[1] (every cell's)
     uint64_t sum_total_ = 0;                 // bytes per summary
-    uint32_t *item_cell_ = nullptr;          // [total items] the cell of each work item (rebuilt with the tiling)
+    uint32_t *item_cell_ = nullptr;          // [2 * total items] cell and item within it per queue entry (rebuilt with the tiling)
     uint8_t *reg_ = nullptr;
     void *regp_ = nullptr;             // padded labels of every cell (the wave kernel's label windows)
     uint32_t *tab_ = nullptr;          // all cells' table blobs
