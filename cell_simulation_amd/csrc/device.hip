@@ -4256,7 +4256,7 @@ struct EnsCell {
     uint64_t lat_off;          // byte offset of the cell's slots in each lattice buffer
     uint32_t vox_off;          // index of the cell's voxel 0 among all voxels (labels, tight list)
     uint32_t R0, R1, R2;
-    uint32_t tiles_z, wtiles_y, btiles_y, chunks, first_item;  // wave tiles in z and y, groups in y, x-chunks
+    uint32_t tiles_z, wtiles_y, btiles_y, chunks, first_item;  // wave tiles in z and y, groups per x-chunk, x-chunks
     uint32_t regp_off, PY, PZ4;  // the cell's padded labels: word offset, rows per plane, words per row
     uint32_t tab_off, tab_words;  // the cell's table blob, in 32-bit words of the tables buffer
     uint32_t off_p2, off_rates, off_ridx, off_stoich, off_rmask, off_bound, pair_n1;
@@ -4418,12 +4418,14 @@ __global__ __launch_bounds__(kWaveBlock) __attribute__((amdgpu_waves_per_eu(3)))
             if (*e.skew_n == 0u)
                 a.live = nullptr;  // no skewed voxel: the live-bit loads are skipped
             const WaveArgs w = cell_wave_args(e, c);
-            const uint32_t tz = local % c.tiles_z, by = (local / c.tiles_z) % c.btiles_y;
-            const uint32_t chunk = local / (c.tiles_z * c.btiles_y);
-            const uint32_t wy = by * kWavesPerBlock + wv;
+            // the group's waves take consecutive wave tiles of the cell, y fastest (a column of tiles, then the
+            // next column), so that only the last group of an x-chunk may have waves without a tile
+            const uint32_t chunk = local / c.btiles_y;
+            const uint32_t t = (local % c.btiles_y) * kWavesPerBlock + wv;
+            const uint32_t tz = t / c.wtiles_y, wy = t % c.wtiles_y;
             const int xb = int(chunk * e.xc), xe = min(int(c.R0), int((chunk + 1u) * e.xc));
             // a wave whose march has nothing to do stops here; a group of such waves skips the table staging
-            bool busy = wy < c.wtiles_y;
+            bool busy = tz < c.tiles_z;
             if (busy && w.sum_src != nullptr && c.has_zero_order == 0u && xe - xb <= 60) {
                 const MarchSummary ms = march_summary(a, w, tz, wy, xb, xe);
                 busy = (ms.act | ms.old) != 0ull;
@@ -6062,10 +6064,10 @@ void Ensemble::build_descriptors()
     for (uint32_t c = 0; c < n; ++c) {
         d[c].tiles_z = (d[c].R2 + kWZ - 1) / kWZ;
         d[c].wtiles_y = (d[c].R1 + kWY - 1) / kWY;
-        d[c].btiles_y = (d[c].wtiles_y + kWavesPerBlock - 1) / kWavesPerBlock;
+        d[c].btiles_y = (d[c].tiles_z * d[c].wtiles_y + kWavesPerBlock - 1) / kWavesPerBlock;
         d[c].chunks = (d[c].R0 + tiling_.xc - 1) / tiling_.xc;
         d[c].first_item = uint32_t(items);
-        items += uint64_t(d[c].tiles_z) * d[c].btiles_y * d[c].chunks;
+        items += uint64_t(d[c].btiles_y) * d[c].chunks;
     }
     if (items >= (uint64_t(1) << 31))
         throw Error(CS_ERR_UNSUPPORTED, "too many work items");
@@ -6074,10 +6076,10 @@ void Ensemble::build_descriptors()
     // group in cell order), so that the items still running when the queue empties are the short ones
     std::vector<uint32_t> order(items);
     for (uint32_t c = 0; c < n; ++c)
-        for (uint32_t i = 0; i < d[c].tiles_z * d[c].btiles_y * d[c].chunks; ++i)
+        for (uint32_t i = 0; i < d[c].btiles_y * d[c].chunks; ++i)
             order[d[c].first_item + i] = c;
     auto item_len = [&](uint32_t c, uint32_t local) {
-        const uint32_t chunk = local / (d[c].tiles_z * d[c].btiles_y);
+        const uint32_t chunk = local / d[c].btiles_y;
         return std::min(tiling_.xc, d[c].R0 - chunk * tiling_.xc);
     };
     std::vector<uint32_t> item_cell(2 * std::max<uint64_t>(items, 1), 0u);
