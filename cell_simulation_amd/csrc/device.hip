@@ -3474,6 +3474,13 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
     if (xrow_ok)
         xm = xb - 1 >= 0 ? uint32_t(xb - 1) % uint32_t(xrow) : uint32_t(xrow - 1);
     const uint32_t xm_zero = a.mod_zero;
+    // the label planes xb-2 and xb-1, loaded before the summaries' loads so that one wait covers both
+    uint32_t lab0[kWLabLane], lab1[kWLabLane];
+#pragma unroll
+    for (int i = 0; i < kWLabLane; ++i) {
+        lab0[i] = w.regp[lab_base(xb - 2) + loff[i]];
+        lab1[i] = w.regp[lab_base(xb - 1) + loff[i]];
+    }
 
     // Occupancy summaries: a plane whose records (this tile's voxels and the four face-neighbour tiles') hold
     // no particle is neither loaded nor listed; a plane gathered empty where the destination was empty
@@ -3509,17 +3516,8 @@ __device__ __forceinline__ void wave_march(WaveLds<S> &L, const Tab &tb, const S
                 dst[l + 64 * i] = r[i];
     };
     WaveStaged<S> st0, st1;
-    {
-        uint32_t r[kWLabLane];
-#pragma unroll
-        for (int i = 0; i < kWLabLane; ++i)
-            r[i] = w.regp[lab_base(xb - 2) + loff[i]];
-        store_lab(xb - 2, r);
-#pragma unroll
-        for (int i = 0; i < kWLabLane; ++i)
-            r[i] = w.regp[lab_base(xb - 1) + loff[i]];
-        store_lab(xb - 1, r);
-    }
+    store_lab(xb - 2, lab0);
+    store_lab(xb - 1, lab1);
 #pragma unroll
     for (int k = 0; k < kWBatches; ++k)  // the side plane xb - 1: interior rows only (the other loads at one line)
         st0.rec[k] = load_head<S>(a.src, k < kWY ? slot_base(xb - 1) + off[k] : (off[k] & 7u));
